@@ -1,0 +1,122 @@
+/*
+ * oracle.h -- CPU restatement of scl-core's graph-diffusion path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or the timed CPU baseline.  The product (libscl_f32_i64.so) never links,
+ * loads or calls it.
+ *
+ * Two families of functions:
+ *   orc_*_f32   restate the reference's FP32 algorithm step by step (same
+ *               loop structure, same per-row accumulation order, FMA where the
+ *               reference's Highway MulAdd / contracted "+= a*b" produce one).
+ *   orc_*_f64   an FP64 oracle of the same mathematics (double accumulation),
+ *               used to state errors; it is not a restatement of any code.
+ *
+ * Index arrays are int64 as at the reference ABI (scl_index_t = int64_t).
+ * Reference citations are relative to the scl-core source tree.
+ *
+ * Parity status of each function is recorded next to it; every function is
+ * pinned by a test in tests/test_oracle.py (closed forms, reference known
+ * answers, invariants, brute force).
+ */
+#ifndef SCL_ORACLE_H
+#define SCL_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One step Y = P X, following scl/kernel/diffusion.hpp:227-278 (spmm_block):
+ * zero Y, then for each row i and each stored nonzero k in stored order,
+ * Y[i,:] = fma(v_k, X[j_k,:], Y[i,:]) (axpy_simd, diffusion.hpp:100-134).
+ * OpenMP static over rows when n >= 256 (diffusion.hpp:240); results do not
+ * depend on the thread count. */
+void orc_spmm_step_f32(int64_t n, int64_t F, const int64_t* indptr,
+                       const int64_t* indices, const float* vals,
+                       const float* X, float* Y);
+
+/* X <- P^t X in place, following diffusion.hpp:638-664 (diffuse_matrix):
+ * value-initialised n*F temporary, t x { spmm_block; memcpy back }.
+ * Returns 0, or -1 when the temporary cannot be allocated. */
+int orc_diffuse_matrix_f32(int64_t n, int64_t F, int64_t t,
+                           const int64_t* indptr, const int64_t* indices,
+                           const float* vals, float* X);
+
+/* Column subset of orc_diffuse_matrix_f32: X_sub is n x n_sub (row-major) and
+ * holds columns of the full X; since columns never mix, the result equals the
+ * same columns of the full computation bit for bit. */
+int orc_diffuse_columns_f32(int64_t n, int64_t n_sub, int64_t t,
+                            const int64_t* indptr, const int64_t* indices,
+                            const float* vals, float* X_sub);
+
+/* x <- P^t x, following diffusion.hpp:609-632 (diffuse_vector) and
+ * spmv_parallel 169-221: sum = fma(v_k, x[j_k], sum) in stored order. */
+int orc_diffuse_vector_f32(int64_t n, int64_t t, const int64_t* indptr,
+                           const int64_t* indices, const float* vals,
+                           float* x);
+
+/* In-place normalisation of values, following diffusion.hpp:492-603
+ * (compute_transition_matrix) with compute_row_sums_parallel 452-484:
+ * FP32 sequential row sums; symmetric == 0: v *= (s > 1e-15f ? 1/s : 0);
+ * symmetric != 0: v *= d_i^-1/2 * d_j^-1/2. */
+void orc_transition_f32(int64_t n, const int64_t* indptr,
+                        const int64_t* indices, float* vals, int symmetric);
+
+/* FP64 normalisation (what the device computes): d_i = sum_k (double)a_ik in
+ * stored order; symmetric == 0: v = (float)(a / d_i), 0 when d_i <= 1e-15;
+ * symmetric != 0: v = (float)(a * (1/sqrt(d_i)) * (1/sqrt(d_j))).
+ * Writes d to deg (length n) when deg != NULL. */
+void orc_transition_f64(int64_t n, const int64_t* indptr,
+                        const int64_t* indices, float* vals, int symmetric,
+                        double* deg);
+
+/* FP64 oracle: Y = P^t X with double values/accumulators, X and Y double. */
+int orc_diffuse_f64(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                    const int64_t* indices, const double* vals,
+                    const double* X, double* Y);
+
+/* FP64 oracle of the transposed operator: Y = (P^T)^t X. */
+int orc_diffuse_transpose_f64(int64_t n, int64_t F, int64_t t,
+                              const int64_t* indptr, const int64_t* indices,
+                              const double* vals, const double* X, double* Y);
+
+/* FP64 reference gradient of L = 0.5 ||P^t X - T||^2:
+ *   dX = (P^T)^t (Y - T),   dP_k (k = nonzero (i,j)) = sum_{s=1..t} <G_s[i,:], X_{s-1}[j,:]>
+ * with G_t = Y - T and G_{s-1} = P^T G_s.  Writes Y, dX (n x F) and dP (nnz).
+ * Returns L. */
+double orc_grad_f64(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                    const int64_t* indices, const double* vals,
+                    const double* X, const double* T, double* Y, double* dX,
+                    double* dP);
+
+/* FP64 loss only (for finite differences). */
+double orc_loss_f64(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                    const int64_t* indices, const double* vals,
+                    const double* X, const double* T);
+
+/* FP64 stationarity residual r = || P^T pi - pi ||_1 with pi = deg / sum(deg),
+ * P given in FP32 values; counterpart of scl/kernel/transition.hpp:398-416
+ * (sparse_matvec_transpose) used by stationary_distribution 571-627. */
+double orc_stationarity_residual_f64(int64_t n, const int64_t* indptr,
+                                     const int64_t* indices, const float* vals,
+                                     const double* deg);
+
+/* Relative error in the reference tests' convention
+ * (tests/c_api/include/precision.hpp:102-120): max|a-b| / max(max|a|,max|b|),
+ * or max|a-b| when both are below 1e-15. a is float, b is double. */
+double orc_rel_err_f32_f64(size_t len, const float* a, const double* b);
+double orc_rel_err_f32_f32(size_t len, const float* a, const float* b);
+/* Median over elements of |a-b| / max|b|. */
+double orc_median_rel_err_f32_f64(size_t len, const float* a, const double* b);
+
+/* Thread count the restatement runs with (omp_get_max_threads). */
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""ctypes loader of the CPU restatement / FP64 oracle (oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product never imports it.
+
+The AVX-512 build (liboracle_v4.so) is used when the host CPU has the
+x86-64-v4 features, else the AVX2 build; both give identical results.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_lib = None
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+D = ctypes.c_double
+SZ = ctypes.c_size_t
+
+
+def _has_v4() -> bool:
+    try:
+        flags = set()
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+        return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        return False
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    name = "liboracle_v4.so" if _has_v4() else "liboracle_v3.so"
+    path = HERE / name
+    if not path.exists():
+        raise ImportError(f"{path} is missing: run `make -C oracle`")
+    lib = ctypes.CDLL(os.fspath(path))
+    sig = {
+        "orc_spmm_step_f32": (None, [I64, I64, P, P, P, P, P]),
+        "orc_diffuse_matrix_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, P]),
+        "orc_diffuse_columns_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, P]),
+        "orc_diffuse_vector_f32": (ctypes.c_int, [I64, I64, P, P, P, P]),
+        "orc_transition_f32": (None, [I64, P, P, P, ctypes.c_int]),
+        "orc_transition_f64": (None, [I64, P, P, P, ctypes.c_int, P]),
+        "orc_diffuse_f64": (ctypes.c_int, [I64, I64, I64, P, P, P, P, P]),
+        "orc_diffuse_transpose_f64": (ctypes.c_int, [I64, I64, I64, P, P, P, P, P]),
+        "orc_grad_f64": (D, [I64, I64, I64, P, P, P, P, P, P, P, P]),
+        "orc_loss_f64": (D, [I64, I64, I64, P, P, P, P, P]),
+        "orc_stationarity_residual_f64": (D, [I64, P, P, P, P]),
+        "orc_rel_err_f32_f64": (D, [SZ, P, P]),
+        "orc_rel_err_f32_f32": (D, [SZ, P, P]),
+        "orc_median_rel_err_f32_f64": (D, [SZ, P, P]),
+        "orc_num_threads": (ctypes.c_int, []),
+    }
+    for k, (res, args) in sig.items():
+        fn = getattr(lib, k)
+        fn.restype = res
+        fn.argtypes = args
+    lib.variant = name
+    _lib = lib
+    return lib
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def spmm_step_f32(indptr, indices, vals, X):
+    X = _f32(X)
+    n, F = X.shape
+    Y = np.empty_like(X)
+    ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    load().orc_spmm_step_f32(n, F, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                             X.ctypes.data, Y.ctypes.data)
+    return Y
+
+
+def diffuse_matrix_f32(indptr, indices, vals, X, t):
+    """Reference-order FP32 P^t X (returns a new array)."""
+    Y = _f32(X).copy()
+    n = Y.shape[0]
+    F = Y.shape[1] if Y.ndim == 2 else 1
+    ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    rc = load().orc_diffuse_matrix_f32(n, F, t, ip.ctypes.data, ix.ctypes.data,
+                                       v.ctypes.data, Y.ctypes.data)
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return Y
+
+
+def diffuse_vector_f32(indptr, indices, vals, x, t):
+    y = _f32(x).copy()
+    ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    rc = load().orc_diffuse_vector_f32(y.shape[0], t, ip.ctypes.data, ix.ctypes.data,
+                                       v.ctypes.data, y.ctypes.data)
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return y
+
+
+def transition_f32(indptr, indices, vals, symmetric=False):
+    v = _f32(vals).copy()
+    ip, ix = _i64(indptr), _i64(indices)
+    load().orc_transition_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data,
+                              v.ctypes.data, 1 if symmetric else 0)
+    return v
+
+
+def transition_f64(indptr, indices, vals, symmetric=False):
+    v = _f32(vals).copy()
+    ip, ix = _i64(indptr), _i64(indices)
+    deg = np.empty(len(ip) - 1, np.float64)
+    load().orc_transition_f64(len(ip) - 1, ip.ctypes.data, ix.ctypes.data,
+                              v.ctypes.data, 1 if symmetric else 0, deg.ctypes.data)
+    return v, deg
+
+
+def diffuse_f64(indptr, indices, vals, X, t, transpose=False):
+    X = _f64(X)
+    n = X.shape[0]
+    F = X.shape[1] if X.ndim == 2 else 1
+    Y = np.empty_like(X)
+    ip, ix, v = _i64(indptr), _i64(indices), _f64(vals)
+    fn = load().orc_diffuse_transpose_f64 if transpose else load().orc_diffuse_f64
+    rc = fn(n, F, t, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+            X.ctypes.data, Y.ctypes.data)
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return Y
+
+
+def grad_f64(indptr, indices, vals, X, T, t):
+    """Returns (L, Y, dX, dP) of L = 0.5 ||P^t X - T||^2 in FP64."""
+    X, T = _f64(X), _f64(T)
+    n, F = X.shape
+    ip, ix, v = _i64(indptr), _i64(indices), _f64(vals)
+    Y, dX = np.empty_like(X), np.empty_like(X)
+    dP = np.empty(max(len(ix), 1), np.float64)
+    L = load().orc_grad_f64(n, F, t, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                            X.ctypes.data, T.ctypes.data, Y.ctypes.data,
+                            dX.ctypes.data, dP.ctypes.data)
+    return L, Y, dX, dP[:len(ix)]
+
+
+def loss_f64(indptr, indices, vals, X, T, t):
+    X, T = _f64(X), _f64(T)
+    n, F = X.shape
+    ip, ix, v = _i64(indptr), _i64(indices), _f64(vals)
+    return load().orc_loss_f64(n, F, t, ip.ctypes.data, ix.ctypes.data,
+                               v.ctypes.data, X.ctypes.data, T.ctypes.data)
+
+
+def stationarity_residual_f64(indptr, indices, vals, deg):
+    ip, ix, v, d = _i64(indptr), _i64(indices), _f32(vals), _f64(deg)
+    return load().orc_stationarity_residual_f64(len(ip) - 1, ip.ctypes.data,
+                                                ix.ctypes.data, v.ctypes.data,
+                                                d.ctypes.data)
+
+
+def rel_err(a, b) -> float:
+    """max|a-b| / max(max|a|, max|b|) (reference precision.hpp:102-120)."""
+    a = _f32(a).ravel()
+    if np.asarray(b).dtype == np.float64:
+        b = _f64(b).ravel()
+        return load().orc_rel_err_f32_f64(a.size, a.ctypes.data, b.ctypes.data)
+    b = _f32(b).ravel()
+    return load().orc_rel_err_f32_f32(a.size, a.ctypes.data, b.ctypes.data)
+
+
+def median_rel_err(a, b) -> float:
+    a, b = _f32(a).ravel(), _f64(b).ravel()
+    return load().orc_median_rel_err_f32_f64(a.size, a.ctypes.data, b.ctypes.data)
+
+
+def num_threads() -> int:
+    return load().orc_num_threads()

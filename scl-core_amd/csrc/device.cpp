@@ -1,0 +1,181 @@
+// device.cpp -- device runtime of the library: device discovery, the
+// library's streams, grow-only staging buffers, and the device copies of
+// sparse handles (int32 indices, float values, optional transposed pattern).
+#include "internal.hpp"
+
+#include <limits>
+
+namespace scl_amd {
+
+namespace {
+
+struct DeviceState {
+    hipStream_t stream = nullptr;
+    std::mutex call_mu;   // serialises host-pointer entry points
+    std::mutex ws_mu;
+    std::vector<std::pair<void*, size_t>> ws;
+};
+
+std::mutex g_mu;
+// Never destroyed: device memory and streams are reclaimed by the runtime at
+// process exit, and destroying them from a static destructor can run after
+// the HIP runtime itself has been torn down.
+std::map<int, DeviceState*>* g_dev = new std::map<int, DeviceState*>();
+
+DeviceState& state(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_dev->find(device);
+    if (it != g_dev->end()) return *it->second;
+    auto* st = new DeviceState();
+    (*g_dev)[device] = st;
+    return *st;
+}
+
+template <typename T>
+T* dev_alloc(size_t count, const char* what) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(&p, count * sizeof(T)), what);
+    return static_cast<T*>(p);
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace
+
+int current_device() {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        fail(SCL_ERROR_FEATURE_UNAVAILABLE, "No HIP device available");
+    }
+    int d = 0;
+    hip_check(hipGetDevice(&d), "hipGetDevice");
+    return d;
+}
+
+hipStream_t library_stream(int device) {
+    DeviceState& st = state(device);
+    std::lock_guard<std::mutex> lk(st.ws_mu);
+    if (!st.stream)
+        hip_check(hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking),
+                  "hipStreamCreate");
+    return st.stream;
+}
+
+std::mutex& device_mutex(int device) { return state(device).call_mu; }
+
+void* device_workspace(int device, int slot, size_t bytes) {
+    DeviceState& st = state(device);
+    std::lock_guard<std::mutex> lk(st.ws_mu);
+    if ((int)st.ws.size() <= slot) st.ws.resize(slot + 1, {nullptr, 0});
+    auto& w = st.ws[slot];
+    if (w.second < bytes) {
+        dev_free(w.first);
+        w = {nullptr, 0};
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, bytes), "workspace allocation");
+        w = {p, bytes};
+    }
+    return w.first;
+}
+
+DeviceCSR::~DeviceCSR() {
+    int cur = -1;
+    if (device >= 0 && hipGetDevice(&cur) == hipSuccess && cur != device)
+        (void)hipSetDevice(device);
+    dev_free(row_ptr);
+    dev_free(col);
+    dev_free(val);
+    dev_free(sched);
+    release_transpose();
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+}
+
+void DeviceCSR::release_transpose() {
+    dev_free(t_row_ptr);
+    dev_free(t_col);
+    dev_free(t_val);
+    dev_free(t_src);
+    dev_free(t_sched);
+    t_row_ptr = t_col = t_src = t_sched = nullptr;
+    t_val = nullptr;
+    t_generation = ~uint64_t(0);
+}
+
+DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
+    auto& slot = h->dev[device];
+    if (slot && slot->generation == h->generation) return *slot;
+    const int64_t p = h->primary(), q = h->secondary(), nnz = h->nnz;
+    const int64_t lim = std::numeric_limits<int32_t>::max();
+    if (p >= lim || q >= lim || nnz >= lim)
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "operator too large for int32 device indices (dims and nnz must be < 2^31-1)");
+    if (slot && slot->n == p && slot->nnz == nnz) {
+        // same shape: refresh in place
+    } else {
+        slot.reset(new DeviceCSR());
+        slot->device = device;
+        slot->n = (int32_t)p;
+        slot->n_sec = (int32_t)q;
+        slot->nnz = nnz;
+        slot->row_ptr = dev_alloc<int32_t>((size_t)p + 1, "operator row_ptr");
+        slot->col = dev_alloc<int32_t>((size_t)nnz, "operator col");
+        slot->val = dev_alloc<float>((size_t)nnz, "operator val");
+    }
+    std::vector<int32_t> rp((size_t)p + 1), ci((size_t)nnz);
+    for (int64_t i = 0; i <= p; ++i) rp[i] = (int32_t)h->indptr[i];
+    for (int64_t k = 0; k < nnz; ++k) ci[k] = (int32_t)h->indices[k];
+    hip_check(hipMemcpy(slot->row_ptr, rp.data(), rp.size() * 4, hipMemcpyHostToDevice),
+              "upload row_ptr");
+    if (nnz > 0) {
+        hip_check(hipMemcpy(slot->col, ci.data(), ci.size() * 4, hipMemcpyHostToDevice),
+                  "upload col");
+        hip_check(hipMemcpy(slot->val, h->data, (size_t)nnz * 4, hipMemcpyHostToDevice),
+                  "upload val");
+    }
+    slot->release_transpose();
+    slot->generation = h->generation;
+    return *slot;
+}
+
+void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
+    (void)s;
+    if (d.t_row_ptr && d.t_generation == d.generation) return;
+    d.release_transpose();
+    const int64_t p = h->primary(), q = h->secondary(), nnz = h->nnz;
+    // counting sort by column; visiting source rows in order keeps each
+    // transposed row sorted by source row
+    std::vector<int32_t> trp((size_t)q + 1, 0), tcol((size_t)nnz), tsrc((size_t)nnz);
+    std::vector<float> tval((size_t)nnz);
+    for (int64_t k = 0; k < nnz; ++k) trp[h->indices[k] + 1]++;
+    for (int64_t j = 0; j < q; ++j) trp[j + 1] += trp[j];
+    std::vector<int32_t> pos(trp.begin(), trp.end() - 1);
+    for (int64_t i = 0; i < p; ++i)
+        for (int64_t k = h->indptr[i]; k < h->indptr[i + 1]; ++k) {
+            const int32_t dst = pos[h->indices[k]]++;
+            tcol[dst] = (int32_t)i;
+            tval[dst] = h->data[k];
+            tsrc[dst] = (int32_t)k;
+        }
+    d.t_row_ptr = dev_alloc<int32_t>((size_t)q + 1, "transpose row_ptr");
+    d.t_col = dev_alloc<int32_t>((size_t)nnz, "transpose col");
+    d.t_val = dev_alloc<float>((size_t)nnz, "transpose val");
+    d.t_src = dev_alloc<int32_t>((size_t)nnz, "transpose src");
+    hip_check(hipMemcpy(d.t_row_ptr, trp.data(), trp.size() * 4, hipMemcpyHostToDevice),
+              "upload transpose");
+    if (nnz > 0) {
+        hip_check(hipMemcpy(d.t_col, tcol.data(), tcol.size() * 4, hipMemcpyHostToDevice),
+                  "upload transpose");
+        hip_check(hipMemcpy(d.t_val, tval.data(), tval.size() * 4, hipMemcpyHostToDevice),
+                  "upload transpose");
+        hip_check(hipMemcpy(d.t_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice),
+                  "upload transpose");
+    }
+    d.t_generation = d.generation;
+}
+
+}  // namespace scl_amd

@@ -1,0 +1,299 @@
+// diffusion.cpp -- the scl_diffusion_* drop-in entry points and their
+// device-pointer extensions (counterpart of scl/binding/c_api/diffusion.cpp).
+//
+// The reference runs diffuse_matrix (scl/kernel/diffusion.hpp:638-664) as
+// t x { spmm_block into a zeroed temporary; memcpy back }.  Here the t steps
+// ping-pong between two device buffers: no zero fill (every output element
+// is written once per step) and no copy between steps.
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "scl_diffusion.h"
+#include "scl_diffusion_gpu.h"
+
+#include <cstring>
+#include <limits>
+
+using namespace scl_amd;
+
+namespace {
+
+constexpr int64_t kInt32Max = std::numeric_limits<int32_t>::max();
+
+CsrView view_of(const DeviceCSR& d) {
+    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
+}
+
+CsrView transposed_view_of(const DeviceCSR& d) {
+    return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched};
+}
+
+// Checks the operator against the dense operand: the reference never checks
+// n_nodes against the operator (diffusion.cpp:94-116) and reads out of
+// bounds on a mismatch; here it is SCL_ERROR_DIMENSION_MISMATCH.
+void check_operator(const scl_sparse_matrix* P, int64_t n_nodes, int64_t F) {
+    if (P->rows != P->cols)
+        fail(SCL_ERROR_DIMENSION_MISMATCH, "Transition matrix must be square");
+    if (P->rows != n_nodes)
+        fail(SCL_ERROR_DIMENSION_MISMATCH,
+             "n_nodes does not match the transition matrix dimensions");
+    if (F > 0 && n_nodes > std::numeric_limits<int64_t>::max() / F / 4)
+        fail(SCL_ERROR_INVALID_ARGUMENT, "n_nodes * n_features overflows");
+    if (F > kInt32Max)
+        fail(SCL_ERROR_INVALID_ARGUMENT, "n_features too large");
+}
+
+// t steps between two device buffers, starting from buf[start]; returns the
+// index of the buffer holding the result.
+int pingpong(const CsrView& P, float* buf[2], int start, int64_t F, int64_t t,
+             hipStream_t s) {
+    const SpmmShape shape = spmm_default_shape(F, buf[0], buf[1]);
+    int cur = start;
+    for (int64_t step = 0; step < t; ++step) {
+        launch_spmm_f32(P, buf[cur], buf[cur ^ 1], F, shape, s);
+        cur ^= 1;
+    }
+    return cur;
+}
+
+// Host-pointer diffusion: upload X, t steps on the device, download.
+void diffuse_host(scl_sparse_matrix* P, float* X, int64_t n, int64_t F,
+                  int64_t t) {
+    const int dev = current_device();
+    std::lock_guard<std::mutex> call_lk(device_mutex(dev));
+    std::lock_guard<std::mutex> lk(P->mu);
+    // the caller may have edited a wrapped handle's arrays since last time
+    if (P->wrapped) P->generation++;
+    DeviceCSR& d = device_csr(P, dev);
+    hipStream_t s = library_stream(dev);
+    const size_t bytes = (size_t)n * (size_t)F * sizeof(float);
+    float* buf[2] = {static_cast<float*>(device_workspace(dev, 0, bytes)),
+                     static_cast<float*>(device_workspace(dev, 1, bytes))};
+    hip_check(hipMemcpyAsync(buf[0], X, bytes, hipMemcpyHostToDevice, s), "upload X");
+    const int fin = pingpong(view_of(d), buf, 0, F, t, s);
+    hip_check(hipMemcpyAsync(X, buf[fin], bytes, hipMemcpyDeviceToHost, s), "download X");
+    hip_check(hipStreamSynchronize(s), "diffusion");
+}
+
+void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
+                     double* residual_out) {
+    if (symmetric && A->rows != A->cols)
+        fail(SCL_ERROR_DIMENSION_MISMATCH,
+             "Symmetric normalisation needs a square matrix");
+    const int dev = current_device();
+    std::lock_guard<std::mutex> call_lk(device_mutex(dev));
+    std::lock_guard<std::mutex> lk(A->mu);
+    if (A->wrapped) A->generation++;
+    DeviceCSR& d = device_csr(A, dev);
+    hipStream_t s = library_stream(dev);
+    const int32_t n = d.n;
+    // degrees (n doubles) + reduction scratch (1025 doubles)
+    double* deg = static_cast<double*>(
+        device_workspace(dev, 2, sizeof(double) * ((size_t)n + 2048)));
+    double* scratch = deg + n;
+    launch_row_sums_f64(d.row_ptr, d.val, n, deg, s);
+    launch_normalize_rows(d.row_ptr, d.col, d.val, n, deg, symmetric, s);
+    if (A->nnz > 0)
+        hip_check(hipMemcpyAsync(A->data, d.val, (size_t)A->nnz * 4,
+                                 hipMemcpyDeviceToHost, s),
+                  "download values");
+    if (degrees_out)
+        hip_check(hipMemcpyAsync(degrees_out, deg, sizeof(double) * (size_t)n,
+                                 hipMemcpyDeviceToHost, s),
+                  "download degrees");
+    hip_check(hipStreamSynchronize(s), "normalisation");
+    // the device copy now holds the new values: advance both generations
+    A->generation++;
+    d.generation = A->generation;
+    d.release_transpose();
+    if (residual_out) {
+        ensure_transpose(A, d, s);
+        double* out = scratch + 1100;
+        launch_stationarity_residual(d.t_row_ptr, d.t_col, d.t_val, d.n_sec, deg,
+                                     scratch, out, s);
+        hip_check(hipMemcpyAsync(residual_out, out, sizeof(double),
+                                 hipMemcpyDeviceToHost, s),
+                  "download residual");
+        hip_check(hipStreamSynchronize(s), "residual");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// diffusion.cpp:36-62
+SCL_EXPORT scl_error_t scl_diffusion_compute_transition_matrix(
+    scl_sparse_t adjacency, int symmetric) {
+    SCL_API_CHECK_NULL(adjacency, "Adjacency matrix is null");
+    SCL_API_TRY
+        transition_impl(adjacency, symmetric, nullptr, nullptr);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_compute_transition_matrix_ex(
+    scl_sparse_t adjacency, int symmetric, double* degrees_out,
+    double* residual_out) {
+    SCL_API_CHECK_NULL(adjacency, "Adjacency matrix is null");
+    SCL_API_TRY
+        transition_impl(adjacency, symmetric, degrees_out, residual_out);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+// diffusion.cpp:68-88
+SCL_EXPORT scl_error_t scl_diffusion_diffuse_vector(scl_sparse_t transition,
+                                                    scl_real_t* x,
+                                                    scl_size_t n_nodes,
+                                                    scl_index_t n_steps) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(x, "Vector x is null");
+    SCL_API_CHECK(n_nodes > 0 && n_steps > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    SCL_API_TRY
+        check_operator(transition, (int64_t)n_nodes, 1);
+        diffuse_host(transition, x, (int64_t)n_nodes, 1, n_steps);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+// diffusion.cpp:94-116
+SCL_EXPORT scl_error_t scl_diffusion_diffuse_matrix(scl_sparse_t transition,
+                                                    scl_real_t* X,
+                                                    scl_index_t n_nodes,
+                                                    scl_index_t n_features,
+                                                    scl_index_t n_steps) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(X, "Matrix X is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0 && n_steps > 0,
+                  SCL_ERROR_INVALID_ARGUMENT, "Dimensions must be positive");
+    SCL_API_TRY
+        check_operator(transition, n_nodes, n_features);
+        diffuse_host(transition, X, n_nodes, n_features, n_steps);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+// diffusion.cpp:318-341
+SCL_EXPORT scl_error_t scl_diffusion_magic_impute(scl_sparse_t transition,
+                                                  scl_real_t* X,
+                                                  scl_index_t n_nodes,
+                                                  scl_index_t n_features,
+                                                  scl_index_t t) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(X, "Matrix X is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    SCL_API_TRY
+        check_operator(transition, n_nodes, n_features);
+        const int64_t steps = (t == 0) ? 3 : t;  // diffusion.cpp:333
+        if (steps > 0) diffuse_host(transition, X, n_nodes, n_features, steps);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_device_count(int* out) {
+    SCL_API_CHECK_NULL(out, "Output pointer is null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    *out = c;
+    SCL_API_RETURN_OK;
+}
+
+SCL_EXPORT scl_error_t scl_gpu_sparse_invalidate(scl_sparse_t matrix) {
+    SCL_API_CHECK_NULL(matrix, "Matrix is null");
+    SCL_API_TRY
+        std::lock_guard<std::mutex> lk(matrix->mu);
+        matrix->generation++;
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_sparse_prepare(scl_sparse_t matrix) {
+    SCL_API_CHECK_NULL(matrix, "Matrix is null");
+    SCL_API_TRY
+        const int dev = current_device();
+        std::lock_guard<std::mutex> lk(matrix->mu);
+        device_csr(matrix, dev);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
+    scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
+    scl_real_t* work_dev, scl_index_t n_nodes, scl_index_t n_features,
+    scl_index_t n_steps, void* stream) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(X_dev, "X is null");
+    SCL_API_CHECK_NULL(Y_dev, "Y is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0 && n_steps > 0,
+                  SCL_ERROR_INVALID_ARGUMENT, "Dimensions must be positive");
+    SCL_API_CHECK(n_steps == 1 || work_dev != nullptr, SCL_ERROR_NULL_POINTER,
+                  "work buffer is required for n_steps >= 2");
+    SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev &&
+                      (const void*)X_dev != (const void*)work_dev &&
+                      (const void*)Y_dev != (const void*)work_dev,
+                  SCL_ERROR_INVALID_ARGUMENT, "X, Y and work must be distinct");
+    SCL_API_TRY
+        check_operator(transition, n_nodes, n_features);
+        const int dev = current_device();
+        CsrView v;
+        {
+            std::lock_guard<std::mutex> lk(transition->mu);
+            v = view_of(device_csr(transition, dev));
+        }
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const SpmmShape shape = spmm_default_shape(n_features, X_dev, Y_dev);
+        const SpmmShape wshape =
+            work_dev ? spmm_default_shape(n_features, work_dev, Y_dev) : shape;
+        const SpmmShape sh = (wshape.vec < shape.vec) ? wshape : shape;
+        // step s (1-based) writes Y when t - s is even, else work, so the
+        // last step lands in Y and consecutive steps alternate buffers
+        const float* src = X_dev;
+        for (int64_t step = 1; step <= n_steps; ++step) {
+            float* dst = ((n_steps - step) % 2 == 0) ? Y_dev : work_dev;
+            launch_spmm_f32(v, src, dst, n_features, sh, s);
+            src = dst;
+        }
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,
+                                           const scl_real_t* X_dev,
+                                           scl_real_t* Y_dev,
+                                           scl_index_t n_nodes,
+                                           scl_index_t n_features,
+                                           int transpose, void* stream) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(X_dev, "X is null");
+    SCL_API_CHECK_NULL(Y_dev, "Y is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev,
+                  SCL_ERROR_INVALID_ARGUMENT, "X and Y must be distinct");
+    SCL_API_TRY
+        check_operator(transition, n_nodes, n_features);
+        const int dev = current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        CsrView v;
+        {
+            std::lock_guard<std::mutex> lk(transition->mu);
+            DeviceCSR& d = device_csr(transition, dev);
+            if (transpose) {
+                ensure_transpose(transition, d, s);
+                v = transposed_view_of(d);
+            } else {
+                v = view_of(d);
+            }
+        }
+        launch_spmm_f32(v, X_dev, Y_dev, n_features,
+                        spmm_default_shape(n_features, X_dev, Y_dev), s);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+}  // extern "C"

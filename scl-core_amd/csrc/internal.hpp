@@ -1,0 +1,138 @@
+// internal.hpp -- shared declarations of the scl diffusion library (host side).
+//
+// Error channel and ABI guard macros mirror scl/binding/c_api/core/internal.hpp
+// (307-343) and core.cpp (33-253): argument checks return a code early, bodies
+// run inside a try block, any exception becomes a code plus a thread-local
+// message, and success clears the last error.
+#pragma once
+
+#include "scl_core.h"
+#include "scl_sparse.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace scl_amd {
+
+// ----------------------------------------------------------------- errors --
+
+struct Error {
+    scl_error_t code;
+    std::string msg;
+};
+
+[[noreturn]] void fail(scl_error_t code, const std::string& msg);
+void set_last_error(scl_error_t code, const char* msg) noexcept;
+void clear_last_error() noexcept;
+scl_error_t handle_exception() noexcept;
+
+// HIP status -> exception: out-of-memory maps to SCL_ERROR_OUT_OF_MEMORY,
+// a missing device to SCL_ERROR_FEATURE_UNAVAILABLE, anything else to
+// SCL_ERROR_INTERNAL.
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace scl_amd
+
+#define SCL_API_CHECK(cond, code, msg)                   \
+    do {                                                 \
+        if (!(cond)) {                                   \
+            ::scl_amd::set_last_error((code), (msg));    \
+            return (code);                               \
+        }                                                \
+    } while (0)
+
+#define SCL_API_CHECK_NULL(p, msg) \
+    SCL_API_CHECK((p) != nullptr, SCL_ERROR_NULL_POINTER, msg)
+
+#define SCL_API_TRY try {
+#define SCL_API_CATCH                               \
+    }                                               \
+    catch (...) {                                   \
+        return ::scl_amd::handle_exception();       \
+    }
+
+#define SCL_API_RETURN_OK                 \
+    do {                                  \
+        ::scl_amd::clear_last_error();    \
+        return SCL_OK;                    \
+    } while (0)
+
+namespace scl_amd {
+
+// ------------------------------------------------------------ device data --
+
+// A handle's arrays on one GPU: the primary-slot CSR with int32 indices, plus
+// lazily built companions (transposed pattern, bf16 values, FP64 degrees).
+struct DeviceCSR {
+    int device = -1;
+    uint64_t generation = ~uint64_t(0);
+    int32_t n = 0;       // primary dimension (rows of the applied operator)
+    int32_t n_sec = 0;   // secondary dimension
+    int64_t nnz = 0;
+    int32_t* row_ptr = nullptr;  // n + 1
+    int32_t* col = nullptr;      // nnz
+    float* val = nullptr;        // nnz
+    int32_t* sched = nullptr;    // n: row visiting order (locality), may be null
+    // transposed pattern (for adjoints): t_row_ptr (n_sec + 1), t_col, t_val,
+    // t_src[p] = position in (col, val) of transposed entry p
+    uint64_t t_generation = ~uint64_t(0);
+    int32_t* t_row_ptr = nullptr;
+    int32_t* t_col = nullptr;
+    float* t_val = nullptr;
+    int32_t* t_src = nullptr;
+    int32_t* t_sched = nullptr;
+
+    ~DeviceCSR();
+    void release_transpose();
+};
+
+}  // namespace scl_amd
+
+// The opaque handle of the C ABI (scl_sparse_t points at one of these).
+struct scl_sparse_matrix {
+    int64_t rows = 0, cols = 0, nnz = 0;
+    bool is_csr = true;
+    bool wrapped = false;      // scl_sparse_wrap: arrays belong to the caller
+    bool owns_new = false;     // scl_sparse_wrap_and_own: delete[] on destroy
+    std::vector<int64_t> indptr_store, indices_store;
+    std::vector<float> data_store;
+    int64_t* indptr = nullptr;   // primary + 1 (always our own copy)
+    int64_t* indices = nullptr;  // nnz
+    float* data = nullptr;       // nnz
+    std::vector<int64_t> lengths_cache;
+    uint64_t generation = 0;     // bumped whenever values/pattern may change
+    std::mutex mu;
+    std::map<int, std::unique_ptr<scl_amd::DeviceCSR>> dev;
+
+    int64_t primary() const { return is_csr ? rows : cols; }
+    int64_t secondary() const { return is_csr ? cols : rows; }
+    ~scl_sparse_matrix();
+};
+
+namespace scl_amd {
+
+// ---------------------------------------------------------- device runtime --
+
+// Current HIP device, or throws SCL_ERROR_FEATURE_UNAVAILABLE when none.
+int current_device();
+// Library stream of a device (default priority, non-blocking).
+hipStream_t library_stream(int device);
+// Serialises the host-pointer entry points on one device (they share the
+// device's staging buffers).
+std::mutex& device_mutex(int device);
+// Grow-only device buffer `slot` of a device, at least `bytes` long.
+void* device_workspace(int device, int slot, size_t bytes);
+
+// Device copy of a handle, refreshed when its generation changed.
+// Caller holds h->mu.
+DeviceCSR& device_csr(scl_sparse_matrix* h, int device);
+// Builds the transposed pattern on the device if missing or stale.
+void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s);
+
+}  // namespace scl_amd

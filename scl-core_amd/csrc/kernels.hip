@@ -1,0 +1,332 @@
+// kernels.hip -- gfx950 kernels of the diffusion path.
+//
+// Row-split CSR SpMM (one step Y = P X):
+//   * a group of G lanes (G a power of two, 1..64) owns one output row and a
+//     slab of G*VEC consecutive features; a 256-thread workgroup (4 wave64s)
+//     owns 4 * 64/G rows of one slab;
+//   * every lane walks its row's nonzeros in stored order and accumulates
+//     acc = fma(p_ik, X[j_k, f], acc) in fp32 registers, so each output
+//     element is the same FMA chain as the reference's axpy loop
+//     (scl/kernel/diffusion.hpp:249-256 over axpy_simd 100-134);
+//   * X rows are gathered with VEC-wide (up to 16 B) loads, U nonzeros in
+//     flight per lane; for G = 64 the row is wave-uniform and its column
+//     indices and values come through the scalar unit;
+//   * workgroups are numbered slab-major (all rows of slab 0 first) and the
+//     hardware workgroup id is remapped so each XCD walks one contiguous run
+//     of rows (workgroups are dealt to the 8 XCDs round-robin), keeping the
+//     rows gathered by one XCD inside its own L2.
+// No MFMA: the contraction is sparse.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <stdexcept>
+
+namespace scl_amd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kNumXcd = 8;
+
+using f2 = float __attribute__((ext_vector_type(2)));
+using f4 = float __attribute__((ext_vector_type(4)));
+
+template <int VEC> struct Vec;
+template <> struct Vec<1> { using T = float; };
+template <> struct Vec<2> { using T = f2; };
+template <> struct Vec<4> { using T = f4; };
+
+__device__ __forceinline__ void fma_vec(float a, float x, float (&acc)[1]) {
+    acc[0] = __builtin_fmaf(a, x, acc[0]);
+}
+__device__ __forceinline__ void fma_vec(float a, f2 x, float (&acc)[2]) {
+    acc[0] = __builtin_fmaf(a, x.x, acc[0]);
+    acc[1] = __builtin_fmaf(a, x.y, acc[1]);
+}
+__device__ __forceinline__ void fma_vec(float a, f4 x, float (&acc)[4]) {
+    acc[0] = __builtin_fmaf(a, x.x, acc[0]);
+    acc[1] = __builtin_fmaf(a, x.y, acc[1]);
+    acc[2] = __builtin_fmaf(a, x.z, acc[2]);
+    acc[3] = __builtin_fmaf(a, x.w, acc[3]);
+}
+
+template <int VEC, bool NT>
+__device__ __forceinline__ void store_vec(float* p, const float (&acc)[VEC]) {
+    if constexpr (VEC == 1) {
+        if constexpr (NT) __builtin_nontemporal_store(acc[0], p);
+        else *p = acc[0];
+    } else if constexpr (VEC == 2) {
+        f2 v = {acc[0], acc[1]};
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f2*>(p));
+        else *reinterpret_cast<f2*>(p) = v;
+    } else {
+        f4 v = {acc[0], acc[1], acc[2], acc[3]};
+        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
+        else *reinterpret_cast<f4*>(p) = v;
+    }
+}
+
+// Hardware workgroup id -> logical id such that XCD x (which receives ids
+// x, x+8, x+16, ...) gets one contiguous range of logical ids.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int x = b % kNumXcd, idx = b / kNumXcd;
+    const int q = nb / kNumXcd, r = nb % kNumXcd;
+    return x * q + (x < r ? x : r) + idx;
+}
+
+template <int VEC, int G, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const int32_t* __restrict__ sched,
+    const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
+    int32_t n_rowblocks) {
+    using VT = typename Vec<VEC>::T;
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = (kBlock / 64) * RPW;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int slab = L / n_rowblocks;
+    const int rb = L - slab * n_rowblocks;
+    const int lane = threadIdx.x & 63;
+    // wave index is uniform across the wave: keep it in a scalar register
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = lane / G, li = lane % G;
+    const int rr = rb * RPB + wave * RPW + grp;
+    if (rr >= n) return;
+    int row = sched ? sched[rr] : rr;
+    if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int f = slab * (G * VEC) + li * VEC;
+    const bool active = f < F;
+    const int beg = row_ptr[row], end = row_ptr[row + 1];
+    const float* Xf = X + f;
+
+    float acc[VEC];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+
+    int k = beg;
+    for (; k + U <= end; k += U) {
+        int j[U];
+        float v[U];
+        VT x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            j[u] = col[k + u];
+            v[u] = val[k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (active) x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j[u] * F);
+            else x[u] = VT{};
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) fma_vec(v[u], x[u], acc);
+    }
+    for (; k < end; ++k) {
+        const int jj = col[k];
+        const float vv = val[k];
+        VT xx = VT{};
+        if (active) xx = *reinterpret_cast<const VT*>(Xf + (size_t)jj * F);
+        fma_vec(vv, xx, acc);
+    }
+    if (active) store_vec<VEC, NT>(Y + (size_t)row * F + f, acc);
+}
+
+using SpmmFn = void (*)(const int32_t*, const int32_t*, const float*,
+                        const int32_t*, const float*, float*, int32_t, int32_t,
+                        int32_t);
+
+template <int VEC, int G, int U>
+SpmmFn pick_nt(bool nt) {
+    return nt ? k_spmm_rowsplit<VEC, G, U, true> : k_spmm_rowsplit<VEC, G, U, false>;
+}
+
+template <int VEC, int G>
+SpmmFn pick_u(int u, bool nt) {
+    return u >= 8 ? pick_nt<VEC, G, 8>(nt) : pick_nt<VEC, G, 4>(nt);
+}
+
+template <int VEC>
+SpmmFn pick_g(int g, int u, bool nt) {
+    switch (g) {
+        case 1: return pick_u<VEC, 1>(u, nt);
+        case 2: return pick_u<VEC, 2>(u, nt);
+        case 4: return pick_u<VEC, 4>(u, nt);
+        case 8: return pick_u<VEC, 8>(u, nt);
+        case 16: return pick_u<VEC, 16>(u, nt);
+        case 32: return pick_u<VEC, 32>(u, nt);
+        default: return pick_u<VEC, 64>(u, nt);
+    }
+}
+
+SpmmFn pick_spmm(const SpmmShape& s) {
+    switch (s.vec) {
+        case 1: return pick_g<1>(s.group, s.unroll, s.nontemporal);
+        case 2: return pick_g<2>(s.group, s.unroll, s.nontemporal);
+        default: return pick_g<4>(s.group, s.unroll, s.nontemporal);
+    }
+}
+
+int pow2_ceil(int64_t x) {
+    int p = 1;
+    while (p < x && p < 64) p <<= 1;
+    return p;
+}
+
+void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
+    SpmmShape s;
+    const auto a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
+    if (F % 4 == 0 && a % 16 == 0) s.vec = 4;
+    else if (F % 2 == 0 && a % 8 == 0) s.vec = 2;
+    else s.vec = 1;
+    s.group = pow2_ceil((F + s.vec - 1) / s.vec);
+    s.unroll = 8;
+    s.nontemporal = false;
+    return s;
+}
+
+void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
+                     const SpmmShape& shape, hipStream_t s) {
+    if (P.n <= 0 || F <= 0) return;
+    const int rpb = (kBlock / 64) * (64 / shape.group);
+    const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
+    const int64_t n_slabs = (F + (int64_t)shape.group * shape.vec - 1) /
+                            ((int64_t)shape.group * shape.vec);
+    const int64_t nb = n_rowblocks * n_slabs;
+    if (nb > INT32_MAX) throw std::runtime_error("spmm grid too large");
+    hipLaunchKernelGGL(pick_spmm(shape), dim3((unsigned)nb), dim3(kBlock), 0, s,
+                       P.row_ptr, P.col, P.val, P.sched, X, Y, P.n, (int32_t)F,
+                       (int32_t)n_rowblocks);
+    check_launch("spmm launch");
+}
+
+// ---------------------------------------------------------- FP64 degrees --
+
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_row_sums_f64(
+    const int32_t* __restrict__ row_ptr, const float* __restrict__ val,
+    int32_t n, double* __restrict__ deg) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) acc += (double)val[k];
+    deg[i] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_normalize_rows(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    float* __restrict__ val, int32_t n, const double* __restrict__ deg,
+    int symmetric) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double di = deg[i];
+    if (symmetric) {
+        const double ri = di > 1e-15 ? 1.0 / sqrt(di) : 0.0;
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+            const double dj = deg[col[k]];
+            const double rj = dj > 1e-15 ? 1.0 / sqrt(dj) : 0.0;
+            val[k] = (float)(((double)val[k] * ri) * rj);
+        }
+    } else {
+        for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k)
+            val[k] = di > 1e-15 ? (float)((double)val[k] / di) : 0.0f;
+    }
+}
+
+constexpr int kRedBlocks = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kBlock / 64; ++q) t += sh[q];
+    __syncthreads();
+    return t;
+}
+
+// partial[b] = sum of deg over a grid-stride slice
+__global__ __launch_bounds__(kBlock) void k_sum_partial(
+    const double* __restrict__ x, int32_t n, double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64];
+    double v = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        v += x[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// out = sum of partial[0..m)
+__global__ __launch_bounds__(kBlock) void k_sum_final(
+    const double* __restrict__ partial, int m, double* __restrict__ out) {
+    __shared__ double sh[kBlock / 64];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < m; i += kBlock) v += partial[i];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) *out = v;
+}
+
+// partial[b] = sum over transposed rows j of |(P^T pi)_j - pi_j|
+__global__ __launch_bounds__(kBlock) void k_residual_partial(
+    const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
+    const float* __restrict__ t_val, int32_t n, const double* __restrict__ deg,
+    const double* __restrict__ total, double* __restrict__ partial) {
+    __shared__ double sh[kBlock / 64];
+    const double D = *total;
+    double v = 0.0;
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        double y = 0.0;
+        for (int p = t_row_ptr[j]; p < t_row_ptr[j + 1]; ++p)
+            y += (double)t_val[p] * (deg[t_col[p]] / D);
+        v += fabs(y - deg[j] / D);
+    }
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+}  // namespace
+
+void launch_row_sums_f64(const int32_t* row_ptr, const float* val, int32_t n,
+                         double* deg, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_row_sums_f64, dim3((n + kBlock - 1) / kBlock), dim3(kBlock),
+                       0, s, row_ptr, val, n, deg);
+    check_launch("row sums launch");
+}
+
+void launch_normalize_rows(const int32_t* row_ptr, const int32_t* col,
+                           float* val, int32_t n, const double* deg,
+                           int symmetric, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_normalize_rows, dim3((n + kBlock - 1) / kBlock),
+                       dim3(kBlock), 0, s, row_ptr, col, val, n, deg, symmetric);
+    check_launch("normalize launch");
+}
+
+void launch_stationarity_residual(const int32_t* t_row_ptr, const int32_t* t_col,
+                                  const float* t_val, int32_t n,
+                                  const double* deg, double* partial,
+                                  double* out, hipStream_t s) {
+    const int nb = std::max(1, std::min(kRedBlocks, (n + kBlock - 1) / kBlock));
+    double* total = partial + kRedBlocks;
+    hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(kBlock), 0, s, deg, n, partial);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, s, partial, nb, total);
+    hipLaunchKernelGGL(k_residual_partial, dim3(nb), dim3(kBlock), 0, s, t_row_ptr,
+                       t_col, t_val, n, deg, total, partial);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(kBlock), 0, s, partial, nb, out);
+    check_launch("residual launch");
+}
+
+}  // namespace scl_amd

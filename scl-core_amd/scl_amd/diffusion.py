@@ -1,0 +1,74 @@
+"""Host-array wrappers of the drop-in diffusion entry points (scl_diffusion.h).
+
+Each function calls the C ABI exactly as a ctypes binding of the reference's
+``scl/binding/c_api/diffusion.h`` would (the reference's codegen would emit
+such a module into ``python/scl/_bindings/diffusion.py``); arrays are
+float32 / C-contiguous and are modified in place like the C buffers are.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from .sparse import SparseMatrixBase
+
+
+def _f32_inplace(a: np.ndarray) -> np.ndarray:
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float32
+            and a.flags.c_contiguous and a.flags.writeable):
+        raise TypeError("expected a writeable C-contiguous float32 numpy array")
+    return a
+
+
+def compute_transition_matrix(adjacency: SparseMatrixBase, symmetric: bool = False
+                              ) -> None:
+    """scl_diffusion_compute_transition_matrix (values normalised in place)."""
+    _lib.check(_lib.load().scl_diffusion_compute_transition_matrix(
+        adjacency.handle, 1 if symmetric else 0))
+
+
+def compute_transition_matrix_ex(adjacency: SparseMatrixBase,
+                                 symmetric: bool = False
+                                 ) -> Tuple[np.ndarray, float]:
+    """Normalises in place and returns (FP64 degrees, FP64 stationarity residual)."""
+    rows = adjacency.shape[0] if adjacency._is_csr else adjacency.shape[1]
+    deg = np.empty(rows, np.float64)
+    res = ctypes.c_double(0.0)
+    _lib.check(_lib.load().scl_gpu_compute_transition_matrix_ex(
+        adjacency.handle, 1 if symmetric else 0, deg.ctypes.data,
+        ctypes.addressof(res)))
+    return deg, res.value
+
+
+def diffuse_matrix(transition: SparseMatrixBase, X: np.ndarray, n_steps: int,
+                   n_nodes: Optional[int] = None,
+                   n_features: Optional[int] = None) -> np.ndarray:
+    """scl_diffusion_diffuse_matrix: X <- P^t X in place; returns X."""
+    X = _f32_inplace(X)
+    n = X.shape[0] if n_nodes is None else n_nodes
+    F = (X.shape[1] if X.ndim == 2 else 1) if n_features is None else n_features
+    _lib.check(_lib.load().scl_diffusion_diffuse_matrix(
+        transition.handle, X.ctypes.data, n, F, n_steps))
+    return X
+
+
+def magic_impute(transition: SparseMatrixBase, X: np.ndarray, t: int = 0
+                 ) -> np.ndarray:
+    """scl_diffusion_magic_impute (t == 0 means 3)."""
+    X = _f32_inplace(X)
+    _lib.check(_lib.load().scl_diffusion_magic_impute(
+        transition.handle, X.ctypes.data, X.shape[0],
+        X.shape[1] if X.ndim == 2 else 1, t))
+    return X
+
+
+def diffuse_vector(transition: SparseMatrixBase, x: np.ndarray, n_steps: int
+                   ) -> np.ndarray:
+    """scl_diffusion_diffuse_vector: x <- P^t x in place; returns x."""
+    x = _f32_inplace(x)
+    _lib.check(_lib.load().scl_diffusion_diffuse_vector(
+        transition.handle, x.ctypes.data, x.shape[0], n_steps))
+    return x

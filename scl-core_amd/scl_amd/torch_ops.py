@@ -1,0 +1,49 @@
+"""Device-tensor entry points (scl_diffusion_gpu.h) for torch tensors.
+
+Tensors must be float32, C-contiguous and on the current HIP device; work is
+enqueued on torch's current stream, so it orders with surrounding torch ops.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+from .sparse import SparseMatrixBase
+
+
+def _stream_ptr(t: torch.Tensor):
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _check_dense(X: torch.Tensor) -> None:
+    if not X.is_cuda:
+        raise ValueError("expected a device tensor")
+    if X.dtype != torch.float32 or not X.is_contiguous() or X.dim() != 2:
+        raise ValueError("expected a contiguous (n, F) float32 tensor")
+
+
+def diffuse(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> torch.Tensor:
+    """Returns P^t X (X unchanged)."""
+    _check_dense(X)
+    n, F = X.shape
+    Y = torch.empty_like(X)
+    work = torch.empty_like(X) if n_steps >= 2 else None
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().scl_gpu_diffuse_matrix_device(
+            P.handle, X.data_ptr(), Y.data_ptr(),
+            None if work is None else work.data_ptr(), n, F, n_steps,
+            _stream_ptr(X)))
+    return Y
+
+
+def spmm(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
+         ) -> torch.Tensor:
+    """Returns P X, or P^T X when transpose is set."""
+    _check_dense(X)
+    n, F = X.shape
+    Y = torch.empty_like(X)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().scl_gpu_spmm_device(
+            P.handle, X.data_ptr(), Y.data_ptr(), n, F, 1 if transpose else 0,
+            _stream_ptr(X)))
+    return Y

@@ -1,0 +1,155 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol
+declared in include/*.h, validates arguments with the reference's error codes
+(scl/binding/c_api/diffusion.cpp:94-116, core/sparse.cpp:51-700), and manages
+handles.  No compute entry point is called here except to check that, on a
+host without a HIP device, it reports SCL_ERROR_FEATURE_UNAVAILABLE."""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import scl_amd
+from scl_amd import CscMatrix, CsrMatrix, _lib, diffusion
+
+INCLUDE = Path(__file__).resolve().parent.parent / "include"
+
+
+def declared_symbols():
+    names = set()
+    for h in INCLUDE.glob("*.h"):
+        text = h.read_text()
+        for m in re.finditer(r"SCL_EXPORT\s+[\w\s\*]+?\b(scl_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers the same set
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_build_config():
+    lib = _lib.load()
+    assert lib.scl_get_version() == b"1.0.0"
+    assert lib.scl_get_build_config().startswith(b"float32+int64")
+    assert lib.scl_is_ok(0) == 1 and lib.scl_is_error(4) == 1
+
+
+def small_csr():
+    # reference test_algebra.cpp make_test_matrix_3x3: [[1,0,2],[0,3,0],[4,5,6]]
+    return (np.array([0, 2, 3, 6], np.int64), np.array([0, 2, 1, 0, 1, 2], np.int64),
+            np.array([1, 2, 3, 4, 5, 6], np.float32))
+
+
+def test_handle_lifecycle_and_queries():
+    ip, ix, d = small_csr()
+    with CsrMatrix.from_arrays(ip, ix, d, (3, 3)) as P:
+        assert P.shape == (3, 3) and P.nnz == 6 and P.is_contiguous
+        e = P.export()
+        assert np.array_equal(e[0], ip) and np.array_equal(e[1], ix)
+        assert np.array_equal(e[2], d)
+        C = P.clone()
+        assert np.array_equal(C.export()[2], d)
+        T = P.transpose()
+        assert isinstance(T, CscMatrix) and T.shape == (3, 3)
+        # entries regrouped by column: column 0 = rows {0, 2}, ...
+        tp, ti, td = T.export()
+        assert np.array_equal(tp, [0, 2, 4, 6])
+        assert np.array_equal(ti, [0, 2, 1, 2, 0, 2])
+        assert np.array_equal(td, [1, 4, 3, 5, 2, 6])
+        back = T.transpose()
+        assert np.array_equal(back.export()[1], ix)
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.scl_sparse_destroy(ctypes.byref(h)) == 0  # NULL handle is fine
+    assert lib.scl_sparse_destroy(None) == 0
+
+
+def test_wrap_shares_caller_arrays():
+    ip, ix, d = small_csr()
+    P = CsrMatrix.wrap(ip, ix, d, (3, 3))
+    d[0] = 9.0
+    assert P.export()[2][0] == 9.0
+    lib = _lib.load()
+    ptr = ctypes.c_void_p()
+    size = ctypes.c_size_t()
+    assert lib.scl_sparse_get_data(P.handle, ctypes.byref(ptr), ctypes.byref(size)) == 0
+    assert ptr.value == d.ctypes.data and size.value == 6
+    lens = ctypes.c_void_p()
+    assert lib.scl_sparse_get_lengths(P.handle, ctypes.byref(lens), ctypes.byref(size)) == 0
+    got = np.ctypeslib.as_array(ctypes.cast(lens, ctypes.POINTER(ctypes.c_int64)), (3,))
+    assert np.array_equal(got, [2, 1, 3])
+
+
+def test_create_rejects_bad_arguments():
+    lib = _lib.load()
+    ip, ix, d = small_csr()
+    h = ctypes.c_void_p()
+    A = (ip.ctypes.data, ix.ctypes.data, d.ctypes.data)
+    assert lib.scl_sparse_create(None, 3, 3, 6, *A, 1) == _lib.SCL_ERROR_NULL_POINTER
+    assert lib.scl_sparse_create(ctypes.byref(h), 3, 3, 6, None, A[1], A[2], 1) == 4
+    assert lib.scl_sparse_create(ctypes.byref(h), 0, 3, 6, *A, 1) == 10
+    assert lib.scl_sparse_create(ctypes.byref(h), 3, 3, -1, *A, 1) == 10
+    assert b"non-negative" in lib.scl_get_last_error()
+    bad = ix.copy()
+    bad[2] = 7
+    assert lib.scl_sparse_create(ctypes.byref(h), 3, 3, 6, A[0], bad.ctypes.data,
+                                 A[2], 1) == _lib.SCL_ERROR_INDEX_OUT_OF_BOUNDS
+    assert lib.scl_sparse_create_with_strategy(ctypes.byref(h), 3, 3, 6, *A, 1, 9) == 10
+    # success clears the last error
+    assert lib.scl_sparse_create(ctypes.byref(h), 3, 3, 6, *A, 1) == 0
+    assert lib.scl_get_last_error_code() == 0
+    assert lib.scl_get_last_error() == b"No error"
+    lib.scl_sparse_destroy(ctypes.byref(h))
+
+
+def test_diffusion_argument_checks_come_first():
+    """Null / non-positive arguments are rejected before any device work
+    (diffusion.cpp:101-104), and a size mismatch is a dimension error."""
+    lib = _lib.load()
+    ip, ix, d = small_csr()
+    P = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    X = np.ones((3, 2), np.float32)
+    f = lib.scl_diffusion_diffuse_matrix
+    assert f(None, X.ctypes.data, 3, 2, 1) == 4
+    assert f(P.handle, None, 3, 2, 1) == 4
+    assert f(P.handle, X.ctypes.data, 0, 2, 1) == 10
+    assert f(P.handle, X.ctypes.data, 3, 0, 1) == 10
+    assert f(P.handle, X.ctypes.data, 3, 2, 0) == 10
+    assert f(P.handle, X.ctypes.data, 4, 2, 1) == _lib.SCL_ERROR_DIMENSION_MISMATCH
+    assert lib.scl_diffusion_magic_impute(P.handle, X.ctypes.data, 3, 0, 0) == 10
+    assert lib.scl_diffusion_diffuse_vector(P.handle, None, 3, 1) == 4
+    assert lib.scl_diffusion_diffuse_vector(P.handle, X.ctypes.data, 3, 0) == 10
+    assert lib.scl_diffusion_compute_transition_matrix(None, 0) == 4
+    rect = CsrMatrix.from_arrays([0, 1, 2], [0, 2], [1, 1], (2, 3))
+    assert lib.scl_diffusion_compute_transition_matrix(rect.handle, 1) == 11
+    # device entry points validate too
+    g = lib.scl_gpu_diffuse_matrix_device
+    assert g(P.handle, None, X.ctypes.data, None, 3, 2, 1, None) == 4
+    assert g(P.handle, X.ctypes.data, X.ctypes.data, None, 3, 2, 1, None) == 10
+    assert g(P.handle, 16, 32, None, 3, 2, 2, None) == 4  # work missing
+    assert np.array_equal(X, np.ones((3, 2), np.float32))
+
+
+@pytest.mark.skipif(scl_amd.device_count() > 0, reason="host has a HIP device")
+def test_no_device_reports_feature_unavailable():
+    ip, ix, d = small_csr()
+    P = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    X = np.ones((3, 2), np.float32)
+    with pytest.raises(scl_amd.SCLError) as e:
+        diffusion.diffuse_matrix(P, X, 2)
+    assert e.value.code == _lib.SCL_ERROR_FEATURE_UNAVAILABLE
+    with pytest.raises(scl_amd.SCLError) as e:
+        diffusion.compute_transition_matrix(P)
+    assert e.value.code == _lib.SCL_ERROR_FEATURE_UNAVAILABLE
+    assert np.array_equal(X, np.ones((3, 2), np.float32))
+    assert np.array_equal(P.export()[2], d)

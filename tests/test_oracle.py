@@ -1,0 +1,235 @@
+"""Pins the CPU restatement / FP64 oracle (oracle/oracle.c) on the CPU.
+
+Each check is chosen so that a plausible mistake in the oracle fails it:
+reference known answers (test_algebra.cpp, test_impute.cpp), closed forms
+(ring, complete graph), dense FP64 matrix powers, invariants (constant
+columns, pi-weighted means, adjoint identity), finite differences, and
+constructed inputs where the accumulation order or a fused multiply-add
+changes the FP32 result.
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import csr_from_dense, dense_from_csr
+from oracle import oracle as orc
+from scl_amd import synth
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+FIXTURES = sorted(GOLDEN.glob("*.npz"))
+
+
+def load_fixture(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("path", FIXTURES, ids=[p.stem for p in FIXTURES])
+def test_oracle_reproduces_golden(path):
+    fx = load_fixture(path)
+    t = int(fx["t"])
+    y64 = orc.diffuse_f64(fx["indptr"], fx["indices"], fx["data"],
+                          fx["X"].astype(np.float64), t)
+    np.testing.assert_allclose(y64, fx["expected_f64"], rtol=0, atol=1e-12 *
+                               max(1.0, np.abs(fx["expected_f64"]).max()))
+    y32 = orc.diffuse_matrix_f32(fx["indptr"], fx["indices"], fx["data"], fx["X"], t)
+    assert orc.rel_err(y32, fx["expected_f64"]) <= 1e-5
+    if "expected_f32" in fx:
+        assert np.array_equal(y32, fx["expected_f32"])
+
+
+def test_reference_spmm_known_answers():
+    fx = load_fixture(GOLDEN / "algebra_3x3.npz")
+    Y = orc.spmm_step_f32(fx["indptr"], fx["indices"], fx["data"], fx["X"])
+    assert np.array_equal(Y, np.array([[11, 14], [9, 12], [49, 64]], np.float32))
+    y = orc.diffuse_vector_f32(fx["indptr"], fx["indices"], fx["data"], fx["x"], 1)
+    assert np.array_equal(y, np.array([7, 6, 32], np.float32))
+    # and the alpha/beta combination of test_algebra.cpp:309-325
+    assert np.array_equal(2 * y + 3 * np.array([10, 20, 30], np.float32),
+                          np.array([44, 72, 154], np.float32))
+
+
+def test_fp32_chain_is_fma_in_stored_order():
+    """Row 0 = [1 at col 0, a at col 1]; X[:,0] = [c, x] with
+    a*x = 1 + 2^-11 + 2^-24 and c = -(1 + 2^-11).  The stored-order FMA chain
+    gives fma(a, x, c) = 2^-24; an unfused multiply-add or the reverse order
+    gives 0."""
+    a = np.float32(1 + 2.0 ** -12)
+    x = np.float32(1 + 2.0 ** -12)
+    c = np.float32(-(1 + 2.0 ** -11))
+    indptr = np.array([0, 2, 3], np.int64)
+    indices = np.array([0, 1, 1], np.int64)
+    data = np.array([1.0, a, 1.0], np.float32)
+    X = np.array([[c], [x]], np.float32)
+    Y = orc.spmm_step_f32(indptr, indices, data, X)
+    assert Y[0, 0] == np.float32(2.0 ** -24)
+    v = orc.diffuse_vector_f32(indptr, indices, data, X[:, 0], 1)
+    assert v[0] == np.float32(2.0 ** -24)
+
+
+def test_unsorted_slot_is_accumulated_in_stored_order():
+    # -1e8 + 1 + 1e8 = 0 in float (1 is absorbed); 1 - 1e8 + 1e8 = 0 too;
+    # 1e8 - 1e8 + 1 = 1: only the stored order [0, 2, 1] gives 1.
+    X = np.array([[1e8], [1.0], [-1e8]], np.float32)
+    ip = np.array([0, 3, 3, 3], np.int64)
+    dv = np.ones(3, np.float32)
+    assert orc.spmm_step_f32(ip, np.array([0, 2, 1], np.int64), dv, X)[0, 0] == 1.0
+    assert orc.spmm_step_f32(ip, np.array([0, 1, 2], np.int64), dv, X)[0, 0] == 0.0
+
+
+def _random_csr(rng, n, density, normalize=True, n_cols=None):
+    m = n if n_cols is None else n_cols
+    A = (rng.random((n, m)) < density) * rng.random((n, m))
+    if normalize:
+        s = A.sum(axis=1, keepdims=True)
+        A = np.divide(A, s, out=np.zeros_like(A), where=s > 0)
+    return A.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_brute_force_dense_power(seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, F, t = 50, 7, 4
+    A = _random_csr(rng, n, 0.15)
+    ip, ix, d = csr_from_dense(A)
+    X = rng.standard_normal((n, F)).astype(np.float32)
+    ref = np.linalg.matrix_power(A, t) @ X.astype(np.float64)
+    y64 = orc.diffuse_f64(ip, ix, d, X, t)
+    np.testing.assert_allclose(y64, ref, rtol=1e-12, atol=1e-12)
+    y32 = orc.diffuse_matrix_f32(ip, ix, d, X, t)
+    assert orc.rel_err(y32, ref) < 1e-5
+    yt = orc.diffuse_f64(ip, ix, d, X, t, transpose=True)
+    np.testing.assert_allclose(yt, np.linalg.matrix_power(A.T, t) @ X, rtol=1e-12,
+                               atol=1e-12)
+    # the vector path is the F == 1 case
+    v = orc.diffuse_vector_f32(ip, ix, d, X[:, 0], t)
+    assert np.array_equal(v, orc.diffuse_matrix_f32(ip, ix, d, X[:, :1], t)[:, 0])
+
+
+def test_constant_columns_preserved():
+    g = synth.uniform_knn(2000, 15, seed=7).normalized()
+    X = np.full((2000, 3), 2.5, np.float32)
+    y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X, 5)
+    # P rows sum to 1 within 15 fp32 roundings
+    assert np.abs(y64 - 2.5).max() < 2.5 * 16 * 2.0 ** -24 * 5
+    y32 = orc.diffuse_matrix_f32(g.indptr, g.indices, g.data, X, 5)
+    assert np.abs(y32 - 2.5).max() < 2.5 * 1e-5
+
+
+def test_pi_weighted_means_preserved_on_symmetric_graph():
+    g = synth.knn_graph(600, 10, seed=3)
+    W = dense_from_csr(600, g.indptr, g.indices, g.data)
+    assert np.array_equal(W, W.T)  # union-max symmetrisation
+    deg = W.sum(axis=1)
+    P = W / deg[:, None]
+    ip, ix, _ = csr_from_dense(P)
+    X = np.random.Generator(np.random.PCG64(5)).standard_normal((600, 4))
+    pi = deg / deg.sum()
+    vals = P[P != 0]  # row-major order == stored order
+    Y = orc.diffuse_f64(ip, ix, vals, X, 6)
+    np.testing.assert_allclose(pi @ Y, pi @ X, rtol=1e-11, atol=1e-12)
+
+
+def test_transition_f32_follows_reference_formula():
+    # rows: [1,2,3], empty, tiny (sum <= 1e-15 -> 0), single
+    indptr = np.array([0, 3, 3, 5, 6], np.int64)
+    indices = np.array([0, 1, 3, 0, 2, 1], np.int64)
+    vals = np.array([1, 2, 3, 4e-16, 5e-16, 7], np.float32)
+    out = orc.transition_f32(indptr, indices, vals, symmetric=False)
+    inv = np.float32(1) / np.float32(6)
+    assert np.array_equal(out[:3], np.array([1, 2, 3], np.float32) * inv)
+    assert np.array_equal(out[3:5], np.zeros(2, np.float32))
+    assert out[5] == np.float32(7) * (np.float32(1) / np.float32(7))
+    # symmetric: v *= d_i^-1/2 * d_j^-1/2, all in float
+    s = np.array([6, 0, np.float32(4e-16) + np.float32(5e-16), 7], np.float32)
+    with np.errstate(divide="ignore"):
+        dis = np.where(s > np.float32(1e-15), np.float32(1) / np.sqrt(s), np.float32(0))
+    out = orc.transition_f32(indptr, indices, vals, symmetric=True)
+    exp = []
+    for i in range(4):
+        for k in range(indptr[i], indptr[i + 1]):
+            exp.append(np.float32(vals[k] * np.float32(dis[i] * dis[indices[k]])))
+    assert np.array_equal(out, np.array(exp, np.float32))
+
+
+def test_transition_f64_formula_and_stochastic_rows():
+    g = synth.uniform_knn(500, 15, seed=11)
+    out, deg = orc.transition_f64(g.indptr, g.indices, g.data)
+    exp = np.empty_like(g.data)
+    for i in range(500):
+        s = 0.0
+        for k in range(g.indptr[i], g.indptr[i + 1]):
+            s += float(g.data[k])
+        assert deg[i] == s
+        for k in range(g.indptr[i], g.indptr[i + 1]):
+            exp[k] = np.float32(float(g.data[k]) / s)
+    assert np.array_equal(out, exp)
+    rs = np.add.reduceat(out.astype(np.float64), g.indptr[:-1])
+    assert np.abs(rs - 1).max() < 15 * 2.0 ** -24
+    # f32 reference formula differs by a few ulps only
+    ref32 = orc.transition_f32(g.indptr, g.indices, g.data)
+    assert np.abs(ref32.astype(np.float64) - out).max() / out.max() < 20 * 2.0 ** -24
+    # symmetric mode: v * r_i * r_j with r = 1/sqrt(d) in FP64
+    sym, deg2 = orc.transition_f64(g.indptr, g.indices, g.data, symmetric=True)
+    r = 1 / np.sqrt(deg2)
+    rows = np.repeat(np.arange(500), np.diff(g.indptr))
+    exp = ((g.data.astype(np.float64) * r[rows]) * r[g.indices]).astype(np.float32)
+    assert np.array_equal(sym, exp)
+
+
+def test_grad_matches_finite_differences_and_adjoint_identity():
+    rng = np.random.Generator(np.random.PCG64(9))
+    n, F, t = 30, 3, 3
+    A = _random_csr(rng, n, 0.2)
+    ip, ix, d = csr_from_dense(A)
+    vals = d.astype(np.float64)
+    X = rng.standard_normal((n, F))
+    T = rng.standard_normal((n, F))
+    L, Y, dX, dP = orc.grad_f64(ip, ix, vals, X, T, t)
+    np.testing.assert_allclose(Y, np.linalg.matrix_power(A, t) @ X, rtol=1e-12, atol=1e-12)
+    assert abs(L - 0.5 * ((Y - T) ** 2).sum()) < 1e-10
+    eps = 1e-6
+    for (i, c) in [(0, 0), (5, 2), (17, 1), (29, 0)]:
+        Xp, Xm = X.copy(), X.copy()
+        Xp[i, c] += eps
+        Xm[i, c] -= eps
+        fd = (orc.loss_f64(ip, ix, vals, Xp, T, t) - orc.loss_f64(ip, ix, vals, Xm, T, t)) / (2 * eps)
+        assert abs(fd - dX[i, c]) < 1e-6 * max(1, abs(fd))
+    for k in [0, len(vals) // 3, len(vals) - 1]:
+        vp, vm = vals.copy(), vals.copy()
+        vp[k] += eps
+        vm[k] -= eps
+        fd = (orc.loss_f64(ip, ix, vp, X, T, t) - orc.loss_f64(ip, ix, vm, X, T, t)) / (2 * eps)
+        assert abs(fd - dP[k]) < 1e-6 * max(1, abs(fd))
+    # <P^t X, G> == <X, (P^T)^t G>
+    G = rng.standard_normal((n, F))
+    lhs = (orc.diffuse_f64(ip, ix, vals, X, t) * G).sum()
+    rhs = (X * orc.diffuse_f64(ip, ix, vals, G, t, transpose=True)).sum()
+    assert abs(lhs - rhs) < 1e-10
+
+
+def test_stationarity_residual():
+    g = synth.knn_graph(400, 8, seed=4)
+    P, deg = orc.transition_f64(g.indptr, g.indices, g.data)
+    r = orc.stationarity_residual_f64(g.indptr, g.indices, P, deg)
+    assert r < 1e-6  # fp32 rounding of P only
+    # dense FP64 evaluation of the same formula on an asymmetric operator
+    u = synth.uniform_knn(300, 6, seed=4)
+    Pu, du = orc.transition_f64(u.indptr, u.indices, u.data)
+    Pd = dense_from_csr(300, u.indptr, u.indices, Pu)
+    pi = du / du.sum()
+    exp = np.abs(Pd.T @ pi - pi).sum()
+    got = orc.stationarity_residual_f64(u.indptr, u.indices, Pu, du)
+    assert got > 1e-3
+    assert abs(got - exp) < 1e-12 * max(1, exp)
+
+
+def test_relative_error_convention():
+    a = np.array([1.0, -3.0, 2.0], np.float32)
+    b = np.array([1.0, -2.0, 2.0], np.float64)
+    assert orc.rel_err(a, b) == pytest.approx(1.0 / 3.0)
+    z = np.zeros(3, np.float32)
+    assert orc.rel_err(z, np.full(3, 1e-16)) == pytest.approx(1e-16)
