@@ -65,6 +65,85 @@ SCL_EXPORT scl_error_t scl_gpu_compute_transition_matrix_ex(
     scl_sparse_t adjacency, int symmetric, double* degrees_out,
     double* residual_out);
 
+/* ---------------------------------------------------------------- shapes --
+ * Launch shape of the row-split SpMM for one operator (transpose == 0) or its
+ * transpose (transpose != 0) and one feature count:
+ *   vec     floats per lane per access: 1, 2 or 4 (must divide n_features)
+ *   group   lanes sharing one row: a power of two in 1..64; one launch covers
+ *           group * vec features of every row per workgroup column (a slab)
+ *   unroll  nonzeros in flight per lane group: 4, 8 or 16
+ *   flags   SCL_GPU_SHAPE_NONTEMPORAL: streaming stores of the output;
+ *           SCL_GPU_SHAPE_NO_SCHEDULE: natural row order instead of the
+ *           operator's locality order (operators of >= 16384 rows get one);
+ *           SCL_GPU_SHAPE_XCD_ROWS: each of the 8 XCDs owns a contiguous range
+ *           of rows and walks it slab by slab (default: each XCD owns a
+ *           range of slabs)
+ * Any shape gives bit-identical results; only the speed differs. */
+#define SCL_GPU_SHAPE_NONTEMPORAL 1
+#define SCL_GPU_SHAPE_NO_SCHEDULE 2
+#define SCL_GPU_SHAPE_XCD_ROWS 4
+
+SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
+                                              scl_index_t n_features,
+                                              int transpose, int vec, int group,
+                                              int unroll, int flags);
+
+/* The shape the next call would use with 16-byte aligned buffers. */
+SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
+                                              scl_index_t n_features,
+                                              int transpose, int* vec,
+                                              int* group, int* unroll,
+                                              int* flags);
+
+/* Times the candidate shapes for this operator and feature count on the
+ * caller's buffers (Y_dev is overwritten), keeps the fastest for later calls
+ * and returns its time per launch in best_ms (nullable).  Synchronises
+ * `stream`. */
+SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
+                                             const scl_real_t* X_dev,
+                                             scl_real_t* Y_dev,
+                                             scl_index_t n_nodes,
+                                             scl_index_t n_features,
+                                             int transpose, void* stream,
+                                             float* best_ms);
+
+/* ------------------------------------------------------------- gradients --
+ * Reverse mode of Y = P^t X for a loss L(Y): given dY = dL/dY, returns
+ *   dX = (P^T)^t dY   and   dP_k = sum_{s=1..t} <G_s[i,:], X_{s-1}[j,:]>
+ * for every stored nonzero k = (i, j) (CSR order), where X_s = P X_{s-1},
+ * G_t = dY and G_{s-1} = P^T G_s.  The forward keeps the iterates X_c, X_2c,
+ * ... (c = every) as checkpoints; the backward recomputes the others. */
+
+/* Replaces the operator's values with values_dev (nnz floats, device, stored
+ * order) in the device copies and in the handle's host arrays; synchronises
+ * `stream`. */
+SCL_EXPORT scl_error_t scl_gpu_sparse_set_values_device(scl_sparse_t matrix,
+                                                        const scl_real_t* values_dev,
+                                                        void* stream);
+
+/* Y_dev = P^t X_dev, also writing the checkpoints X_every, X_2every, ...
+ * (ceil(t/every) - 1 planes of n x F floats) to ckpt_dev; work_dev holds
+ * 2 planes and is required when n_steps >= 2. */
+SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
+    scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
+    scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
+/* Bytes of device workspace scl_gpu_diffuse_backward_device needs
+ * ((every + 1) planes, plus nnz floats when want_dp). */
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_workspace(
+    scl_sparse_t transition, scl_index_t n_nodes, scl_index_t n_features,
+    scl_index_t every, int want_dp, scl_size_t* bytes);
+
+/* dX_dev (and dP_dev, nnz floats, when not NULL) from dY_dev, the forward
+ * input X_dev and the checkpoints written by the forward call with the same
+ * n_steps and every.  dX_dev must differ from dY_dev. */
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
+    scl_sparse_t transition, const scl_real_t* X_dev, const scl_real_t* ckpt_dev,
+    const scl_real_t* dY_dev, scl_real_t* dX_dev, scl_real_t* dP_dev,
+    void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,32 @@ void DeviceCSR::release_transpose() {
     t_generation = ~uint64_t(0);
 }
 
+namespace {
+
+// FNV-1a over the pattern: decides whether a refreshed copy can keep its
+// schedule (values may change without the pattern changing).
+template <typename I>
+uint64_t pattern_fingerprint(int64_t n, const I* ptr, const I* idx) {
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ULL;
+    };
+    mix((uint64_t)n);
+    for (int64_t i = 0; i <= n; ++i) mix((uint64_t)ptr[i]);
+    for (int64_t k = 0; k < (int64_t)ptr[n]; ++k) mix((uint64_t)idx[k]);
+    return h;
+}
+
+int32_t* upload_schedule(const std::vector<int32_t>& order) {
+    int32_t* d = dev_alloc<int32_t>(order.size(), "schedule");
+    hip_check(hipMemcpy(d, order.data(), order.size() * 4, hipMemcpyHostToDevice),
+              "upload schedule");
+    return d;
+}
+
+}  // namespace
+
 DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
     auto& slot = h->dev[device];
     if (slot && slot->generation == h->generation) return *slot;
@@ -114,9 +140,7 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
     if (p >= lim || q >= lim || nnz >= lim)
         fail(SCL_ERROR_INVALID_ARGUMENT,
              "operator too large for int32 device indices (dims and nnz must be < 2^31-1)");
-    if (slot && slot->n == p && slot->nnz == nnz) {
-        // same shape: refresh in place
-    } else {
+    if (!(slot && slot->n == p && slot->n_sec == q && slot->nnz == nnz)) {
         slot.reset(new DeviceCSR());
         slot->device = device;
         slot->n = (int32_t)p;
@@ -136,6 +160,17 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
                   "upload col");
         hip_check(hipMemcpy(slot->val, h->data, (size_t)nnz * 4, hipMemcpyHostToDevice),
                   "upload val");
+    }
+    if (p >= kScheduleMinRows) {
+        const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
+        if (!slot->sched || slot->fingerprint != fp) {
+            dev_free(slot->sched);
+            slot->sched = nullptr;
+            slot->sched = upload_schedule(locality_schedule(p, h->indptr, h->indices));
+            slot->fingerprint = fp;
+            slot->shapes.clear();
+            slot->t_shapes.clear();
+        }
     }
     slot->release_transpose();
     slot->generation = h->generation;
@@ -175,6 +210,7 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
         hip_check(hipMemcpy(d.t_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice),
                   "upload transpose");
     }
+    if (q >= kScheduleMinRows) d.t_sched = upload_schedule(locality_schedule(q, trp.data(), tcol.data()));
     d.t_generation = d.generation;
 }
 

@@ -42,11 +42,20 @@ void check_operator(const scl_sparse_matrix* P, int64_t n_nodes, int64_t F) {
         fail(SCL_ERROR_INVALID_ARGUMENT, "n_features too large");
 }
 
+// Launch shape for F features: the one tuned or set for this operator, when
+// it fits the buffers, else the default.  Caller holds the handle's mutex.
+SpmmShape choose_shape(DeviceCSR& d, bool transposed, int64_t F, const void* X,
+                       const void* Y) {
+    auto& m = transposed ? d.t_shapes : d.shapes;
+    auto it = m.find(F);
+    if (it != m.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
+    return spmm_default_shape(F, X, Y);
+}
+
 // t steps between two device buffers, starting from buf[start]; returns the
 // index of the buffer holding the result.
-int pingpong(const CsrView& P, float* buf[2], int start, int64_t F, int64_t t,
-             hipStream_t s) {
-    const SpmmShape shape = spmm_default_shape(F, buf[0], buf[1]);
+int pingpong(const CsrView& P, const SpmmShape& shape, float* buf[2], int start,
+             int64_t F, int64_t t, hipStream_t s) {
     int cur = start;
     for (int64_t step = 0; step < t; ++step) {
         launch_spmm_f32(P, buf[cur], buf[cur ^ 1], F, shape, s);
@@ -69,7 +78,8 @@ void diffuse_host(scl_sparse_matrix* P, float* X, int64_t n, int64_t F,
     float* buf[2] = {static_cast<float*>(device_workspace(dev, 0, bytes)),
                      static_cast<float*>(device_workspace(dev, 1, bytes))};
     hip_check(hipMemcpyAsync(buf[0], X, bytes, hipMemcpyHostToDevice, s), "upload X");
-    const int fin = pingpong(view_of(d), buf, 0, F, t, s);
+    const SpmmShape shape = choose_shape(d, false, F, buf[0], buf[1]);
+    const int fin = pingpong(view_of(d), shape, buf, 0, F, t, s);
     hip_check(hipMemcpyAsync(X, buf[fin], bytes, hipMemcpyDeviceToHost, s), "download X");
     hip_check(hipStreamSynchronize(s), "diffusion");
 }
@@ -241,15 +251,20 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
         check_operator(transition, n_nodes, n_features);
         const int dev = current_device();
         CsrView v;
+        SpmmShape sh;
         {
             std::lock_guard<std::mutex> lk(transition->mu);
-            v = view_of(device_csr(transition, dev));
+            DeviceCSR& d = device_csr(transition, dev);
+            v = view_of(d);
+            sh = choose_shape(d, false, n_features, X_dev, Y_dev);
+            if (work_dev && !spmm_shape_valid(sh, n_features, work_dev, Y_dev))
+                sh = spmm_default_shape(n_features, work_dev, Y_dev);
+            if (!spmm_shape_valid(sh, n_features, X_dev, Y_dev))
+                sh = spmm_default_shape(n_features, X_dev, Y_dev);
+            if (work_dev && !spmm_shape_valid(sh, n_features, work_dev, X_dev))
+                sh = spmm_default_shape(n_features, work_dev, X_dev);
         }
         hipStream_t s = static_cast<hipStream_t>(stream);
-        const SpmmShape shape = spmm_default_shape(n_features, X_dev, Y_dev);
-        const SpmmShape wshape =
-            work_dev ? spmm_default_shape(n_features, work_dev, Y_dev) : shape;
-        const SpmmShape sh = (wshape.vec < shape.vec) ? wshape : shape;
         // step s (1-based) writes Y when t - s is even, else work, so the
         // last step lands in Y and consecutive steps alternate buffers
         const float* src = X_dev;
@@ -280,6 +295,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,
         const int dev = current_device();
         hipStream_t s = static_cast<hipStream_t>(stream);
         CsrView v;
+        SpmmShape sh;
         {
             std::lock_guard<std::mutex> lk(transition->mu);
             DeviceCSR& d = device_csr(transition, dev);
@@ -289,9 +305,119 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,
             } else {
                 v = view_of(d);
             }
+            sh = choose_shape(d, transpose != 0, n_features, X_dev, Y_dev);
         }
-        launch_spmm_f32(v, X_dev, Y_dev, n_features,
-                        spmm_default_shape(n_features, X_dev, Y_dev), s);
+        launch_spmm_f32(v, X_dev, Y_dev, n_features, sh, s);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
+                                              scl_index_t n_features,
+                                              int transpose, int vec, int group,
+                                              int unroll, int flags) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK(n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "n_features must be positive");
+    SpmmShape sh;
+    sh.vec = vec;
+    sh.group = group;
+    sh.unroll = unroll;
+    sh.nontemporal = (flags & kShapeNontemporal) != 0;
+    sh.schedule = (flags & kShapeNoSchedule) == 0;
+    sh.xcd_rows = (flags & kShapeXcdRows) != 0;
+    SCL_API_CHECK(spmm_shape_valid(sh, n_features, nullptr, nullptr),
+                  SCL_ERROR_INVALID_ARGUMENT,
+                  "invalid shape: vec in {1,2,4} dividing n_features, group a "
+                  "power of two in 1..64, unroll 4, 8 or 16");
+    SCL_API_TRY
+        const int dev = current_device();
+        std::lock_guard<std::mutex> lk(transition->mu);
+        DeviceCSR& d = device_csr(transition, dev);
+        (transpose ? d.t_shapes : d.shapes)[n_features] = sh;
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
+                                              scl_index_t n_features,
+                                              int transpose, int* vec,
+                                              int* group, int* unroll,
+                                              int* flags) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK(vec && group && unroll && flags, SCL_ERROR_NULL_POINTER,
+                  "Output pointer is null");
+    SCL_API_CHECK(n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "n_features must be positive");
+    SCL_API_TRY
+        const int dev = current_device();
+        std::lock_guard<std::mutex> lk(transition->mu);
+        DeviceCSR& d = device_csr(transition, dev);
+        // the choice for 16-byte aligned buffers
+        const SpmmShape sh = choose_shape(d, transpose != 0, n_features,
+                                          nullptr, nullptr);
+        *vec = sh.vec;
+        *group = sh.group;
+        *unroll = sh.unroll;
+        *flags = (sh.nontemporal ? kShapeNontemporal : 0) |
+                 (sh.schedule ? 0 : kShapeNoSchedule) |
+                 (sh.xcd_rows ? kShapeXcdRows : 0);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
+                                             const scl_real_t* X_dev,
+                                             scl_real_t* Y_dev,
+                                             scl_index_t n_nodes,
+                                             scl_index_t n_features,
+                                             int transpose, void* stream,
+                                             float* best_ms) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(X_dev, "X is null");
+    SCL_API_CHECK_NULL(Y_dev, "Y is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev,
+                  SCL_ERROR_INVALID_ARGUMENT, "X and Y must be distinct");
+    SCL_API_TRY
+        check_operator(transition, n_nodes, n_features);
+        const int dev = current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        std::lock_guard<std::mutex> lk(transition->mu);
+        DeviceCSR& d = device_csr(transition, dev);
+        if (transpose) ensure_transpose(transition, d, s);
+        const CsrView v = transpose ? transposed_view_of(d) : view_of(d);
+        const auto cands = spmm_candidate_shapes(n_features, X_dev, Y_dev,
+                                                 v.sched != nullptr);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        float best = -1.0f;
+        SpmmShape best_shape = cands.front();
+        try {
+            for (const SpmmShape& c : cands) {
+                launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);  // warm
+                hip_check(hipEventRecord(e0, s), "event");
+                for (int r = 0; r < 2; ++r) launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);
+                hip_check(hipEventRecord(e1, s), "event");
+                hip_check(hipEventSynchronize(e1), "event");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    best_shape = c;
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (transpose ? d.t_shapes : d.shapes)[n_features] = best_shape;
+        if (best_ms) *best_ms = best / 2.0f;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

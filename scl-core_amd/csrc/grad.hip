@@ -1,0 +1,229 @@
+// grad.hip -- gfx950 kernels of the reverse-mode diffusion.
+//
+// For L(Y), Y = X_t, X_s = P X_{s-1}, the backward pass needs per step s:
+//   G_{s-1} = P^T G_s                          (adjoint SpMM)
+//   dP_ij  += <G_s[i,:], X_{s-1}[j,:]>         (edge gradient, per nonzero)
+// Both read G_s[i,:] for every nonzero (i, j).  Walking the transposed
+// pattern (one slot per column j, entries = source rows i in increasing
+// order), one gather of G_s[i, slab] feeds both: the adjoint accumulates
+// p_ij * G_s[i, :] into G_{s-1}[j, :] (fp32 FMA, stored order), and the edge
+// term is the dot product of that same gathered row with X_{s-1}[j, :],
+// which is loaded once per column.  The dot product is reduced across the
+// lanes of the row's group with xor shuffles and written once per nonzero,
+// in transposed order (dPT[p]); the host driver permutes dPT to CSR order
+// once at the end of the backward pass.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace scl_amd {
+
+using namespace dev;
+
+namespace {
+
+__device__ __forceinline__ float dot4(f4 a, f4 b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+__device__ __forceinline__ float dotv(f4 a, f4 b) { return dot4(a, b); }
+__device__ __forceinline__ float dotv(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ float dotv(float a, float b) { return a * b; }
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int m = G / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+// One backward step over a feature range [f0, f0 + Fp) of rows of stride F.
+// NC chunks of G*VEC features per lane group cover the range.
+template <int VEC, int G, int U, int NC, bool EDGE>
+__global__ __launch_bounds__(kBlock) void k_adjoint_edge(
+    const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
+    const float* __restrict__ t_val, const int32_t* __restrict__ sched,
+    const float* __restrict__ Gs, const float* __restrict__ Xprev,
+    float* __restrict__ Gp, float* __restrict__ dPT, int accumulate, int32_t n,
+    int32_t F, int32_t f0, int32_t Fp) {
+    using VT = typename Vec<VEC>::T;
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = (kBlock / 64) * RPW;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = lane / G, li = lane % G;
+    const int rr = L * RPB + wave * RPW + grp;
+    if (rr >= n) return;
+    int row = sched ? sched[rr] : rr;
+    if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int beg = t_row_ptr[row], end = t_row_ptr[row + 1];
+
+    bool act[NC];
+    int off[NC];  // feature offset of chunk c for this lane (0 when inactive)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int f = c * (G * VEC) + li * VEC;
+        act[c] = f < Fp;
+        off[c] = act[c] ? f0 + f : 0;
+    }
+    VT xj[NC];
+    if constexpr (EDGE) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            xj[c] = *reinterpret_cast<const VT*>(Xprev + (size_t)row * F + off[c]);
+            if (!act[c]) xj[c] = VT{};
+        }
+    }
+    float acc[NC][VEC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[c][e] = 0.0f;
+
+    for (int k = beg; k < end; k += U) {
+        int i[U];
+        float w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u, end - 1);
+            i[u] = t_col[kk];
+            w[u] = t_val[kk];
+        }
+        VT g[U][NC];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                g[u][c] = *reinterpret_cast<const VT*>(Gs + (size_t)i[u] * F + off[c]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k + u < end) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) fma_vec(w[u], g[u][c], acc[c]);
+            }
+        }
+        if constexpr (EDGE) {
+            float d[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float s = 0.0f;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) s += dotv(g[u][c], xj[c]);
+                d[u] = group_sum<G>(s);
+            }
+            if (li == 0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (k + u < end) {
+                        float* p = dPT + (k + u);
+                        *p = accumulate ? *p + d[u] : d[u];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        if (act[c]) store_vec<VEC, false>(Gp + (size_t)row * F + off[c], acc[c]);
+}
+
+using AdjFn = void (*)(const int32_t*, const int32_t*, const float*, const int32_t*,
+                       const float*, const float*, float*, float*, int, int32_t,
+                       int32_t, int32_t, int32_t);
+
+constexpr int kAdjU = 8;
+
+template <int VEC, int G, int NC>
+AdjFn pick_edge(bool edge) {
+    return edge ? k_adjoint_edge<VEC, G, kAdjU, NC, true>
+                : k_adjoint_edge<VEC, G, kAdjU, NC, false>;
+}
+
+template <int VEC>
+AdjFn pick_adj(int group, int nc, bool edge) {
+    if (group < 64) {
+        switch (group) {
+            case 1: return pick_edge<VEC, 1, 1>(edge);
+            case 2: return pick_edge<VEC, 2, 1>(edge);
+            case 4: return pick_edge<VEC, 4, 1>(edge);
+            case 8: return pick_edge<VEC, 8, 1>(edge);
+            case 16: return pick_edge<VEC, 16, 1>(edge);
+            default: return pick_edge<VEC, 32, 1>(edge);
+        }
+    }
+    switch (nc) {
+        case 1: return pick_edge<VEC, 64, 1>(edge);
+        case 2: return pick_edge<VEC, 64, 2>(edge);
+        case 4: return pick_edge<VEC, 64, 4>(edge);
+        default: return pick_edge<VEC, 64, 8>(edge);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const int32_t* __restrict__ idx,
+                                                    const float* __restrict__ src,
+                                                    float* __restrict__ dst, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) dst[idx[p]] = src[p];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ idx,
+                                                   const float* __restrict__ src,
+                                                   float* __restrict__ dst, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) dst[p] = src[idx[p]];
+}
+
+}  // namespace
+
+void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
+                         float* Gp, float* dPT, bool accumulate, int64_t F,
+                         hipStream_t s) {
+    if (PT.n <= 0 || F <= 0) return;
+    const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
+                   reinterpret_cast<uintptr_t>(Xprev);
+    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
+    const bool edge = Xprev != nullptr && dPT != nullptr;
+    // feature passes of at most 8 chunks of 64 lanes; the edge term of the
+    // later passes accumulates onto the first pass's value
+    const int64_t per_pass = (int64_t)64 * vec * 8;
+    for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
+        const int64_t Fp = std::min<int64_t>(per_pass, F - f0);
+        const int64_t lanes = (Fp + vec - 1) / vec;
+        int group = 1;
+        while (group < lanes && group < 64) group <<= 1;
+        int nc = 1;
+        if (group == 64)
+            while ((int64_t)nc * 64 < lanes) nc <<= 1;
+        const int rpb = (kBlock / 64) * (64 / group);
+        const int64_t nb = (PT.n + rpb - 1) / rpb;
+        AdjFn fn = vec == 4 ? pick_adj<4>(group, nc, edge)
+                 : vec == 2 ? pick_adj<2>(group, nc, edge)
+                            : pick_adj<1>(group, nc, edge);
+        hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(kBlock), 0, s, PT.row_ptr,
+                           PT.col, PT.val, PT.sched, Gs, Xprev, Gp, dPT,
+                           (accumulate || f0 > 0) ? 1 : 0, PT.n, (int32_t)F,
+                           (int32_t)f0, (int32_t)Fp);
+        check_launch("adjoint launch");
+    }
+}
+
+void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                        hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, idx, src, dst, m);
+    check_launch("scatter launch");
+}
+
+void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                       hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, idx, src, dst, m);
+    check_launch("gather launch");
+}
+
+}  // namespace scl_amd

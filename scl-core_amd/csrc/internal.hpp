@@ -6,6 +6,7 @@
 // message, and success clears the last error.
 #pragma once
 
+#include "kernels.hpp"
 #include "scl_core.h"
 #include "scl_sparse.h"
 
@@ -79,6 +80,7 @@ struct DeviceCSR {
     int32_t* col = nullptr;      // nnz
     float* val = nullptr;        // nnz
     int32_t* sched = nullptr;    // n: row visiting order (locality), may be null
+    uint64_t fingerprint = 0;    // pattern the schedule was built for
     // transposed pattern (for adjoints): t_row_ptr (n_sec + 1), t_col, t_val,
     // t_src[p] = position in (col, val) of transposed entry p
     uint64_t t_generation = ~uint64_t(0);
@@ -87,6 +89,9 @@ struct DeviceCSR {
     float* t_val = nullptr;
     int32_t* t_src = nullptr;
     int32_t* t_sched = nullptr;
+    // launch shapes chosen per feature count (autotuned or set by the
+    // caller) for the operator and for its transpose
+    std::map<int64_t, SpmmShape> shapes, t_shapes;
 
     ~DeviceCSR();
     void release_transpose();
@@ -128,6 +133,14 @@ hipStream_t library_stream(int device);
 std::mutex& device_mutex(int device);
 // Grow-only device buffer `slot` of a device, at least `bytes` long.
 void* device_workspace(int device, int slot, size_t bytes);
+
+// Operators with at least this many rows get a locality schedule.
+constexpr int64_t kScheduleMinRows = 16384;
+
+// Cuthill-McKee visiting order of a CSR pattern, sorted by decreasing row
+// length within windows of 64 rows (schedule.cpp).
+template <typename I>
+std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 
 // Device copy of a handle, refreshed when its generation changed.
 // Caller holds h->mu.

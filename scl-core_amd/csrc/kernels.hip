@@ -18,6 +18,7 @@
 // No MFMA: the contraction is sparse.
 #include <hip/hip_runtime.h>
 
+#include "device_common.hpp"
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -25,69 +26,32 @@
 
 namespace scl_amd {
 
+using namespace dev;
+
 namespace {
-
-constexpr int kBlock = 256;
-constexpr int kNumXcd = 8;
-
-using f2 = float __attribute__((ext_vector_type(2)));
-using f4 = float __attribute__((ext_vector_type(4)));
-
-template <int VEC> struct Vec;
-template <> struct Vec<1> { using T = float; };
-template <> struct Vec<2> { using T = f2; };
-template <> struct Vec<4> { using T = f4; };
-
-__device__ __forceinline__ void fma_vec(float a, float x, float (&acc)[1]) {
-    acc[0] = __builtin_fmaf(a, x, acc[0]);
-}
-__device__ __forceinline__ void fma_vec(float a, f2 x, float (&acc)[2]) {
-    acc[0] = __builtin_fmaf(a, x.x, acc[0]);
-    acc[1] = __builtin_fmaf(a, x.y, acc[1]);
-}
-__device__ __forceinline__ void fma_vec(float a, f4 x, float (&acc)[4]) {
-    acc[0] = __builtin_fmaf(a, x.x, acc[0]);
-    acc[1] = __builtin_fmaf(a, x.y, acc[1]);
-    acc[2] = __builtin_fmaf(a, x.z, acc[2]);
-    acc[3] = __builtin_fmaf(a, x.w, acc[3]);
-}
-
-template <int VEC, bool NT>
-__device__ __forceinline__ void store_vec(float* p, const float (&acc)[VEC]) {
-    if constexpr (VEC == 1) {
-        if constexpr (NT) __builtin_nontemporal_store(acc[0], p);
-        else *p = acc[0];
-    } else if constexpr (VEC == 2) {
-        f2 v = {acc[0], acc[1]};
-        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f2*>(p));
-        else *reinterpret_cast<f2*>(p) = v;
-    } else {
-        f4 v = {acc[0], acc[1], acc[2], acc[3]};
-        if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<f4*>(p));
-        else *reinterpret_cast<f4*>(p) = v;
-    }
-}
-
-// Hardware workgroup id -> logical id such that XCD x (which receives ids
-// x, x+8, x+16, ...) gets one contiguous range of logical ids.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-    const int x = b % kNumXcd, idx = b / kNumXcd;
-    const int q = nb / kNumXcd, r = nb % kNumXcd;
-    return x * q + (x < r ? x : r) + idx;
-}
 
 template <int VEC, int G, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const int32_t* __restrict__ sched,
     const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
-    int32_t n_rowblocks) {
+    int32_t n_rowblocks, int32_t rb_per_xcd) {
     using VT = typename Vec<VEC>::T;
     constexpr int RPW = 64 / G;
     constexpr int RPB = (kBlock / 64) * RPW;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int slab = L / n_rowblocks;
-    const int rb = L - slab * n_rowblocks;
+    int slab, rb;
+    if (rb_per_xcd > 0) {
+        // XCD x owns row blocks [x * rb_per_xcd, (x+1) * rb_per_xcd) and walks
+        // them slab by slab, so all XCDs work on the same slab at a time
+        const int x = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
+        slab = idx / rb_per_xcd;
+        rb = x * rb_per_xcd + (idx - slab * rb_per_xcd);
+        if (rb >= n_rowblocks) return;
+    } else {
+        const int L = xcd_remap(blockIdx.x, gridDim.x);
+        slab = L / n_rowblocks;
+        rb = L - slab * n_rowblocks;
+    }
     const int lane = threadIdx.x & 63;
     // wave index is uniform across the wave: keep it in a scalar register
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -98,44 +62,78 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
     const int f = slab * (G * VEC) + li * VEC;
     const bool active = f < F;
+    // lanes past the last feature gather a valid address (column 0 of the
+    // same rows) and store nothing, so the loop below has no branches
+    const float* Xf = X + (active ? f : 0);
     const int beg = row_ptr[row], end = row_ptr[row + 1];
-    const float* Xf = X + f;
 
     float acc[VEC];
 #pragma unroll
     for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
 
     int k = beg;
-    for (; k + U <= end; k += U) {
+    if (k + U <= end) {
+        // software pipeline: the (col, val) pairs of chunk k+U are fetched
+        // while the gathers of chunk k are in flight
         int j[U];
         float v[U];
-        VT x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             j[u] = col[k + u];
             v[u] = val[k + u];
         }
+        while (true) {
+            VT x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j[u] * F);
+            const int kn = k + U;
+            const bool more = kn + U <= end;
+            int jn[U];
+            float vn[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // clamped: always inside this (non-empty) row
+                const int kk = min(kn + u, end - 1);
+                jn[u] = col[kk];
+                vn[u] = val[kk];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) fma_vec(v[u], x[u], acc);
+            k = kn;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = jn[u];
+                v[u] = vn[u];
+            }
+        }
+    }
+    if (k < end) {
+        // tail of fewer than U nonzeros: all gathers issued at once, then the
+        // FMAs of the real entries in order
+        int j[U];
+        float v[U];
+        VT x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (active) x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j[u] * F);
-            else x[u] = VT{};
+            const int kk = min(k + u, end - 1);
+            j[u] = col[kk];
+            v[u] = val[kk];
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) fma_vec(v[u], x[u], acc);
-    }
-    for (; k < end; ++k) {
-        const int jj = col[k];
-        const float vv = val[k];
-        VT xx = VT{};
-        if (active) xx = *reinterpret_cast<const VT*>(Xf + (size_t)jj * F);
-        fma_vec(vv, xx, acc);
+        for (int u = 0; u < U; ++u)
+            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j[u] * F);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k + u < end) fma_vec(v[u], x[u], acc);
     }
     if (active) store_vec<VEC, NT>(Y + (size_t)row * F + f, acc);
 }
 
 using SpmmFn = void (*)(const int32_t*, const int32_t*, const float*,
                         const int32_t*, const float*, float*, int32_t, int32_t,
-                        int32_t);
+                        int32_t, int32_t);
 
 template <int VEC, int G, int U>
 SpmmFn pick_nt(bool nt) {
@@ -144,6 +142,7 @@ SpmmFn pick_nt(bool nt) {
 
 template <int VEC, int G>
 SpmmFn pick_u(int u, bool nt) {
+    if (u >= 16) return pick_nt<VEC, G, 16>(nt);
     return u >= 8 ? pick_nt<VEC, G, 8>(nt) : pick_nt<VEC, G, 4>(nt);
 }
 
@@ -174,13 +173,16 @@ int pow2_ceil(int64_t x) {
     return p;
 }
 
-void check_launch(const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
-}
 
 }  // namespace
+
+bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* Y) {
+    const auto a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
+    if (!(s.vec == 1 || s.vec == 2 || s.vec == 4)) return false;
+    if (F % s.vec != 0 || a % (4 * (uintptr_t)s.vec) != 0) return false;
+    if (s.group < 1 || s.group > 64 || (s.group & (s.group - 1)) != 0) return false;
+    return s.unroll == 4 || s.unroll == 8 || s.unroll == 16;
+}
 
 SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
     SpmmShape s;
@@ -191,7 +193,33 @@ SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
     s.group = pow2_ceil((F + s.vec - 1) / s.vec);
     s.unroll = 8;
     s.nontemporal = false;
+    s.schedule = true;
     return s;
+}
+
+std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const void* Y,
+                                             bool has_schedule) {
+    std::vector<SpmmShape> out;
+    const SpmmShape base = spmm_default_shape(F, X, Y);
+    const int gmax = base.group;
+    for (int g = 4; g <= 64; g <<= 1) {
+        if (g > gmax) break;
+        if (g < gmax / 16) continue;  // slabs far narrower than F: skip
+        for (int u : {4, 8, 16})
+            for (int nt = 0; nt < 2; ++nt)
+                for (int xr = 0; xr < 2; ++xr)
+                    for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
+                        SpmmShape s = base;
+                        s.group = g;
+                        s.unroll = u;
+                        s.nontemporal = nt != 0;
+                        s.xcd_rows = xr != 0;
+                        s.schedule = sc != 0;
+                        out.push_back(s);
+                    }
+    }
+    if (out.empty()) out.push_back(base);
+    return out;
 }
 
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
@@ -201,11 +229,13 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
     const int64_t n_slabs = (F + (int64_t)shape.group * shape.vec - 1) /
                             ((int64_t)shape.group * shape.vec);
-    const int64_t nb = n_rowblocks * n_slabs;
+    const int64_t rb_per_xcd = shape.xcd_rows ? (n_rowblocks + kNumXcd - 1) / kNumXcd : 0;
+    const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : n_rowblocks * n_slabs;
     if (nb > INT32_MAX) throw std::runtime_error("spmm grid too large");
+    const int32_t* sched = shape.schedule ? P.sched : nullptr;
     hipLaunchKernelGGL(pick_spmm(shape), dim3((unsigned)nb), dim3(kBlock), 0, s,
-                       P.row_ptr, P.col, P.val, P.sched, X, Y, P.n, (int32_t)F,
-                       (int32_t)n_rowblocks);
+                       P.row_ptr, P.col, P.val, sched, X, Y, P.n, (int32_t)F,
+                       (int32_t)n_rowblocks, (int32_t)rb_per_xcd);
     check_launch("spmm launch");
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace scl_amd {
 
@@ -22,16 +23,45 @@ struct CsrView {
 struct SpmmShape {
     int vec = 4;        // floats per lane per access (1, 2 or 4)
     int group = 64;     // lanes sharing one row (power of two, 1..64)
-    int unroll = 8;     // nonzeros in flight per lane group
-    bool nontemporal = false;
+    int unroll = 8;     // nonzeros in flight per lane group (4, 8 or 16)
+    bool nontemporal = false;  // streaming (non-temporal) stores of Y
+    bool schedule = true;      // visit rows in the operator's locality order
+    bool xcd_rows = false;     // each XCD owns a row range (else a slab range)
 };
+
+// Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
+constexpr int kShapeNontemporal = 1;
+constexpr int kShapeNoSchedule = 2;
+constexpr int kShapeXcdRows = 4;
+
+// True when the shape can run on F features with these X / Y pointers.
+bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* Y);
 
 // Picks vec/group for a feature count and the alignment of X and Y.
 SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y);
 
+// Candidate shapes the autotuner times for F features.
+std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const void* Y,
+                                             bool has_schedule);
+
 // One step Y = P X (Y[i,:] = sum over row i, stored order, FMA in fp32).
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
                      const SpmmShape& shape, hipStream_t s);
+
+// One fused backward step over the transposed pattern PT (rows = columns j
+// of P, entries = source rows i in increasing order, values p_ij):
+//   Gp[j, :] = sum_i p_ij Gs[i, :]               (fp32 FMA, stored order)
+//   dPT[p]  (+)= <Gs[i, :], Xprev[j, :]>          (when Xprev and dPT are set)
+// dPT is indexed by transposed position; accumulate adds onto its contents.
+void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
+                         float* Gp, float* dPT, bool accumulate, int64_t F,
+                         hipStream_t s);
+
+// dst[idx[p]] = src[p] and dst[p] = src[idx[p]] for p < m.
+void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                        hipStream_t s);
+void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                       hipStream_t s);
 
 // FP64 degree normalisation of values in place: deg[i] = sum_k a_ik (FP64,
 // stored order); symmetric == 0: a_ik <- (float)(a_ik / deg[i]) (0 if

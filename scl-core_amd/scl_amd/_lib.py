@@ -82,6 +82,18 @@ SIGNATURES = {
         c_err, [P, P, P, P, c_index, c_index, c_index, P]),
     "scl_gpu_spmm_device": (c_err, [P, P, P, c_index, c_index, ctypes.c_int, P]),
     "scl_gpu_compute_transition_matrix_ex": (c_err, [P, ctypes.c_int, P, P]),
+    "scl_gpu_spmm_set_shape": (
+        c_err, [P, c_index, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                ctypes.c_int]),
+    "scl_gpu_spmm_get_shape": (c_err, [P, c_index, ctypes.c_int, P, P, P, P]),
+    "scl_gpu_spmm_autotune": (c_err, [P, P, P, c_index, c_index, ctypes.c_int, P, P]),
+    "scl_gpu_sparse_set_values_device": (c_err, [P, P, P]),
+    "scl_gpu_diffuse_forward_ckpt_device": (
+        c_err, [P, P, P, P, P, c_index, c_index, c_index, c_index, P]),
+    "scl_gpu_diffuse_backward_workspace": (
+        c_err, [P, c_index, c_index, c_index, ctypes.c_int, ctypes.POINTER(c_size)]),
+    "scl_gpu_diffuse_backward_device": (
+        c_err, [P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
 }
 
 _lock = threading.Lock()
@@ -112,6 +124,15 @@ def load() -> ctypes.CDLL:
             raise ImportError(
                 f"{path} is missing: build it with `make -C scl-core_amd` "
                 "(or __graft_entry__.build())")
+        # PyTorch's ROCm wheel ships its own libamdhip64.so (same SONAME as
+        # /opt/rocm's).  If this library were loaded first, torch would later
+        # load its copy under another file name and the process would hold two
+        # HIP runtimes, which do not share devices or memory.  Importing torch
+        # first makes the dynamic linker bind this library to torch's runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(os.fspath(path))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

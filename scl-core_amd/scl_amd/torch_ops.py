@@ -5,6 +5,8 @@ enqueued on torch's current stream, so it orders with surrounding torch ops.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 from . import _lib
@@ -34,6 +36,38 @@ def diffuse(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> torch.Tensor:
             None if work is None else work.data_ptr(), n, F, n_steps,
             _stream_ptr(X)))
     return Y
+
+
+def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
+             ) -> float:
+    """Picks the fastest SpMM launch shape for P and X's width; returns ms/launch."""
+    _check_dense(X)
+    n, F = X.shape
+    Y = torch.empty_like(X)
+    best = ctypes.c_float(0.0)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().scl_gpu_spmm_autotune(
+            P.handle, X.data_ptr(), Y.data_ptr(), n, F, 1 if transpose else 0,
+            _stream_ptr(X), ctypes.addressof(best)))
+    return best.value
+
+
+def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False) -> dict:
+    v, g, u, f = (ctypes.c_int() for _ in range(4))
+    _lib.check(_lib.load().scl_gpu_spmm_get_shape(
+        P.handle, n_features, 1 if transpose else 0, ctypes.addressof(v),
+        ctypes.addressof(g), ctypes.addressof(u), ctypes.addressof(f)))
+    return {"vec": v.value, "group": g.value, "unroll": u.value,
+            "nontemporal": bool(f.value & 1), "schedule": not (f.value & 2),
+            "xcd_rows": bool(f.value & 4)}
+
+
+def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
+              unroll: int = 8, nontemporal: bool = False, schedule: bool = True,
+              transpose: bool = False, xcd_rows: bool = False) -> None:
+    flags = (1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
+    _lib.check(_lib.load().scl_gpu_spmm_set_shape(
+        P.handle, n_features, 1 if transpose else 0, vec, group, unroll, flags))
 
 
 def spmm(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False

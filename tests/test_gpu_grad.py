@@ -1,0 +1,99 @@
+"""GPU parity of the reverse-mode diffusion (dX, per-nonzero dP) against the
+FP64 oracle gradient (oracle/oracle.c orc_grad_f64, itself pinned by finite
+differences in test_oracle.py) and against central finite differences.
+
+Tolerance: relative error <= 1e-3 in the reference's max-norm convention
+(fp32 gradients of an fp32 forward vs the FP64 gradient)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from scl_amd import CsrMatrix, autograd, synth, torch_ops
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(n, k, F, seed, knn=True):
+    g = (synth.knn_graph(n, k, seed=seed) if knn else synth.uniform_knn(n, k, seed=seed)).normalized()
+    X = synth.normal(n, F, seed=seed)
+    T = synth.normal(n, F, seed=seed + 100)
+    return g, X, T
+
+
+@pytest.mark.parametrize("n,k,F,t,every", [
+    (3000, 10, 16, 3, 2), (4000, 8, 128, 6, 2), (2500, 6, 50, 4, 3),
+    (2000, 5, 3, 2, 1), (5000, 7, 300, 5, 2), (1500, 12, 1100, 2, 2)])
+def test_gradients_match_fp64(gpu, n, k, F, t, every):
+    g, X, T = _problem(n, k, F, seed=n + F)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev).requires_grad_(True)
+    Td = torch.from_numpy(T).to(dev)
+    vals = torch.from_numpy(g.data.copy()).to(dev).requires_grad_(True)
+    Y = autograd.diffuse(P, Xd, t, values=vals, checkpoint_every=every)
+    L = 0.5 * ((Y - Td) ** 2).sum()
+    L.backward()
+    L64, Y64, dX64, dP64 = orc.grad_f64(g.indptr, g.indices, g.data, X, T, t)
+    # forward identical to the plain kernel
+    assert torch.equal(Y.detach(), torch_ops.diffuse(P, Xd.detach(), t))
+    ex = orc.rel_err(Xd.grad.cpu().numpy(), dX64)
+    ep = orc.rel_err(vals.grad.cpu().numpy(), dP64)
+    print(f"\nn={n} F={F} t={t} every={every}: dX err {ex:.2e}, dP err {ep:.2e}")
+    assert ex <= 1e-3 and ep <= 1e-3
+
+
+def test_dx_only_and_constant_operator(gpu):
+    g, X, T = _problem(3000, 8, 32, seed=5)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (3000, 3000))
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Y = autograd.diffuse(P, Xd, 4)
+    (0.5 * ((Y - torch.from_numpy(T).cuda()) ** 2).sum()).backward()
+    _, _, dX64, _ = orc.grad_f64(g.indptr, g.indices, g.data, X, T, 4)
+    assert orc.rel_err(Xd.grad.cpu().numpy(), dX64) <= 1e-3
+    # the adjoint of the adjoint: dX of <P^t X, G> is (P^T)^t G exactly as the
+    # transposed SpMM computes it step by step
+    G = torch.from_numpy(T).cuda()
+    Xd.grad = None
+    Y = autograd.diffuse(P, Xd, 2, checkpoint_every=1)
+    (Y * G).sum().backward()
+    ref = torch_ops.spmm(P, torch_ops.spmm(P, G, transpose=True), transpose=True)
+    assert torch.equal(Xd.grad, ref)
+
+
+def test_finite_differences(gpu):
+    """Central differences of the FP64 loss (eps 1e-3) at 32 entries of X and
+    32 nonzeros of P, against the GPU gradient (FP32)."""
+    n, F, t = 2000, 8, 6
+    g, X, T = _problem(n, 8, F, seed=33)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    vals = torch.from_numpy(g.data.copy()).cuda().requires_grad_(True)
+    Y = autograd.diffuse(P, Xd, t, values=vals, checkpoint_every=2)
+    (0.5 * ((Y - torch.from_numpy(T).cuda()) ** 2).sum()).backward()
+    gX, gP = Xd.grad.cpu().numpy(), vals.grad.cpu().numpy()
+    rng = np.random.Generator(np.random.PCG64(3))
+    eps = 1e-3
+    v64 = g.data.astype(np.float64)
+    X64 = X.astype(np.float64)
+    errs = []
+    for _ in range(32):
+        i, c = rng.integers(n), rng.integers(F)
+        Xp, Xm = X64.copy(), X64.copy()
+        Xp[i, c] += eps
+        Xm[i, c] -= eps
+        fd = (orc.loss_f64(g.indptr, g.indices, v64, Xp, T, t) -
+              orc.loss_f64(g.indptr, g.indices, v64, Xm, T, t)) / (2 * eps)
+        errs.append(abs(fd - gX[i, c]) / max(1e-2, abs(fd)))
+    for _ in range(32):
+        kk = rng.integers(len(v64))
+        vp, vm = v64.copy(), v64.copy()
+        vp[kk] += eps
+        vm[kk] -= eps
+        fd = (orc.loss_f64(g.indptr, g.indices, vp, X64, T, t) -
+              orc.loss_f64(g.indptr, g.indices, vm, X64, T, t)) / (2 * eps)
+        errs.append(abs(fd - gP[kk]) / max(1e-2, abs(fd)))
+    print(f"\nfinite differences: max rel deviation {max(errs):.2e}")
+    assert max(errs) < 1e-3
