@@ -107,6 +107,21 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
                                              int transpose, void* stream,
                                              float* best_ms);
 
+/* ----------------------------------------------------------- bf16 storage --
+ * Y_dev = P^t X_dev (both fp32) with the t-1 intermediate iterates stored as
+ * bf16 (round to nearest even) and fp32 accumulation: the first step gathers
+ * the fp32 input, the last writes fp32.  flags:
+ *   SCL_GPU_BF16_VALUES       read P's values rounded to bf16 (else fp32)
+ *   SCL_GPU_BF16_NO_SCHEDULE  natural row order
+ * n_features must be even; work_dev holds 2 planes of n x F bf16
+ * (n * F * 4 bytes) and is required when n_steps >= 2. */
+#define SCL_GPU_BF16_VALUES 1
+#define SCL_GPU_BF16_NO_SCHEDULE 2
+SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_bf16_device(
+    scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
+    void* work_dev, scl_index_t n_nodes, scl_index_t n_features,
+    scl_index_t n_steps, int flags, void* stream);
+
 /* ------------------------------------------------------------- gradients --
  * Reverse mode of Y = P^t X for a loss L(Y): given dY = dL/dY, returns
  *   dX = (P^T)^t dY   and   dP_k = sum_{s=1..t} <G_s[i,:], X_{s-1}[j,:]>

@@ -165,6 +165,46 @@ void orc_transition_f64(int64_t n, const int64_t* indptr,
     free(d);
 }
 
+/* ------------------------------------------------------- bf16 iterates -- */
+
+/* fp32 -> bf16 -> fp32, round to nearest even (NaN kept quiet) */
+static inline float orc_round_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        u = (u | 0x00400000u) & 0xffff0000u;
+    } else {
+        u = (u + (((u >> 16) & 1u) + 0x7fffu)) & 0xffff0000u;
+    }
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+float orc_bf16_round(float f) { return orc_round_bf16(f); }
+
+int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                         const int64_t* indices, const float* vals,
+                         int bf16_values, const float* X, float* Y) {
+    const size_t total = (size_t)n * (size_t)F;
+    const int64_t nnz = indptr[n];
+    float* v = (float*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(float));
+    float* a = (float*)malloc(total * sizeof(float));
+    float* b = (float*)malloc(total * sizeof(float));
+    if (!v || !a || !b) { free(v); free(a); free(b); return -1; }
+    for (int64_t k = 0; k < nnz; ++k) v[k] = bf16_values ? orc_round_bf16(vals[k]) : vals[k];
+    memcpy(a, X, total * sizeof(float));
+    for (int64_t s = 1; s <= t; ++s) {
+        orc_spmm_strided_f32(n, F, indptr, indices, v, a, b);
+        if (s < t)
+            for (size_t e = 0; e < total; ++e) b[e] = orc_round_bf16(b[e]);
+        float* tmp = a; a = b; b = tmp;
+    }
+    memcpy(Y, a, total * sizeof(float));
+    free(v); free(a); free(b);
+    return 0;
+}
+
 /* ---------------------------------------------------------------- FP64 -- */
 
 static void orc_spmm_f64(int64_t n, int64_t F, const int64_t* indptr,

@@ -74,6 +74,19 @@ void orc_transition_f64(int64_t n, const int64_t* indptr,
                         const int64_t* indices, float* vals, int symmetric,
                         double* deg);
 
+/* Reduced-precision restatement: Y = P^t X with the t-1 intermediate
+ * iterates rounded to bf16 (round to nearest even) and fp32 FMA accumulation
+ * in stored order; the input X and the output Y are fp32; P's values are
+ * rounded to bf16 first when bf16_values != 0.  This is the arithmetic the
+ * bf16 device path performs (the reference has no reduced-precision path;
+ * parity of this function is pinned by tests against the FP32 restatement
+ * and closed forms with bf16-exact data). */
+int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                         const int64_t* indices, const float* vals,
+                         int bf16_values, const float* X, float* Y);
+/* fp32 -> nearest-even bf16, returned as fp32. */
+float orc_bf16_round(float f);
+
 /* FP64 oracle: Y = P^t X with double values/accumulators, X and Y double. */
 int orc_diffuse_f64(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                     const int64_t* indices, const double* vals,

@@ -62,6 +62,8 @@ def load():
         "orc_rel_err_f32_f32": (D, [SZ, P, P]),
         "orc_median_rel_err_f32_f64": (D, [SZ, P, P]),
         "orc_num_threads": (ctypes.c_int, []),
+        "orc_diffuse_bf16_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int, P, P]),
+        "orc_bf16_round": (ctypes.c_float, [ctypes.c_float]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -105,6 +107,30 @@ def diffuse_matrix_f32(indptr, indices, vals, X, t):
     if rc:
         raise MemoryError("oracle allocation failed")
     return Y
+
+
+def diffuse_bf16_f32(indptr, indices, vals, X, t, bf16_values=False):
+    """bf16-iterate restatement (fp32 accumulate, fp32 in and out)."""
+    X = _f32(X)
+    n, F = X.shape
+    Y = np.empty_like(X)
+    ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    rc = load().orc_diffuse_bf16_f32(n, F, t, ip.ctypes.data, ix.ctypes.data,
+                                     v.ctypes.data, 1 if bf16_values else 0,
+                                     X.ctypes.data, Y.ctypes.data)
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return Y
+
+
+def bf16_round(a):
+    """Nearest-even bf16 rounding of a float32 array (returned as float32)."""
+    a = _f32(a)
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + (((u >> 16) & 1) + 0x7FFF)) & 0xFFFF0000)
+    r = np.where(nan, (u | 0x00400000) & 0xFFFF0000, r)
+    return r.astype(np.uint32).view(np.float32)
 
 
 def diffuse_vector_f32(indptr, indices, vals, x, t):

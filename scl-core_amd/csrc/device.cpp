@@ -91,6 +91,7 @@ DeviceCSR::~DeviceCSR() {
     dev_free(col);
     dev_free(val);
     dev_free(sched);
+    dev_free(val_bf16);
     release_transpose();
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
 }

@@ -89,6 +89,9 @@ struct DeviceCSR {
     float* t_val = nullptr;
     int32_t* t_src = nullptr;
     int32_t* t_sched = nullptr;
+    // values rounded to bf16 (built on first use, for the current values)
+    uint16_t* val_bf16 = nullptr;
+    uint64_t bf16_generation = ~uint64_t(0);
     // launch shapes chosen per feature count (autotuned or set by the
     // caller) for the operator and for its transpose
     std::map<int64_t, SpmmShape> shapes, t_shapes;

@@ -63,6 +63,17 @@ void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_
 void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
                        hipStream_t s);
 
+// Input / output storage of one reduced-precision SpMM step.
+enum class LowpIo { F32_TO_BF16, BF16_TO_BF16, BF16_TO_F32, F32_TO_F32 };
+
+// One step Y = P X with fp32 accumulation, X and Y stored as the io pair
+// says, P's values read from vals_bf16 when not null (else P.val, fp32).
+// F must be even; rows stored in stored order, as in launch_spmm_f32.
+void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
+                      const void* X, void* Y, int64_t F, bool use_sched, hipStream_t s);
+void launch_to_bf16(const float* src, uint16_t* dst, int64_t m, hipStream_t s);
+void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s);
+
 // FP64 degree normalisation of values in place: deg[i] = sum_k a_ik (FP64,
 // stored order); symmetric == 0: a_ik <- (float)(a_ik / deg[i]) (0 if
 // deg <= 1e-15); symmetric != 0: a_ik <- (float)(a_ik * r_i * r_j),

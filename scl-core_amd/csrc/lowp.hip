@@ -1,0 +1,284 @@
+// lowp.hip -- reduced-precision storage for the diffusion (gfx950).
+//
+// Y = P^t X with the intermediate iterates X_1 .. X_{t-1} stored as bf16
+// (half the bytes of every gather and of every iterate write), fp32
+// accumulation, and fp32 at both ends: the first step gathers the fp32 input
+// and the last step writes fp32.  P's values are read either as fp32 or as
+// bf16.  Every gathered element is widened to fp32 exactly (bf16 -> fp32 is
+// a 16-bit shift), accumulated with fp32 FMAs in stored order, and the row
+// result is rounded to bf16 once (round to nearest even) when the step's
+// output is an iterate.
+//
+// Launch layout is the row-split one of kernels.hip: a group of G lanes owns
+// a row, each lane E consecutive features (one 4/8/16-byte load per
+// nonzero), rows visited in the operator's locality order, workgroups
+// remapped so each XCD walks a contiguous slab range.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace scl_amd {
+
+using namespace dev;
+
+namespace {
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+    return __uint_as_float((uint32_t)h << 16);
+}
+
+// round to nearest even; NaN stays NaN (quiet bit kept)
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    const uint32_t r = ((u >> 16) & 1u) + 0x7fffu;
+    return (uint16_t)((u + r) >> 16);
+}
+
+template <int BYTES> struct Raw;
+template <> struct Raw<4> { using T = uint32_t; };
+template <> struct Raw<8> { using T = uint32_t __attribute__((ext_vector_type(2))); };
+template <> struct Raw<16> { using T = uint32_t __attribute__((ext_vector_type(4))); };
+
+// Loads E elements of type IT starting at p and widens them to fp32.
+template <typename IT, int E>
+__device__ __forceinline__ void load_wide(const IT* p, float (&x)[E]) {
+    constexpr int B = E * (int)sizeof(IT);
+    using R = typename Raw<B>::T;
+    const R r = *reinterpret_cast<const R*>(p);
+    uint32_t w[B / 4];
+    if constexpr (B == 4) {
+        w[0] = r;
+    } else {
+#pragma unroll
+        for (int q = 0; q < B / 4; ++q) w[q] = r[q];
+    }
+    if constexpr (sizeof(IT) == 4) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = __uint_as_float(w[e]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            x[e] = bf16_to_f32((uint16_t)(w[e / 2] >> (16 * (e % 2))));
+    }
+}
+
+template <typename OT, int E>
+__device__ __forceinline__ void store_narrow(OT* p, const float (&a)[E]) {
+    constexpr int B = E * (int)sizeof(OT);
+    using R = typename Raw<B>::T;
+    uint32_t w[B / 4];
+    if constexpr (sizeof(OT) == 4) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) w[e] = __float_as_uint(a[e]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < B / 4; ++q)
+            w[q] = (uint32_t)f32_to_bf16(a[2 * q]) | ((uint32_t)f32_to_bf16(a[2 * q + 1]) << 16);
+    }
+    R r;
+    if constexpr (B == 4) {
+        r = w[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < B / 4; ++q) r[q] = w[q];
+    }
+    *reinterpret_cast<R*>(p) = r;
+}
+
+template <typename VT>
+__device__ __forceinline__ float value_of(const VT* v, int k) {
+    if constexpr (sizeof(VT) == 4) return v[k];
+    else return bf16_to_f32(v[k]);
+}
+
+template <typename IT, typename OT, typename VT, int E, int G, int U>
+__global__ __launch_bounds__(kBlock) void k_spmm_mixed(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const VT* __restrict__ val, const int32_t* __restrict__ sched,
+    const IT* __restrict__ X, OT* __restrict__ Y, int32_t n, int32_t F,
+    int32_t n_rowblocks) {
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = (kBlock / 64) * RPW;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int slab = L / n_rowblocks;
+    const int rb = L - slab * n_rowblocks;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = lane / G, li = lane % G;
+    const int rr = rb * RPB + wave * RPW + grp;
+    if (rr >= n) return;
+    int row = sched ? sched[rr] : rr;
+    if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int f = slab * (G * E) + li * E;
+    const bool active = f < F;
+    const IT* Xf = X + (active ? f : 0);
+    const int beg = row_ptr[row], end = row_ptr[row + 1];
+
+    float acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) acc[e] = 0.0f;
+
+    int k = beg;
+    if (k + U <= end) {
+        int j[U];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            j[u] = col[k + u];
+            v[u] = value_of(val, k + u);
+        }
+        while (true) {
+            float x[U][E];
+#pragma unroll
+            for (int u = 0; u < U; ++u) load_wide<IT, E>(Xf + (size_t)j[u] * F, x[u]);
+            const int kn = k + U;
+            const bool more = kn + U <= end;
+            int jn[U];
+            float vn[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(kn + u, end - 1);
+                jn[u] = col[kk];
+                vn[u] = value_of(val, kk);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int e = 0; e < E; ++e) acc[e] = __builtin_fmaf(v[u], x[u][e], acc[e]);
+            k = kn;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = jn[u];
+                v[u] = vn[u];
+            }
+        }
+    }
+    if (k < end) {
+        int j[U];
+        float v[U];
+        float x[U][E];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u, end - 1);
+            j[u] = col[kk];
+            v[u] = value_of(val, kk);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) load_wide<IT, E>(Xf + (size_t)j[u] * F, x[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k + u < end)
+#pragma unroll
+                for (int e = 0; e < E; ++e) acc[e] = __builtin_fmaf(v[u], x[u][e], acc[e]);
+    }
+    if (active) store_narrow<OT, E>(Y + (size_t)row * F + f, acc);
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_bf16(const float* __restrict__ src,
+                                                    uint16_t* __restrict__ dst, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) dst[p] = f32_to_bf16(src[p]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_from_bf16(const uint16_t* __restrict__ src,
+                                                      float* __restrict__ dst, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) dst[p] = bf16_to_f32(src[p]);
+}
+
+constexpr int kLowpU = 8;
+
+template <typename IT, typename OT, typename VT>
+void launch_mixed(const CsrView& P, const VT* vals, const IT* X, OT* Y, int64_t F,
+                  bool use_sched, hipStream_t s) {
+    // elements per lane: one 16-byte load when the row is wide enough for
+    // 64 lanes, else narrower loads that still give every lane of the wave
+    // its own features
+    constexpr int wmax = sizeof(IT) > sizeof(OT) ? (int)sizeof(IT) : (int)sizeof(OT);
+    const int e_max = 16 / wmax;
+    const uintptr_t a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
+    int E = e_max;
+    while (E > 2 && (F % E != 0 || (a % (E * sizeof(IT))) != 0 ||
+                     (a % (E * sizeof(OT))) != 0 || (F / E) < 64))
+        E /= 2;
+    if (F % E != 0 || a % (E * sizeof(IT)) != 0 || a % (E * sizeof(OT)) != 0)
+        throw std::runtime_error("reduced-precision SpMM needs an even feature count "
+                                 "and aligned buffers");
+    int G = 1;
+    while (G < (F + E - 1) / E && G < 64) G <<= 1;
+    G = std::max(G, 16);
+    const int rpb = (kBlock / 64) * (64 / G);
+    const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
+    const int64_t n_slabs = (F + (int64_t)G * E - 1) / ((int64_t)G * E);
+    const int64_t nb = n_rowblocks * n_slabs;
+    const int32_t* sched = use_sched ? P.sched : nullptr;
+#define SCL_LAUNCH_MIXED(EE, GG)                                                       \
+    hipLaunchKernelGGL((k_spmm_mixed<IT, OT, VT, EE, GG, kLowpU>), dim3((unsigned)nb), \
+                       dim3(kBlock), 0, s, P.row_ptr, P.col, vals, sched, X, Y, P.n,   \
+                       (int32_t)F, (int32_t)n_rowblocks)
+#define SCL_LAUNCH_MIXED_G(EE)                       \
+    switch (G) {                                     \
+        case 16: SCL_LAUNCH_MIXED(EE, 16); break;    \
+        case 32: SCL_LAUNCH_MIXED(EE, 32); break;    \
+        default: SCL_LAUNCH_MIXED(EE, 64); break;    \
+    }
+    if (E == 8) {
+        if constexpr (16 / wmax >= 8) {
+            SCL_LAUNCH_MIXED_G(8);
+        }
+    } else if (E == 4) {
+        SCL_LAUNCH_MIXED_G(4);
+    } else {
+        SCL_LAUNCH_MIXED_G(2);
+    }
+#undef SCL_LAUNCH_MIXED_G
+#undef SCL_LAUNCH_MIXED
+    check_launch("bf16 spmm launch");
+}
+
+}  // namespace
+
+void launch_to_bf16(const float* src, uint16_t* dst, int64_t m, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, src, dst, m);
+    check_launch("bf16 convert launch");
+}
+
+void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_from_bf16, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, src, dst, m);
+    check_launch("bf16 convert launch");
+}
+
+void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
+                      const void* X, void* Y, int64_t F, bool use_sched, hipStream_t s) {
+    if (P.n <= 0 || F <= 0) return;
+    const bool vb = vals_bf16 != nullptr;
+    switch (io) {
+        case LowpIo::F32_TO_BF16:
+            if (vb) launch_mixed<float, uint16_t, uint16_t>(P, vals_bf16, (const float*)X, (uint16_t*)Y, F, use_sched, s);
+            else launch_mixed<float, uint16_t, float>(P, P.val, (const float*)X, (uint16_t*)Y, F, use_sched, s);
+            break;
+        case LowpIo::BF16_TO_BF16:
+            if (vb) launch_mixed<uint16_t, uint16_t, uint16_t>(P, vals_bf16, (const uint16_t*)X, (uint16_t*)Y, F, use_sched, s);
+            else launch_mixed<uint16_t, uint16_t, float>(P, P.val, (const uint16_t*)X, (uint16_t*)Y, F, use_sched, s);
+            break;
+        case LowpIo::BF16_TO_F32:
+            if (vb) launch_mixed<uint16_t, float, uint16_t>(P, vals_bf16, (const uint16_t*)X, (float*)Y, F, use_sched, s);
+            else launch_mixed<uint16_t, float, float>(P, P.val, (const uint16_t*)X, (float*)Y, F, use_sched, s);
+            break;
+        case LowpIo::F32_TO_F32:
+            if (vb) launch_mixed<float, float, uint16_t>(P, vals_bf16, (const float*)X, (float*)Y, F, use_sched, s);
+            else launch_mixed<float, float, float>(P, P.val, (const float*)X, (float*)Y, F, use_sched, s);
+            break;
+    }
+}
+
+}  // namespace scl_amd

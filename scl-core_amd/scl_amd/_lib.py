@@ -88,6 +88,8 @@ SIGNATURES = {
     "scl_gpu_spmm_get_shape": (c_err, [P, c_index, ctypes.c_int, P, P, P, P]),
     "scl_gpu_spmm_autotune": (c_err, [P, P, P, c_index, c_index, ctypes.c_int, P, P]),
     "scl_gpu_sparse_set_values_device": (c_err, [P, P, P]),
+    "scl_gpu_diffuse_matrix_bf16_device": (
+        c_err, [P, P, P, P, c_index, c_index, c_index, ctypes.c_int, P]),
     "scl_gpu_diffuse_forward_ckpt_device": (
         c_err, [P, P, P, P, P, c_index, c_index, c_index, c_index, P]),
     "scl_gpu_diffuse_backward_workspace": (

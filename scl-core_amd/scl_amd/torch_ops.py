@@ -38,6 +38,23 @@ def diffuse(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> torch.Tensor:
     return Y
 
 
+def diffuse_bf16(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
+                 bf16_values: bool = False, schedule: bool = True) -> torch.Tensor:
+    """Returns P^t X (fp32) with bf16 intermediate iterates and fp32 accumulation."""
+    _check_dense(X)
+    n, F = X.shape
+    Y = torch.empty_like(X)
+    work = torch.empty((2, n, F), dtype=torch.bfloat16, device=X.device) \
+        if n_steps >= 2 else None
+    flags = (1 if bf16_values else 0) | (0 if schedule else 2)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().scl_gpu_diffuse_matrix_bf16_device(
+            P.handle, X.data_ptr(), Y.data_ptr(),
+            None if work is None else work.data_ptr(), n, F, n_steps, flags,
+            _stream_ptr(X)))
+    return Y
+
+
 def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
              ) -> float:
     """Picks the fastest SpMM launch shape for P and X's width; returns ms/launch."""

@@ -1,0 +1,56 @@
+"""GPU parity of the bf16-iterate diffusion against its CPU restatement
+(oracle orc_diffuse_bf16_f32: the same fp32 FMA chains in stored order, the
+same nearest-even bf16 rounding of every intermediate iterate), bit for bit,
+plus the error against FP64 stated for the uniform15 workload shape."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from scl_amd import CsrMatrix, synth, torch_ops
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("F,t,vb", [(2, 3, False), (8, 2, True), (50, 4, False),
+                                    (64, 4, True), (256, 4, False), (256, 4, True),
+                                    (300, 3, False), (1000, 2, True), (128, 1, True)])
+def test_bf16_path_matches_restatement(gpu, F, t, vb):
+    n = 20_000 if F <= 256 else 5_000
+    g = synth.uniform_knn(n, 15, seed=F + t).normalized()
+    X = synth.normal(n, F, seed=F)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Y = torch_ops.diffuse_bf16(P, torch.from_numpy(X).cuda(), t, bf16_values=vb)
+    ref = orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, t, bf16_values=vb)
+    assert np.array_equal(Y.cpu().numpy(), ref)
+
+
+def test_bf16_error_vs_fp64_uniform15(gpu):
+    """n = 200,000 (the 1M-cell shape scaled down 5x), k = 15 uniform
+    neighbours, F = 256 N(0,1), t = 4."""
+    n, F, t = 200_000, 256, 4
+    g = synth.uniform_knn(n, 15, seed=4).normalized()
+    X = synth.normal(n, F, seed=4)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd = torch.from_numpy(X).cuda()
+    cols = np.arange(0, F, 16)
+    y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X[:, cols], t)
+    y64q = orc.diffuse_f64(g.indptr, g.indices, g.data, orc.bf16_round(X[:, cols]), t)
+    rows = []
+    for vb in (False, True):
+        Y = torch_ops.diffuse_bf16(P, Xd, t, bf16_values=vb)[:, torch.from_numpy(cols).cuda()]
+        Yh = Y.cpu().numpy()
+        rows.append((vb, orc.rel_err(Yh, y64), orc.median_rel_err(Yh, y64)))
+    Y32 = torch_ops.diffuse(P, Xd, t)[:, torch.from_numpy(cols).cuda()].cpu().numpy()
+    e32 = orc.rel_err(Y32, y64)
+    Xq = torch.from_numpy(orc.bf16_round(X)).cuda()
+    Yq = torch_ops.diffuse_bf16(P, Xq, t)[:, torch.from_numpy(cols).cuda()].cpu().numpy()
+    eq = orc.rel_err(Yq, y64q)
+    print(f"\nfp32 kernel vs FP64: {e32:.2e}")
+    for vb, emax, emed in rows:
+        print(f"bf16 iterates, {'bf16' if vb else 'fp32'} values: max {emax:.2e} median {emed:.2e}")
+    print(f"bf16 iterates on bf16-rounded input vs FP64 of that input: {eq:.2e}")
+    assert e32 <= 1e-5
+    assert all(emax < 1e-2 for _, emax, _ in rows)

@@ -233,3 +233,42 @@ def test_relative_error_convention():
     assert orc.rel_err(a, b) == pytest.approx(1.0 / 3.0)
     z = np.zeros(3, np.float32)
     assert orc.rel_err(z, np.full(3, 1e-16)) == pytest.approx(1e-16)
+
+
+def test_bf16_rounding_matches_library_rne():
+    import torch
+    rng = np.random.Generator(np.random.PCG64(17))
+    x = np.concatenate([rng.standard_normal(100_000).astype(np.float32),
+                        np.float32([1 + 2 ** -8, 1 + 3 * 2 ** -8, -1 - 2 ** -8, 0.0, -0.0,
+                                    np.inf, -np.inf, 3.4e38, 1e-40])])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    got = orc.bf16_round(x)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    one = [orc.load().orc_bf16_round(float(v)) for v in x[-9:-4]]
+    assert one == [1.0, 1 + 2 ** -6, -1.0, 0.0, -0.0]
+
+
+def test_bf16_restatement_reduces_to_fp32_when_nothing_rounds():
+    fx = load_fixture(GOLDEN / "impute_3x3.npz")  # 0.5 weights, small integers
+    y = orc.diffuse_bf16_f32(fx["indptr"], fx["indices"], fx["data"], fx["X"], 2,
+                             bf16_values=True)
+    assert np.array_equal(y, fx["expected_f32"])
+    g = synth.uniform_knn(800, 9, seed=2).normalized()
+    X = synth.normal(800, 6, seed=2)
+    # t = 1: no iterate is rounded
+    assert np.array_equal(orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, 1),
+                          orc.diffuse_matrix_f32(g.indptr, g.indices, g.data, X, 1))
+    # t = 2 by hand: round the first iterate, then an fp32 step
+    mid = orc.bf16_round(orc.spmm_step_f32(g.indptr, g.indices, g.data, X))
+    exp = orc.spmm_step_f32(g.indptr, g.indices, g.data, mid)
+    assert np.array_equal(orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, 2), exp)
+    # bf16 values: the same with P rounded first
+    pb = orc.bf16_round(g.data)
+    mid = orc.bf16_round(orc.spmm_step_f32(g.indptr, g.indices, pb, X))
+    exp = orc.spmm_step_f32(g.indptr, g.indices, pb, mid)
+    assert np.array_equal(orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, 2,
+                                               bf16_values=True), exp)
+    # and the error against FP64 stays within a few bf16 roundings
+    y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X, 4)
+    e = orc.rel_err(orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, 4), y64)
+    assert 1e-5 < e < 3 * 2.0 ** -8
