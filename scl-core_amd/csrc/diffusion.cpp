@@ -213,6 +213,19 @@ SCL_EXPORT scl_error_t scl_gpu_device_count(int* out) {
     SCL_API_RETURN_OK;
 }
 
+SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
+                                                 scl_index_t* order_out) {
+    SCL_API_CHECK_NULL(matrix, "Matrix is null");
+    SCL_API_CHECK_NULL(order_out, "Output pointer is null");
+    SCL_API_TRY
+        std::lock_guard<std::mutex> lk(matrix->mu);
+        const auto order = locality_schedule(matrix->primary(), matrix->indptr,
+                                             matrix->indices);
+        for (size_t i = 0; i < order.size(); ++i) order_out[i] = order[i];
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
 SCL_EXPORT scl_error_t scl_gpu_sparse_invalidate(scl_sparse_t matrix) {
     SCL_API_CHECK_NULL(matrix, "Matrix is null");
     SCL_API_TRY

@@ -32,7 +32,10 @@ int64_t bfs(int64_t start, const I* ptr, const I* idx, const std::vector<int64_t
         scratch.clear();
         for (I k = ptr[u]; k < ptr[u + 1]; ++k) {
             const int64_t v = idx[k];
-            if (seen[v] != mark) {
+            // mark 2 = already placed in the order: never walked again, so
+            // every node is traversed at most twice (probe + placement) even
+            // when the pattern is not symmetric
+            if (seen[v] != mark && seen[v] != 2) {
                 seen[v] = mark;
                 scratch.push_back((int32_t)v);
             }

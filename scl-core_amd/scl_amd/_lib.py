@@ -77,6 +77,7 @@ SIGNATURES = {
     # scl_diffusion_gpu.h
     "scl_gpu_device_count": (c_err, [ctypes.POINTER(ctypes.c_int)]),
     "scl_gpu_sparse_invalidate": (c_err, [P]),
+    "scl_sparse_locality_order": (c_err, [P, P]),
     "scl_gpu_sparse_prepare": (c_err, [P]),
     "scl_gpu_diffuse_matrix_device": (
         c_err, [P, P, P, P, c_index, c_index, c_index, P]),

@@ -98,6 +98,13 @@ class SparseMatrixBase:
         _lib.check(_lib.load().scl_sparse_clone(self.handle, ctypes.byref(out)))
         return type(self)(out)
 
+    def locality_order(self) -> np.ndarray:
+        """The row visiting order the device kernels use (host computation)."""
+        rows, cols = self.shape
+        order = np.empty(rows if self._is_csr else cols, np.int64)
+        _lib.check(_lib.load().scl_sparse_locality_order(self.handle, order.ctypes.data))
+        return order
+
     def prepare_device(self) -> None:
         """Uploads the operator to the current GPU now."""
         _lib.check(_lib.load().scl_gpu_sparse_prepare(self.handle))

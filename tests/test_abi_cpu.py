@@ -153,3 +153,27 @@ def test_no_device_reports_feature_unavailable():
     assert e.value.code == _lib.SCL_ERROR_FEATURE_UNAVAILABLE
     assert np.array_equal(X, np.ones((3, 2), np.float32))
     assert np.array_equal(P.export()[2], d)
+
+
+def test_locality_order_is_a_permutation_and_fast():
+    import time
+    from scl_amd import synth
+    for g in (synth.uniform_knn(200_000, 15, seed=4),      # not symmetric
+              synth.knn_graph(30_000, 10, seed=2)):         # symmetric
+        P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
+        t0 = time.perf_counter()
+        order = P.locality_order()
+        assert time.perf_counter() - t0 < 20
+        assert np.array_equal(np.sort(order), np.arange(g.n))
+        deg = np.diff(g.indptr)[order]
+        # rows sorted by decreasing length inside each window of 64
+        for w in range(0, min(g.n, 64 * 50), 64):
+            assert np.all(np.diff(deg[w:w + 64]) <= 0)
+    # a structured directed pattern (all columns in a small range)
+    n = 100_000
+    ip = np.arange(0, 3 * n + 1, 3, dtype=np.int64)
+    ix = (np.arange(3 * n, dtype=np.int64) * 7919) % 4096
+    P = CsrMatrix.from_arrays(ip, np.sort(ix.reshape(n, 3), axis=1).ravel(),
+                              np.ones(3 * n, np.float32), (n, n))
+    order = P.locality_order()
+    assert np.array_equal(np.sort(order), np.arange(n))
