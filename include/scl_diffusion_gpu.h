@@ -84,11 +84,15 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           operator's locality order (operators of >= 16384 rows get one);
  *           SCL_GPU_SHAPE_XCD_ROWS: each of the 8 XCDs owns a contiguous range
  *           of rows and walks it slab by slab (default: each XCD owns a
- *           range of slabs)
+ *           range of slabs);
+ *           SCL_GPU_SHAPE_SCALAR_ROWS: with group 16 or 32, the 64/group
+ *           rows of a wave keep their indices and values on the scalar unit
+ *           (each lane selects its row's) instead of loading them per lane
  * Any shape gives bit-identical results; only the speed differs. */
 #define SCL_GPU_SHAPE_NONTEMPORAL 1
 #define SCL_GPU_SHAPE_NO_SCHEDULE 2
 #define SCL_GPU_SHAPE_XCD_ROWS 4
+#define SCL_GPU_SHAPE_SCALAR_ROWS 8
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,
@@ -151,6 +155,15 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
     scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
 
+/* Edge gradient of one step on its own (CSR order):
+ * dP_dev[k] (+)= <G_dev[i,:], Xprev_dev[j,:]> for every nonzero k = (i, j);
+ * accumulate == 0 overwrites dP_dev.  The reverse pass above computes this
+ * fused with the adjoint; this call is the unfused building block. */
+SCL_EXPORT scl_error_t scl_gpu_edge_gradient_device(
+    scl_sparse_t transition, const scl_real_t* G_dev, const scl_real_t* Xprev_dev,
+    scl_real_t* dP_dev, scl_index_t n_nodes, scl_index_t n_features, int accumulate,
+    void* stream);
+
 /* Bytes of device workspace scl_gpu_diffuse_backward_device needs
  * ((every + 1) planes, plus nnz floats when want_dp). */
 SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_workspace(
@@ -165,6 +178,29 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
     const scl_real_t* dY_dev, scl_real_t* dX_dev, scl_real_t* dP_dev,
     void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
+/* ----------------------------------------------------------------- plans --
+ * A fixed sequence recorded once into a hipGraph and replayed with one
+ * launch: Y = P^t X with checkpoints (scl_gpu_diffuse_forward_ckpt_device);
+ * then, when T_dev is set, dY = Y - T (the gradient of 0.5 ||Y - T||^2);
+ * then, when dY_dev is set, the reverse pass into dX_dev and dP_dev
+ * (scl_gpu_diffuse_backward_device).  All buffers are the caller's and are
+ * bound at record time.  The operator's device arrays are pinned until the
+ * plan is destroyed: value updates are applied in place, a pattern change is
+ * refused; destroy plans before their operator.  The graph holds kernel
+ * nodes only. */
+typedef struct scl_gpu_plan scl_gpu_plan;
+typedef scl_gpu_plan* scl_gpu_plan_t;
+
+SCL_EXPORT scl_error_t scl_gpu_plan_diffusion(
+    scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
+    scl_real_t* ckpt_dev, scl_real_t* fwd_work_dev, const scl_real_t* T_dev,
+    scl_real_t* dY_dev, scl_real_t* dX_dev, scl_real_t* dP_dev,
+    void* bwd_work_dev, scl_size_t bwd_work_bytes, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every,
+    scl_gpu_plan_t* out);
+SCL_EXPORT scl_error_t scl_gpu_plan_launch(scl_gpu_plan_t plan, void* stream);
+SCL_EXPORT scl_error_t scl_gpu_plan_destroy(scl_gpu_plan_t* plan);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,15 @@ SCL_EXPORT scl_error_t scl_sparse_get_data(scl_sparse_t matrix,
  * swapped.  Diffusion through the result applies the source's transpose. */
 SCL_EXPORT scl_error_t scl_sparse_transpose(scl_sparse_t src, scl_sparse_t* out);
 
+/* Block-diagonal stack of `count` handles of one orientation (a new deep
+ * copy): block g's slots and indices are offset by the sizes of blocks
+ * 0..g-1, values and in-slot order are kept.  No counterpart in the
+ * reference (which has vstack / hstack, sparse.h:444-460): it lets many
+ * independent graphs be diffused by one launch per step, since rows of
+ * different blocks never mix. */
+SCL_EXPORT scl_error_t scl_sparse_block_diag(const scl_sparse_t* mats,
+                                             scl_size_t count, scl_sparse_t* out);
+
 /* sparse.h:304 */
 SCL_EXPORT scl_error_t scl_sparse_to_contiguous(scl_sparse_t src,
                                                 scl_sparse_t* out);

@@ -141,7 +141,13 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
     if (p >= lim || q >= lim || nnz >= lim)
         fail(SCL_ERROR_INVALID_ARGUMENT,
              "operator too large for int32 device indices (dims and nnz must be < 2^31-1)");
-    if (!(slot && slot->n == p && slot->n_sec == q && slot->nnz == nnz)) {
+    const bool same_shape = slot && slot->n == p && slot->n_sec == q && slot->nnz == nnz;
+    if (slot && slot->pins > 0 &&
+        (!same_shape || (p >= kScheduleMinRows &&
+                         pattern_fingerprint(p, h->indptr, h->indices) != slot->fingerprint)))
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "the operator's pattern changed while a captured plan uses it");
+    if (!same_shape) {
         slot.reset(new DeviceCSR());
         slot->device = device;
         slot->n = (int32_t)p;
@@ -173,7 +179,14 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->t_shapes.clear();
         }
     }
-    slot->release_transpose();
+    if (slot->pins > 0 && slot->t_row_ptr) {
+        // a plan holds the transposed arrays: refresh their values in place
+        launch_gather_f32(slot->t_src, slot->val, slot->t_val, nnz, nullptr);
+        hip_check(hipStreamSynchronize(nullptr), "refresh transposed values");
+        slot->t_generation = h->generation;
+    } else {
+        slot->release_transpose();
+    }
     slot->generation = h->generation;
     return *slot;
 }

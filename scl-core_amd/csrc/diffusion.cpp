@@ -114,7 +114,13 @@ void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
     // the device copy now holds the new values: advance both generations
     A->generation++;
     d.generation = A->generation;
-    d.release_transpose();
+    if (d.pins > 0 && d.t_row_ptr) {
+        launch_gather_f32(d.t_src, d.val, d.t_val, d.nnz, s);
+        hip_check(hipStreamSynchronize(s), "refresh transposed values");
+        d.t_generation = d.generation;
+    } else {
+        d.release_transpose();
+    }
     if (residual_out) {
         ensure_transpose(A, d, s);
         double* out = scratch + 1100;
@@ -339,6 +345,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.nontemporal = (flags & kShapeNontemporal) != 0;
     sh.schedule = (flags & kShapeNoSchedule) == 0;
     sh.xcd_rows = (flags & kShapeXcdRows) != 0;
+    sh.scalar_rows = (flags & kShapeScalarRows) != 0;
     SCL_API_CHECK(spmm_shape_valid(sh, n_features, nullptr, nullptr),
                   SCL_ERROR_INVALID_ARGUMENT,
                   "invalid shape: vec in {1,2,4} dividing n_features, group a "
@@ -374,7 +381,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
         *unroll = sh.unroll;
         *flags = (sh.nontemporal ? kShapeNontemporal : 0) |
                  (sh.schedule ? 0 : kShapeNoSchedule) |
-                 (sh.xcd_rows ? kShapeXcdRows : 0);
+                 (sh.xcd_rows ? kShapeXcdRows : 0) |
+                 (sh.scalar_rows ? kShapeScalarRows : 0);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

@@ -88,6 +88,29 @@ SCL_EXPORT scl_error_t scl_gpu_sparse_set_values_device(scl_sparse_t matrix,
     SCL_API_CATCH
 }
 
+SCL_EXPORT scl_error_t scl_gpu_edge_gradient_device(
+    scl_sparse_t transition, const scl_real_t* G_dev, const scl_real_t* Xprev_dev,
+    scl_real_t* dP_dev, scl_index_t n_nodes, scl_index_t n_features, int accumulate,
+    void* stream) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK(G_dev && Xprev_dev && dP_dev, SCL_ERROR_NULL_POINTER,
+                  "G, Xprev and dP must be set");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    SCL_API_TRY
+        check_square(transition, n_nodes, n_features);
+        const int dev = current_device();
+        CsrView v;
+        {
+            std::lock_guard<std::mutex> lk(transition->mu);
+            v = fwd_view(device_csr(transition, dev));
+        }
+        launch_edge_grad(v, G_dev, Xprev_dev, dP_dev, accumulate != 0, n_features,
+                         static_cast<hipStream_t>(stream));
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
 SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,

@@ -162,6 +162,102 @@ AdjFn pick_adj(int group, int nc, bool edge) {
     }
 }
 
+// Standalone edge gradient of one step over the forward pattern: for row i,
+// G[i, :] is loaded once; every nonzero k = (i, j) gathers Xprev[j, :] and
+// writes dP[k] (+)= <G[i, :], Xprev[j, :]> (CSR order).
+template <int VEC, int G, int NC>
+__global__ __launch_bounds__(kBlock) void k_edge_grad(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const int32_t* __restrict__ sched, const float* __restrict__ Gs,
+    const float* __restrict__ Xprev, float* __restrict__ dP, int accumulate,
+    int32_t n, int32_t F, int32_t f0, int32_t Fp) {
+    using VT = typename Vec<VEC>::T;
+    constexpr int RPW = 64 / G;
+    constexpr int RPB = (kBlock / 64) * RPW;
+    constexpr int U = 8;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = lane / G, li = lane % G;
+    const int rr = L * RPB + wave * RPW + grp;
+    if (rr >= n) return;
+    int row = sched ? sched[rr] : rr;
+    if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
+    const int beg = row_ptr[row], end = row_ptr[row + 1];
+    bool act[NC];
+    int off[NC];
+    VT gi[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int f = c * (G * VEC) + li * VEC;
+        act[c] = f < Fp;
+        off[c] = act[c] ? f0 + f : 0;
+        gi[c] = *reinterpret_cast<const VT*>(Gs + (size_t)row * F + off[c]);
+        if (!act[c]) gi[c] = VT{};
+    }
+    for (int k = beg; k < end; k += U) {
+        int j[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = col[min(k + u, end - 1)];
+        VT x[U][NC];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                x[u][c] = *reinterpret_cast<const VT*>(Xprev + (size_t)j[u] * F + off[c]);
+        float d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float sacc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) sacc += dotv(gi[c], x[u][c]);
+            d[u] = group_sum<G>(sacc);
+        }
+        if (li == 0) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k + u < end) dP[k + u] = accumulate ? dP[k + u] + d[u] : d[u];
+        }
+    }
+}
+
+template <int VEC, int G, int NC>
+void edge_launch(const CsrView& P, const float* Gs, const float* Xprev, float* dP,
+                 int accumulate, int64_t F, int64_t f0, int64_t Fp, hipStream_t s) {
+    const int rpb = (kBlock / 64) * (64 / G);
+    const int64_t nb = (P.n + rpb - 1) / rpb;
+    hipLaunchKernelGGL((k_edge_grad<VEC, G, NC>), dim3((unsigned)nb), dim3(kBlock), 0, s,
+                       P.row_ptr, P.col, P.sched, Gs, Xprev, dP, accumulate, P.n,
+                       (int32_t)F, (int32_t)f0, (int32_t)Fp);
+}
+
+template <int VEC>
+void edge_pick(int group, int nc, const CsrView& P, const float* Gs, const float* Xprev,
+               float* dP, int acc, int64_t F, int64_t f0, int64_t Fp, hipStream_t s) {
+    switch (group) {
+        case 1: edge_launch<VEC, 1, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 2: edge_launch<VEC, 2, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 4: edge_launch<VEC, 4, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 8: edge_launch<VEC, 8, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 16: edge_launch<VEC, 16, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 32: edge_launch<VEC, 32, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        default: break;
+    }
+    switch (nc) {
+        case 1: edge_launch<VEC, 64, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 2: edge_launch<VEC, 64, 2>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        case 4: edge_launch<VEC, 64, 4>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+        default: edge_launch<VEC, 64, 8>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sub(const float* __restrict__ a,
+                                                const float* __restrict__ b,
+                                                float* __restrict__ c, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) c[p] = a[p] - b[p];
+}
+
 __global__ __launch_bounds__(kBlock) void k_scatter(const int32_t* __restrict__ idx,
                                                     const float* __restrict__ src,
                                                     float* __restrict__ dst, int64_t m) {
@@ -216,6 +312,35 @@ void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_
     hipLaunchKernelGGL(k_scatter, dim3((unsigned)((m + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, idx, src, dst, m);
     check_launch("scatter launch");
+}
+
+void launch_edge_grad(const CsrView& P, const float* Gs, const float* Xprev, float* dP,
+                      bool accumulate, int64_t F, hipStream_t s) {
+    if (P.n <= 0 || F <= 0) return;
+    const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Xprev);
+    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
+    const int64_t per_pass = (int64_t)64 * vec * 8;
+    for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
+        const int64_t Fp = std::min<int64_t>(per_pass, F - f0);
+        const int64_t lanes = (Fp + vec - 1) / vec;
+        int group = 1;
+        while (group < lanes && group < 64) group <<= 1;
+        int nc = 1;
+        if (group == 64)
+            while ((int64_t)nc * 64 < lanes) nc <<= 1;
+        const int acc = (accumulate || f0 > 0) ? 1 : 0;
+        if (vec == 4) edge_pick<4>(group, nc, P, Gs, Xprev, dP, acc, F, f0, Fp, s);
+        else if (vec == 2) edge_pick<2>(group, nc, P, Gs, Xprev, dP, acc, F, f0, Fp, s);
+        else edge_pick<1>(group, nc, P, Gs, Xprev, dP, acc, F, f0, Fp, s);
+        check_launch("edge gradient launch");
+    }
+}
+
+void launch_sub_f32(const float* a, const float* b, float* c, int64_t m, hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_sub, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, a, b, c, m);
+    check_launch("sub launch");
 }
 
 void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,

@@ -89,6 +89,9 @@ struct DeviceCSR {
     float* t_val = nullptr;
     int32_t* t_src = nullptr;
     int32_t* t_sched = nullptr;
+    // number of captured plans that hold raw pointers into this copy: while
+    // non-zero, refreshes update the arrays in place and never reallocate
+    int pins = 0;
     // values rounded to bf16 (built on first use, for the current values)
     uint16_t* val_bf16 = nullptr;
     uint64_t bf16_generation = ~uint64_t(0);

@@ -30,6 +30,24 @@ using namespace dev;
 
 namespace {
 
+// Workgroup -> (feature slab, row block).  rb_per_xcd > 0: XCD x owns row
+// blocks [x * rb_per_xcd, (x+1) * rb_per_xcd) and walks them slab by slab,
+// so all XCDs work on the same slab at a time; else the logical order is
+// slab-major and each XCD owns a contiguous run of it.
+__device__ __forceinline__ bool block_coords(int n_rowblocks, int rb_per_xcd, int& slab,
+                                             int& rb) {
+    if (rb_per_xcd > 0) {
+        const int x = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
+        slab = idx / rb_per_xcd;
+        rb = x * rb_per_xcd + (idx - slab * rb_per_xcd);
+        return rb < n_rowblocks;
+    }
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    slab = L / n_rowblocks;
+    rb = L - slab * n_rowblocks;
+    return true;
+}
+
 template <int VEC, int G, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -40,18 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     constexpr int RPW = 64 / G;
     constexpr int RPB = (kBlock / 64) * RPW;
     int slab, rb;
-    if (rb_per_xcd > 0) {
-        // XCD x owns row blocks [x * rb_per_xcd, (x+1) * rb_per_xcd) and walks
-        // them slab by slab, so all XCDs work on the same slab at a time
-        const int x = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
-        slab = idx / rb_per_xcd;
-        rb = x * rb_per_xcd + (idx - slab * rb_per_xcd);
-        if (rb >= n_rowblocks) return;
-    } else {
-        const int L = xcd_remap(blockIdx.x, gridDim.x);
-        slab = L / n_rowblocks;
-        rb = L - slab * n_rowblocks;
-    }
+    if (!block_coords(n_rowblocks, rb_per_xcd, slab, rb)) return;
     const int lane = threadIdx.x & 63;
     // wave index is uniform across the wave: keep it in a scalar register
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -131,6 +138,91 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     if (active) store_vec<VEC, NT>(Y + (size_t)row * F + f, acc);
 }
 
+
+// Several rows per wave (RPW = 2 or 4, G = 64 / RPW lanes each) with the
+// rows' column indices and values kept on the scalar unit: every lane picks
+// its row's (j, p) with a select instead of loading them itself, so a
+// narrow slab (G * VEC features, 128 or 64 for VEC = 4) costs one vector
+// gather per nonzero like the one-row-per-wave kernel.  The loop runs to the
+// longest of the wave's rows (rows are sorted by length inside windows of
+// 64 by the schedule); shorter rows skip their FMAs.
+template <int RPW, typename T>
+__device__ __forceinline__ T pick_row(int grp, const T (&a)[RPW]) {
+    T r = a[0];
+#pragma unroll
+    for (int q = 1; q < RPW; ++q) r = (grp == q) ? a[q] : r;
+    return r;
+}
+
+template <int VEC, int RPW, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_spmm_multirow(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const int32_t* __restrict__ sched,
+    const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
+    int32_t n_rowblocks, int32_t rb_per_xcd) {
+    using VT = typename Vec<VEC>::T;
+    constexpr int G = 64 / RPW;
+    constexpr int RPB = (kBlock / 64) * RPW;
+    int slab, rb;
+    if (!block_coords(n_rowblocks, rb_per_xcd, slab, rb)) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = lane / G, li = lane % G;
+    const int rr0 = rb * RPB + wave * RPW;
+    if (rr0 >= n) return;
+    int row[RPW], beg[RPW], len[RPW];
+    int maxlen = 0;
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        int r = -1, b = 0, e = 0;
+        if (rr0 + q < n) {
+            r = __builtin_amdgcn_readfirstlane(sched ? sched[rr0 + q] : rr0 + q);
+            b = __builtin_amdgcn_readfirstlane(row_ptr[r]);
+            e = __builtin_amdgcn_readfirstlane(row_ptr[r + 1]);
+        }
+        row[q] = r;
+        beg[q] = b;
+        len[q] = e - b;
+        maxlen = max(maxlen, e - b);
+    }
+    const int my_row = pick_row<RPW>(grp, row);
+    const int my_len = pick_row<RPW>(grp, len);
+    const int f = slab * (G * VEC) + li * VEC;
+    const bool active = f < F && my_row >= 0;
+    const float* Xf = X + (f < F ? f : 0);
+
+    float acc[VEC];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+
+    for (int k = 0; k < maxlen; k += U) {
+        int jq[U][RPW];
+        float vq[U][RPW];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < RPW; ++q) {
+                // clamped inside row q; an empty row reads entry 0, which
+                // exists because some row of the wave is not empty
+                const int kk = len[q] ? beg[q] + min(k + u, len[q] - 1) : 0;
+                jq[u][q] = col[kk];
+                vq[u][q] = val[kk];
+            }
+        VT x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = pick_row<RPW>(grp, jq[u]);
+            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j * F);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float v = pick_row<RPW>(grp, vq[u]);
+            if (k + u < my_len) fma_vec(v, x[u], acc);
+        }
+    }
+    if (active) store_vec<VEC, NT>(Y + (size_t)my_row * F + f, acc);
+}
+
 using SpmmFn = void (*)(const int32_t*, const int32_t*, const float*,
                         const int32_t*, const float*, float*, int32_t, int32_t,
                         int32_t, int32_t);
@@ -159,7 +251,25 @@ SpmmFn pick_g(int g, int u, bool nt) {
     }
 }
 
+template <int VEC, int RPW>
+SpmmFn pick_multirow(int u, bool nt) {
+    if (u >= 16) return nt ? k_spmm_multirow<VEC, RPW, 16, true> : k_spmm_multirow<VEC, RPW, 16, false>;
+    if (u >= 8) return nt ? k_spmm_multirow<VEC, RPW, 8, true> : k_spmm_multirow<VEC, RPW, 8, false>;
+    return nt ? k_spmm_multirow<VEC, RPW, 4, true> : k_spmm_multirow<VEC, RPW, 4, false>;
+}
+
 SpmmFn pick_spmm(const SpmmShape& s) {
+    if (s.scalar_rows && (s.group == 16 || s.group == 32)) {
+        const int rpw = 64 / s.group;
+        switch (s.vec) {
+            case 1: return rpw == 2 ? pick_multirow<1, 2>(s.unroll, s.nontemporal)
+                                    : pick_multirow<1, 4>(s.unroll, s.nontemporal);
+            case 2: return rpw == 2 ? pick_multirow<2, 2>(s.unroll, s.nontemporal)
+                                    : pick_multirow<2, 4>(s.unroll, s.nontemporal);
+            default: return rpw == 2 ? pick_multirow<4, 2>(s.unroll, s.nontemporal)
+                                     : pick_multirow<4, 4>(s.unroll, s.nontemporal);
+        }
+    }
     switch (s.vec) {
         case 1: return pick_g<1>(s.group, s.unroll, s.nontemporal);
         case 2: return pick_g<2>(s.group, s.unroll, s.nontemporal);
@@ -208,15 +318,17 @@ std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const voi
         for (int u : {4, 8, 16})
             for (int nt = 0; nt < 2; ++nt)
                 for (int xr = 0; xr < 2; ++xr)
-                    for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
-                        SpmmShape s = base;
-                        s.group = g;
-                        s.unroll = u;
-                        s.nontemporal = nt != 0;
-                        s.xcd_rows = xr != 0;
-                        s.schedule = sc != 0;
-                        out.push_back(s);
-                    }
+                    for (int sr = 0; sr < ((g == 16 || g == 32) ? 2 : 1); ++sr)
+                        for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
+                            SpmmShape s = base;
+                            s.group = g;
+                            s.unroll = u;
+                            s.nontemporal = nt != 0;
+                            s.xcd_rows = xr != 0;
+                            s.scalar_rows = sr != 0;
+                            s.schedule = sc != 0;
+                            out.push_back(s);
+                        }
     }
     if (out.empty()) out.push_back(base);
     return out;

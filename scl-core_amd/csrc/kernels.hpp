@@ -27,12 +27,14 @@ struct SpmmShape {
     bool nontemporal = false;  // streaming (non-temporal) stores of Y
     bool schedule = true;      // visit rows in the operator's locality order
     bool xcd_rows = false;     // each XCD owns a row range (else a slab range)
+    bool scalar_rows = false;  // group 16/32: rows' indices on the scalar unit
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
 constexpr int kShapeNontemporal = 1;
 constexpr int kShapeNoSchedule = 2;
 constexpr int kShapeXcdRows = 4;
+constexpr int kShapeScalarRows = 8;
 
 // True when the shape can run on F features with these X / Y pointers.
 bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* Y);
@@ -56,6 +58,14 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
 void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
                          float* Gp, float* dPT, bool accumulate, int64_t F,
                          hipStream_t s);
+
+// Edge gradient of one step over the forward pattern (CSR order):
+// dP[k] (+)= <Gs[i, :], Xprev[j, :]> for every nonzero k = (i, j).
+void launch_edge_grad(const CsrView& P, const float* Gs, const float* Xprev, float* dP,
+                      bool accumulate, int64_t F, hipStream_t s);
+
+// c = a - b elementwise (m floats).
+void launch_sub_f32(const float* a, const float* b, float* c, int64_t m, hipStream_t s);
 
 // dst[idx[p]] = src[p] and dst[p] = src[idx[p]] for p < m.
 void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_t m,

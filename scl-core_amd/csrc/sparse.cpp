@@ -291,6 +291,56 @@ SCL_EXPORT scl_error_t scl_sparse_transpose(scl_sparse_t src, scl_sparse_t* out)
     SCL_API_CATCH
 }
 
+SCL_EXPORT scl_error_t scl_sparse_block_diag(const scl_sparse_t* mats,
+                                             scl_size_t count, scl_sparse_t* out) {
+    SCL_API_CHECK_NULL(mats, "Matrix array is null");
+    SCL_API_CHECK_NULL(out, "Output pointer is null");
+    SCL_API_CHECK(count > 0, SCL_ERROR_INVALID_ARGUMENT, "count must be positive");
+    for (scl_size_t g = 0; g < count; ++g)
+        SCL_API_CHECK(mats[g] != nullptr, SCL_ERROR_NULL_POINTER, "Matrix is null");
+    SCL_API_TRY
+        const bool csr = mats[0]->is_csr;
+        int64_t rows = 0, cols = 0, nnz = 0;
+        for (scl_size_t g = 0; g < count; ++g) {
+            if (mats[g]->is_csr != csr)
+                fail(SCL_ERROR_TYPE_MISMATCH, "all blocks must have the same orientation");
+            rows += mats[g]->rows;
+            cols += mats[g]->cols;
+            nnz += mats[g]->nnz;
+        }
+        auto* h = new_handle(rows, cols, nnz, csr);
+        try {
+            h->indptr_store.reserve((size_t)h->primary() + 1);
+            h->indices_store.reserve((size_t)nnz);
+            h->data_store.reserve((size_t)nnz);
+            h->indptr_store.push_back(0);
+            int64_t prim_off = 0, sec_off = 0, nz_off = 0;
+            for (scl_size_t g = 0; g < count; ++g) {
+                scl_sparse_matrix* m = mats[g];
+                std::lock_guard<std::mutex> lk(m->mu);
+                const int64_t p = m->primary();
+                for (int64_t i = 0; i < p; ++i)
+                    h->indptr_store.push_back(nz_off + m->indptr[i + 1]);
+                for (int64_t k = 0; k < m->nnz; ++k) {
+                    h->indices_store.push_back(sec_off + m->indices[k]);
+                    h->data_store.push_back(m->data[k]);
+                }
+                prim_off += p;
+                sec_off += m->secondary();
+                nz_off += m->nnz;
+            }
+            h->indptr = h->indptr_store.data();
+            h->indices = h->indices_store.data();
+            h->data = h->data_store.data();
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
 SCL_EXPORT scl_error_t scl_sparse_to_contiguous(scl_sparse_t src,
                                                 scl_sparse_t* out) {
     return scl_sparse_clone(src, out);

@@ -69,6 +69,7 @@ SIGNATURES = {
     "scl_sparse_get_data": (c_err, [P, PP, ctypes.POINTER(c_size)]),
     "scl_sparse_transpose": (c_err, [P, PP]),
     "scl_sparse_to_contiguous": (c_err, [P, PP]),
+    "scl_sparse_block_diag": (c_err, [P, c_size, PP]),
     # scl_diffusion.h
     "scl_diffusion_compute_transition_matrix": (c_err, [P, ctypes.c_int]),
     "scl_diffusion_diffuse_vector": (c_err, [P, P, c_size, c_index]),
@@ -93,8 +94,15 @@ SIGNATURES = {
         c_err, [P, P, P, P, c_index, c_index, c_index, ctypes.c_int, P]),
     "scl_gpu_diffuse_forward_ckpt_device": (
         c_err, [P, P, P, P, P, c_index, c_index, c_index, c_index, P]),
+    "scl_gpu_edge_gradient_device": (
+        c_err, [P, P, P, P, c_index, c_index, ctypes.c_int, P]),
     "scl_gpu_diffuse_backward_workspace": (
         c_err, [P, c_index, c_index, c_index, ctypes.c_int, ctypes.POINTER(c_size)]),
+    "scl_gpu_plan_diffusion": (
+        c_err, [P, P, P, P, P, P, P, P, P, P, c_size, c_index, c_index, c_index,
+                c_index, PP]),
+    "scl_gpu_plan_launch": (c_err, [P, P]),
+    "scl_gpu_plan_destroy": (c_err, [PP]),
     "scl_gpu_diffuse_backward_device": (
         c_err, [P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
 }

@@ -55,6 +55,21 @@ def diffuse_bf16(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
     return Y
 
 
+def edge_gradient(P: SparseMatrixBase, G: torch.Tensor, Xprev: torch.Tensor,
+                  dP: torch.Tensor = None, accumulate: bool = False) -> torch.Tensor:
+    """dP[k] (+)= <G[i,:], Xprev[j,:]> for every nonzero k = (i, j) of P."""
+    _check_dense(G)
+    _check_dense(Xprev)
+    n, F = G.shape
+    if dP is None:
+        dP = torch.empty(max(P.nnz, 1), dtype=torch.float32, device=G.device)
+    with torch.cuda.device(G.device):
+        _lib.check(_lib.load().scl_gpu_edge_gradient_device(
+            P.handle, G.data_ptr(), Xprev.data_ptr(), dP.data_ptr(), n, F,
+            1 if accumulate else 0, _stream_ptr(G)))
+    return dP
+
+
 def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
              ) -> float:
     """Picks the fastest SpMM launch shape for P and X's width; returns ms/launch."""
@@ -76,13 +91,15 @@ def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False) -> 
         ctypes.addressof(g), ctypes.addressof(u), ctypes.addressof(f)))
     return {"vec": v.value, "group": g.value, "unroll": u.value,
             "nontemporal": bool(f.value & 1), "schedule": not (f.value & 2),
-            "xcd_rows": bool(f.value & 4)}
+            "xcd_rows": bool(f.value & 4), "scalar_rows": bool(f.value & 8)}
 
 
 def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
               unroll: int = 8, nontemporal: bool = False, schedule: bool = True,
-              transpose: bool = False, xcd_rows: bool = False) -> None:
-    flags = (1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
+              transpose: bool = False, xcd_rows: bool = False,
+              scalar_rows: bool = False) -> None:
+    flags = ((1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
+             | (8 if scalar_rows else 0))
     _lib.check(_lib.load().scl_gpu_spmm_set_shape(
         P.handle, n_features, 1 if transpose else 0, vec, group, unroll, flags))
 

@@ -177,3 +177,22 @@ def test_locality_order_is_a_permutation_and_fast():
                               np.ones(3 * n, np.float32), (n, n))
     order = P.locality_order()
     assert np.array_equal(np.sort(order), np.arange(n))
+
+
+def test_block_diag_stacks_handles():
+    from scl_amd import batch
+    ip, ix, d = small_csr()
+    A = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    B = CsrMatrix.from_arrays([0, 1, 2], [1, 0], [7, 8], (2, 2))
+    S = batch.block_diag([A, B, A])
+    assert S.shape == (8, 8) and S.nnz == 14
+    sp, si, sd = S.export()
+    assert np.array_equal(sp, [0, 2, 3, 6, 7, 8, 10, 11, 14])
+    assert np.array_equal(si, [0, 2, 1, 0, 1, 2, 4, 3, 5, 7, 6, 5, 6, 7])
+    assert np.array_equal(sd, [1, 2, 3, 4, 5, 6, 7, 8, 1, 2, 3, 4, 5, 6])
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    C = CscMatrix.from_arrays(ip, ix, d, (3, 3))
+    arr = (ctypes.c_void_p * 2)(A.handle.value, C.handle.value)
+    assert lib.scl_sparse_block_diag(arr, 2, ctypes.byref(out)) == 21  # TYPE_MISMATCH
+    assert lib.scl_sparse_block_diag(arr, 0, ctypes.byref(out)) == 10

@@ -97,3 +97,26 @@ def test_finite_differences(gpu):
         errs.append(abs(fd - gP[kk]) / max(1e-2, abs(fd)))
     print(f"\nfinite differences: max rel deviation {max(errs):.2e}")
     assert max(errs) < 1e-3
+
+
+def test_unfused_building_blocks_agree_with_fused(gpu):
+    """Forward iterates kept, adjoint by the transposed SpMM, edge gradient by
+    its own kernel: the same dX bit for bit, and dP to fp32 rounding."""
+    n, F, t = 6000, 96, 4
+    g, X, T = _problem(n, 10, F, seed=77)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    its = [Xd]
+    for _ in range(t):
+        its.append(torch_ops.spmm(P, its[-1]))
+    G = its[-1] - Td
+    dP = None
+    for s in range(t, 0, -1):
+        dP = torch_ops.edge_gradient(P, G, its[s - 1], dP, accumulate=s != t)
+        G = torch_ops.spmm(P, G, transpose=True)
+    Xr = Xd.clone().requires_grad_(True)
+    vals = torch.from_numpy(g.data.copy()).cuda().requires_grad_(True)
+    Y = autograd.diffuse(P, Xr, t, values=vals, checkpoint_every=2)
+    (0.5 * ((Y - Td) ** 2).sum()).backward()
+    assert torch.equal(G, Xr.grad)
+    assert torch.allclose(dP, vals.grad, rtol=1e-5, atol=1e-6 * vals.grad.abs().max().item())

@@ -221,3 +221,29 @@ def test_knn30_200k_f2000_t8_column_sample(gpu):
     print(f"\nknn30_200k: nnz {g.nnz}, sampled columns bit-identical to CPU; "
           f"vs FP64 max {err:.3e} median {orc.median_rel_err(Ys, y64):.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("F", [64, 128, 200, 2000])
+def test_every_launch_shape_is_bit_identical(gpu, F):
+    """All launch shapes (row-split, scalar multi-row, both XCD mappings,
+    with and without the locality schedule) reproduce the restatement."""
+    import torch
+    from scl_amd import torch_ops
+    n = 20_000
+    g = synth.knn_graph(n, 12, seed=F)
+    A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    diffusion.compute_transition_matrix(A)
+    ip, ix, p = A.export()
+    X = synth.normal(n, F, seed=F)
+    ref = orc.spmm_step_f32(ip, ix, p, X)
+    Xd = torch.from_numpy(X).cuda()
+    vmax = 4 if F % 4 == 0 else (2 if F % 2 == 0 else 1)
+    for vec in sorted({1, vmax}):
+        for group in (8, 16, 32, 64):
+            for sr in (False, True):
+                for xr in (False, True):
+                    for u in (4, 16):
+                        torch_ops.set_shape(A, F, vec, group, u, nontemporal=u == 16,
+                                            schedule=not xr, xcd_rows=xr, scalar_rows=sr)
+                        Y = torch_ops.spmm(A, Xd).cpu().numpy()
+                        assert np.array_equal(Y, ref), (vec, group, sr, xr, u)
