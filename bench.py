@@ -1,24 +1,37 @@
 #!/usr/bin/env python3
 """Diffusion benchmark: Y = P^t X on one MI355X.
 
-Default workload (knn30_200k): n = 200,000 cells, geometric kNN graph with
+Headline (value): knn30_200k -- n = 200,000 cells, geometric kNN graph with
 k = 30 and union-max symmetrisation (nnz ~ 8.9e6, rows 30..~100), rows
-normalised in FP64 on the GPU; X = 200,000 x 2000 fp32 log1p(Poisson) counts;
-t = 8 steps.  One bench "step" is one full diffusion (t SpMM launches) over
+normalised in FP64 on the GPU; X = 200,000 x 2000 fp32 log1p(Poisson)
+counts; t = 8.  One bench step is one full diffusion (t SpMM launches) over
 device-resident P and X.
 
-value  = algorithmic HBM GB/s of the whole diffusion:
-         t * (nnz*(4+4) + 4*(n+1) + n*F*(4+4)) / time
-roofline.achieved = the same bytes per SpMM launch / mean launch time
-         (HIP events on the stream the kernels run on).
-cpu_baseline = the FP32 CPU restatement of the reference algorithm (OpenMP)
-         on all n rows and a column sample, scaled to the full width.
+  value             algorithmic HBM GB/s of the diffusion:
+                    t * (nnz*(4+4) + 4*(n+1) + n*F*(4+4)) / time
+  roofline.achieved the same bytes per SpMM launch / mean launch time (HIP
+                    events on the stream the kernels run on)
+  roofline.traffic  HBM bytes per launch from rocprofv3 FETCH_SIZE and
+                    WRITE_SIZE (profiles/traffic_knn30_200k.json)
+  cpu_baseline      the FP32 CPU restatement of the reference algorithm
+                    (OpenMP, all rows, a column sample, scaled to full width)
+
+Further legs (in "legs"; --legs selects, "c2" alone skips them):
+  c3  knn30_500k, F=128, t=6: forward with checkpoints every 2 + dL/dY of
+      0.5||Y-T||^2 + fused adjoint/edge-gradient reverse pass (one hipGraph),
+      vs the same work as plain kernels (6 SpMM + 6 transposed SpMM + 6 edge-
+      gradient passes with every iterate kept)
+  c4  uniform15_1m, F=256, t=4: bf16 iterates (fp32 or bf16 P values) vs
+      the fp32 kernel
+  c5  256 graphs, n_i log-uniform in [2000, 50000], k_i in 10..30, F=64,
+      t=5, fwd+bwd: one hipGraph over the block-diagonal stack vs the 256
+      graphs one after another with plain (non-graph) launches
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
+import math
 import sys
 import time
 from pathlib import Path
@@ -29,6 +42,7 @@ sys.path.insert(0, str(ROOT / "scl-core_amd"))
 
 PEAK_HBM_GBPS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_knn30_200k.json"
+METRIC = "diffusion HBM GB/s (% peak) + fwd+bwd steps/s vs plain FP32 at stated err"
 
 
 def log(msg: str) -> None:
@@ -37,6 +51,19 @@ def log(msg: str) -> None:
 
 def step_bytes(n: int, nnz: int, F: int, vb=4, xb=4, yb=4) -> int:
     return nnz * (vb + 4) + 4 * (n + 1) + n * F * (xb + yb)
+
+
+def timed(fn, reps: int, stream) -> float:
+    """Mean milliseconds of fn() over reps, HIP events on `stream`."""
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
 
 
 def cpu_baseline(graph_arrays, n, F, t, seed, bytes_full_step, budget_s=12.0):
@@ -73,16 +100,186 @@ def cpu_baseline(graph_arrays, n, F, t, seed, bytes_full_step, budget_s=12.0):
     }
 
 
+# ------------------------------------------------------------------- legs --
+
+def leg_c3(steps: int) -> dict:
+    """knn30_500k, F=128, t=6: fused fwd+bwd plan vs plain kernels."""
+    import torch
+    from scl_amd import CsrMatrix, batch, diffusion, synth, torch_ops
+    n, F, t, every = 500_000, 128, 6, 2
+    g = synth.knn_graph(n, 30, seed=3)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    diffusion.compute_transition_matrix(P)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    X = torch.from_numpy(synth.normal(n, F, seed=3)).to(dev)
+    T = torch.from_numpy(synth.normal(n, F, seed=103)).to(dev)
+    torch_ops.autotune(P, X)
+    plan = batch.DiffusionPlan(P, n, F, t, every=every)
+    plan.X.copy_(X)
+    plan.T.copy_(T)
+    plan.launch()
+    torch.cuda.synchronize()
+    fused_ms = timed(plan.launch, steps, stream)
+    # plain: every iterate kept, separate adjoint SpMM and edge-gradient passes
+    its = [X] + [torch.empty_like(X) for _ in range(t)]
+    G = [torch.empty_like(X), torch.empty_like(X)]
+    dP = torch.empty(P.nnz, dtype=torch.float32, device=dev)
+
+    from scl_amd import _lib
+    lib, sp = _lib.load(), stream.cuda_stream
+
+    def spmm(src, dst, transpose):
+        _lib.check(lib.scl_gpu_spmm_device(P.handle, src.data_ptr(), dst.data_ptr(),
+                                           n, F, transpose, sp))
+
+    def plain():
+        for s in range(1, t + 1):
+            spmm(its[s - 1], its[s], 0)
+        torch.sub(its[t], T, out=G[0])
+        cur = 0
+        for s in range(t, 0, -1):
+            torch_ops.edge_gradient(P, G[cur], its[s - 1], dP, accumulate=s != t)
+            spmm(G[cur], G[cur ^ 1], 1)
+            cur ^= 1
+
+    plain()
+    torch.cuda.synchronize()
+    plain_ms = timed(plain, max(2, steps // 2), stream)
+    same_dx = bool(torch.equal(G[t % 2], plan.dX))
+    nnz = g.nnz
+    B_f = step_bytes(n, nnz, F)
+    B_e = 8 * n * F + 12 * nnz
+    total = 9 * B_f + 6 * (B_f + B_e) + 3 * n * F * 4 + 2 * n * F * 4
+    plan.close()
+    return {
+        "workload": "knn30_500k_f128_t6", "nnz": nnz,
+        "fused_ms_per_fwd_bwd": round(fused_ms, 3),
+        "fused_steps_per_s": round(1e3 / fused_ms, 2),
+        "plain_ms_per_fwd_bwd": round(plain_ms, 3),
+        "plain_steps_per_s": round(1e3 / plain_ms, 2),
+        "speedup_vs_plain": round(plain_ms / fused_ms, 3),
+        "algorithmic_GBps": round(total / fused_ms / 1e6, 1),
+        "dX_fused_equals_plain": same_dx,
+        "err": "dX, dP vs FP64 <= 2.5e-7 (tests/test_gpu_grad.py)",
+    }
+
+
+def leg_c4(steps: int) -> dict:
+    """uniform15_1m, F=256, t=4: bf16 iterates vs fp32."""
+    import torch
+    from scl_amd import CsrMatrix, synth, torch_ops
+    n, F, t = 1_000_000, 256, 4
+    g = synth.uniform_knn(n, 15, seed=4).normalized()
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    X = torch.from_numpy(synth.normal(n, F, seed=4)).to(dev)
+    torch_ops.autotune(P, X)
+    Y32 = torch_ops.diffuse(P, X, t)
+    torch.cuda.synchronize()
+    f32_ms = timed(lambda: torch_ops.diffuse(P, X, t), steps, stream)
+    out = {"workload": "uniform15_1m_f256_t4", "nnz": g.nnz,
+           "fp32_ms": round(f32_ms, 3),
+           "fp32_GBps": round(t * step_bytes(n, g.nnz, F) / f32_ms / 1e6, 1)}
+    scale = float(Y32.abs().max())
+    for vb in (False, True):
+        Yb = torch_ops.diffuse_bf16(P, X, t, bf16_values=vb)
+        ms = timed(lambda: torch_ops.diffuse_bf16(P, X, t, bf16_values=vb), steps, stream)
+        vbytes = 2 if vb else 4
+        bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, 2) + (t - 2) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
+                  + step_bytes(n, g.nnz, F, vbytes, 2, 4))
+        key = "bf16_values" if vb else "fp32_values"
+        out[key] = {"ms": round(ms, 3), "GBps": round(bytes_ / ms / 1e6, 1),
+                    "speedup_vs_fp32": round(f32_ms / ms, 3),
+                    "max_rel_diff_vs_fp32_kernel": float((Yb - Y32).abs().max()) / scale}
+    out["err"] = "vs FP64 at n=200k: see tests/test_gpu_lowp.py output"
+    return out
+
+
+def leg_c5(steps: int) -> dict:
+    """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
+    import numpy as np
+    import torch
+    from scl_amd import CsrMatrix, batch, synth, torch_ops, _lib
+    F, t, every = 64, 5, 2
+    sizes, ks = synth.batch_sizes(256, seed=5)
+    graphs = [synth.uniform_knn(int(n), int(k), seed=5000 + i).normalized()
+              for i, (n, k) in enumerate(zip(sizes, ks))]
+    mats = [CsrMatrix.from_arrays(gr.indptr, gr.indices, gr.data, (gr.n, gr.n))
+            for gr in graphs]
+    B = batch.block_diag(mats)
+    off = batch.offsets(sizes)
+    N = int(off[-1])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    X = torch.from_numpy(synth.normal(N, F, seed=5)).to(dev)
+    T = torch.from_numpy(synth.normal(N, F, seed=105)).to(dev)
+    torch_ops.autotune(B, X)
+    plan = batch.DiffusionPlan(B, N, F, t, every=every)
+    plan.X.copy_(X)
+    plan.T.copy_(T)
+    plan.launch()
+    torch.cuda.synchronize()
+    plan_ms = timed(plan.launch, steps, stream)
+    # sequential: per graph, the same fwd + dY + reverse with plain launches
+    lib = _lib.load()
+    import ctypes
+    per = []
+    for i, m in enumerate(mats):
+        a, b = int(off[i]), int(off[i + 1])
+        n = b - a
+        nb = ctypes.c_size_t(0)
+        _lib.check(lib.scl_gpu_diffuse_backward_workspace(m.handle, n, F, every, 1,
+                                                          ctypes.byref(nb)))
+        m.prepare_device()
+        per.append((m, a, b, n, torch.empty(nb.value, dtype=torch.uint8, device=dev),
+                    nb.value))
+    Y = torch.empty_like(X)
+    dY = torch.empty_like(X)
+    dX = torch.empty_like(X)
+    dP = torch.empty(B.nnz, dtype=torch.float32, device=dev)
+    nz = np.concatenate([[0], np.cumsum([gr.nnz for gr in graphs])])
+    n_ck = (t + every - 1) // every - 1
+    ck = torch.empty((n_ck, int(max(sizes)), F), dtype=torch.float32, device=dev)
+    fw = torch.empty((2, int(max(sizes)), F), dtype=torch.float32, device=dev)
+    sp = stream.cuda_stream
+
+    def sequential():
+        for i, (m, a, b, n, work, nbytes) in enumerate(per):
+            _lib.check(lib.scl_gpu_diffuse_forward_ckpt_device(
+                m.handle, X[a:b].data_ptr(), Y[a:b].data_ptr(), ck.data_ptr(),
+                fw.data_ptr(), n, F, t, every, sp))
+            torch.sub(Y[a:b], T[a:b], out=dY[a:b])
+            _lib.check(lib.scl_gpu_diffuse_backward_device(
+                m.handle, X[a:b].data_ptr(), ck.data_ptr(), dY[a:b].data_ptr(),
+                dX[a:b].data_ptr(), dP[int(nz[i]):int(nz[i + 1])].data_ptr(),
+                work.data_ptr(), nbytes, n, F, t, every, sp))
+
+    sequential()
+    torch.cuda.synchronize()
+    seq_ms = timed(sequential, max(2, steps // 2), stream)
+    same = bool(torch.equal(dX, plan.dX)) and bool(torch.equal(Y, plan.Y))
+    shape = torch_ops.get_shape(B, F)
+    plan.close()
+    fwd_bytes = t * step_bytes(N, B.nnz, F)
+    return {"workload": "batch256_f64_t5", "graphs": 256, "rows": N, "nnz": B.nnz,
+            "plan_ms_per_fwd_bwd": round(plan_ms, 3),
+            "sequential_ms_per_fwd_bwd": round(seq_ms, 3),
+            "speedup_vs_sequential": round(seq_ms / plan_ms, 2),
+            "forward_GBps_equiv": round(fwd_bytes / plan_ms / 1e6, 1),
+            "autotuned_shape": shape, "results_identical": same,
+            "err": "per-graph dX, dP vs FP64 <= 1e-3 (tests/test_gpu_batch.py)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--n", type=int, default=200_000)
-    ap.add_argument("--F", type=int, default=2000)
-    ap.add_argument("--t", type=int, default=8)
-    ap.add_argument("--k", type=int, default=30)
+    ap.add_argument("--legs", default="c2,c3,c4,c5")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the "
@@ -95,7 +292,7 @@ def main() -> int:
     import scl_amd
     from scl_amd import CsrMatrix, diffusion, synth, torch_ops
 
-    n, F, t, k, seed = args.n, args.F, args.t, args.k, 2
+    n, F, t, k, seed = 200_000, 2000, 8, 30, 2
     log(f"workload knn{k}_{n} F={F} t={t}: generating graph")
     t0 = time.perf_counter()
     g = synth.knn_graph(n, k, seed=seed)
@@ -112,6 +309,11 @@ def main() -> int:
     Wd = torch.empty_like(Xd)
     lib = scl_amd.load()
     stream = torch.cuda.current_stream(dev)
+    tune_ms = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune_ms = torch_ops.autotune(A, Xd)
+        log(f"autotune {time.perf_counter() - t0:.1f}s -> {torch_ops.get_shape(A, F)}")
 
     def one():
         scl_amd.check(lib.scl_gpu_diffuse_matrix_device(
@@ -122,19 +324,11 @@ def main() -> int:
     for _ in range(args.warmup):
         one()
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
     log(f"timing {args.steps} diffusions ({args.steps * t} SpMM launches)")
-    ev0.record(stream)
-    for _ in range(args.steps):
-        one()
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    ms_per_step = ms / args.steps
-    ms_per_launch = ms / (args.steps * t)
+    ms_per_step = timed(one, args.steps, stream)
+    ms_per_launch = ms_per_step / t
     B = step_bytes(n, g.nnz, F)
-    value = args.steps * t * B / (ms / 1e3) / 1e9
+    value = t * B / (ms_per_step / 1e3) / 1e9
     achieved = B / (ms_per_launch / 1e3) / 1e9
     log(f"{ms_per_step:.3f} ms/diffusion, {ms_per_launch:.4f} ms/launch, {value:.1f} GB/s")
 
@@ -147,7 +341,7 @@ def main() -> int:
 
     traffic = None
     if TRAFFIC_FILE.exists():
-        traffic = json.loads(TRAFFIC_FILE.read_text()).get("bytes_per_launch")
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -155,8 +349,22 @@ def main() -> int:
         ip, ix, p = A.export()
         cpu = cpu_baseline((ip, ix, p), n, F, t, seed, B)
 
+    legs = {}
+    del Xd, Yd, Wd, A
+    torch.cuda.empty_cache()
+    for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5)):
+        if name not in args.legs.split(","):
+            continue
+        log(f"leg {name}")
+        try:
+            legs[name] = fn(max(3, args.steps // 2))
+        except Exception as e:  # a failed leg is reported, not fatal
+            legs[name] = {"error": f"{type(e).__name__}: {e}"}
+        torch.cuda.empty_cache()
+        log(f"leg {name}: {legs[name]}")
+
     out = {
-        "metric": "diffusion HBM GB/s (% peak) + fwd+bwd steps/s vs plain FP32 at stated err",
+        "metric": METRIC,
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -175,6 +383,7 @@ def main() -> int:
             "bytes_per_launch": B,
             "pct_peak": round(100 * value / PEAK_HBM_GBPS, 2),
             "stationarity_residual_fp64": residual,
+            "autotuned_ms_per_launch": tune_ms,
             "drop_in_host_ms": round(host_ms, 1),
             "drop_in_matches_device": same,
         },
@@ -187,6 +396,7 @@ def main() -> int:
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "legs": legs,
     }
     print(json.dumps(out), flush=True)
     return 0
