@@ -280,7 +280,11 @@ void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
     if (PT.n <= 0 || F <= 0) return;
     const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
                    reinterpret_cast<uintptr_t>(Xprev);
-    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
+    // one column per wave (indices on the scalar unit) whenever 64 lanes
+    // can each own features, with the widest such access
+    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
+    const int vec = (ok4 && F >= 256) ? 4 : (ok2 && F >= 128) ? 2 : (F >= 64) ? 1
+                  : ok4 ? 4 : ok2 ? 2 : 1;
     const bool edge = Xprev != nullptr && dPT != nullptr;
     // feature passes of at most 8 chunks of 64 lanes; the edge term of the
     // later passes accumulates onto the first pass's value
@@ -318,7 +322,9 @@ void launch_edge_grad(const CsrView& P, const float* Gs, const float* Xprev, flo
                       bool accumulate, int64_t F, hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
     const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Xprev);
-    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
+    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
+    const int vec = (ok4 && F >= 256) ? 4 : (ok2 && F >= 128) ? 2 : (F >= 64) ? 1
+                  : ok4 ? 4 : ok2 ? 2 : 1;
     const int64_t per_pass = (int64_t)64 * vec * 8;
     for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
         const int64_t Fp = std::min<int64_t>(per_pass, F - f0);

@@ -297,9 +297,14 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
 SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
     SpmmShape s;
     const auto a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
-    if (F % 4 == 0 && a % 16 == 0) s.vec = 4;
-    else if (F % 2 == 0 && a % 8 == 0) s.vec = 2;
-    else s.vec = 1;
+    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
+    // prefer one row per wave (indices on the scalar unit) with the widest
+    // access that still gives all 64 lanes a feature; narrow rows fall back
+    // to several rows per wave
+    if (ok4 && F >= 256) s.vec = 4;
+    else if (ok2 && F >= 128) s.vec = 2;
+    else if (F >= 64) s.vec = 1;
+    else s.vec = ok4 ? 4 : (ok2 ? 2 : 1);
     s.group = pow2_ceil((F + s.vec - 1) / s.vec);
     s.unroll = 8;
     s.nontemporal = false;
@@ -310,27 +315,31 @@ SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
 std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const void* Y,
                                              bool has_schedule) {
     std::vector<SpmmShape> out;
-    const SpmmShape base = spmm_default_shape(F, X, Y);
-    const int gmax = base.group;
-    for (int g = 4; g <= 64; g <<= 1) {
-        if (g > gmax) break;
-        if (g < gmax / 16) continue;  // slabs far narrower than F: skip
-        for (int u : {4, 8, 16})
-            for (int nt = 0; nt < 2; ++nt)
-                for (int xr = 0; xr < 2; ++xr)
-                    for (int sr = 0; sr < ((g == 16 || g == 32) ? 2 : 1); ++sr)
-                        for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
-                            SpmmShape s = base;
-                            s.group = g;
-                            s.unroll = u;
-                            s.nontemporal = nt != 0;
-                            s.xcd_rows = xr != 0;
-                            s.scalar_rows = sr != 0;
-                            s.schedule = sc != 0;
-                            out.push_back(s);
-                        }
+    const auto a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
+    for (int vec : {4, 2, 1}) {
+        if (F % vec != 0 || a % (4 * (uintptr_t)vec) != 0) continue;
+        const int gmax = pow2_ceil((F + vec - 1) / vec);
+        for (int g = 4; g <= gmax; g <<= 1) {
+            if (g < gmax / 8) continue;       // slabs far narrower than F
+            if (vec < 4 && g < 64 && g != gmax) continue;
+            for (int u : {4, 8, 16})
+                for (int nt = 0; nt < 2; ++nt)
+                    for (int xr = 0; xr < 2; ++xr)
+                        for (int sr = 0; sr < ((vec == 4 && (g == 16 || g == 32)) ? 2 : 1); ++sr)
+                            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
+                                SpmmShape s;
+                                s.vec = vec;
+                                s.group = g;
+                                s.unroll = u;
+                                s.nontemporal = nt != 0;
+                                s.xcd_rows = xr != 0;
+                                s.scalar_rows = sr != 0;
+                                s.schedule = sc != 0;
+                                out.push_back(s);
+                            }
+        }
     }
-    if (out.empty()) out.push_back(base);
+    if (out.empty()) out.push_back(spmm_default_shape(F, X, Y));
     return out;
 }
 
