@@ -183,13 +183,16 @@ def leg_c4(steps: int) -> dict:
            "fp32_ms": round(f32_ms, 3),
            "fp32_GBps": round(t * step_bytes(n, g.nnz, F) / f32_ms / 1e6, 1)}
     scale = float(Y32.abs().max())
-    for vb in (False, True):
-        Yb = torch_ops.diffuse_bf16(P, X, t, bf16_values=vb)
-        ms = timed(lambda: torch_ops.diffuse_bf16(P, X, t, bf16_values=vb), steps, stream)
+    k16 = torch_ops.lowp_scale_exp(P, X, t)
+    for fmt, vb in (("bf16", False), ("bf16", True), ("fp16", False)):
+        kk = k16 if fmt == "fp16" else 0
+        Yb = torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk)
+        ms = timed(lambda: torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk),
+                   steps, stream)
         vbytes = 2 if vb else 4
         bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, 2) + (t - 2) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
                   + step_bytes(n, g.nnz, F, vbytes, 2, 4))
-        key = "bf16_values" if vb else "fp32_values"
+        key = f"{fmt}_iterates_{'bf16' if vb else 'fp32'}_values"
         out[key] = {"ms": round(ms, 3), "GBps": round(bytes_ / ms / 1e6, 1),
                     "speedup_vs_fp32": round(f32_ms / ms, 3),
                     "max_rel_diff_vs_fp32_kernel": float((Yb - Y32).abs().max()) / scale}

@@ -118,20 +118,35 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
                                              int transpose, void* stream,
                                              float* best_ms);
 
-/* ----------------------------------------------------------- bf16 storage --
- * Y_dev = P^t X_dev (both fp32) with the t-1 intermediate iterates stored as
- * bf16 (round to nearest even) and fp32 accumulation: the first step gathers
- * the fp32 input, the last writes fp32.  flags:
- *   SCL_GPU_BF16_VALUES       read P's values rounded to bf16 (else fp32)
- *   SCL_GPU_BF16_NO_SCHEDULE  natural row order
- * n_features must be even; work_dev holds 2 planes of n x F bf16
+/* ------------------------------------------------- 16-bit iterate storage --
+ * Y_dev = P^t X_dev (both fp32) with the t-1 intermediate iterates stored in
+ * 16 bits and fp32 accumulation: the first step gathers the fp32 input, the
+ * last writes fp32, every row result in between is rounded once (round to
+ * nearest even).  flags:
+ *   SCL_GPU_LOWP_FP16         iterates as fp16 scaled by 2^scale_exp (else bf16,
+ *                             scale_exp must be 0)
+ *   SCL_GPU_LOWP_BF16_VALUES  read P's values rounded to bf16 (else fp32)
+ *   SCL_GPU_LOWP_NO_SCHEDULE  natural row order
+ * n_features must be even; work_dev holds 2 planes of n x F 16-bit values
  * (n * F * 4 bytes) and is required when n_steps >= 2. */
-#define SCL_GPU_BF16_VALUES 1
-#define SCL_GPU_BF16_NO_SCHEDULE 2
-SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_bf16_device(
+#define SCL_GPU_LOWP_BF16_VALUES 1
+#define SCL_GPU_LOWP_NO_SCHEDULE 2
+#define SCL_GPU_LOWP_FP16 4
+SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     void* work_dev, scl_index_t n_nodes, scl_index_t n_features,
-    scl_index_t n_steps, int flags, void* stream);
+    scl_index_t n_steps, int flags, int scale_exp, void* stream);
+
+/* A safe fp16 scale: the largest k with 2^k * max_s max|X_s| <= 2^15 for the
+ * intermediate iterates, bounded through the largest row |sum| of P
+ * (|X_s| <= rho^s max|X|).  Reads X_dev on the device; synchronises
+ * `stream`. */
+SCL_EXPORT scl_error_t scl_gpu_lowp_scale_exp(scl_sparse_t transition,
+                                              const scl_real_t* X_dev,
+                                              scl_index_t n_nodes,
+                                              scl_index_t n_features,
+                                              scl_index_t n_steps, int* scale_exp,
+                                              void* stream);
 
 /* ------------------------------------------------------------- gradients --
  * Reverse mode of Y = P^t X for a loss L(Y): given dY = dL/dY, returns

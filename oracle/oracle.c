@@ -183,26 +183,61 @@ static inline float orc_round_bf16(float f) {
 
 float orc_bf16_round(float f) { return orc_round_bf16(f); }
 
-int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+/* fp32 -> binary16 -> fp32, round to nearest even; overflow -> inf,
+ * subnormals kept (quantum 2^-24). */
+static inline float orc_round_fp16(float f) {
+    if (f != f) return f;
+    const float a = fabsf(f);
+    if (a >= 65520.0f) return copysignf(INFINITY, f);
+    if (a < 6.103515625e-05f) {
+        /* a * 2^24 < 2^10 is exact; rintf rounds half to even */
+        const float r = rintf(a * 16777216.0f) / 16777216.0f;
+        return copysignf(r, f);
+    }
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u + 0x0fffu + ((u >> 13) & 1u)) & 0xffffe000u;
+    float r;
+    memcpy(&r, &u, 4);
+    if (fabsf(r) > 65504.0f) return copysignf(INFINITY, f);
+    return r;
+}
+
+float orc_fp16_round(float f) { return orc_round_fp16(f); }
+
+int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
-                         int bf16_values, const float* X, float* Y) {
+                         int bf16_values, int fp16, int scale_exp, const float* X,
+                         float* Y) {
     const size_t total = (size_t)n * (size_t)F;
     const int64_t nnz = indptr[n];
     float* v = (float*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(float));
     float* a = (float*)malloc(total * sizeof(float));
     float* b = (float*)malloc(total * sizeof(float));
     if (!v || !a || !b) { free(v); free(a); free(b); return -1; }
+    const float up = ldexpf(1.0f, scale_exp), down = ldexpf(1.0f, -scale_exp);
     for (int64_t k = 0; k < nnz; ++k) v[k] = bf16_values ? orc_round_bf16(vals[k]) : vals[k];
     memcpy(a, X, total * sizeof(float));
     for (int64_t s = 1; s <= t; ++s) {
         orc_spmm_strided_f32(n, F, indptr, indices, v, a, b);
-        if (s < t)
-            for (size_t e = 0; e < total; ++e) b[e] = orc_round_bf16(b[e]);
+        if (s < t) {
+            /* stored as round16(y * 2^k), read back as that * 2^-k */
+            if (fp16)
+                for (size_t e = 0; e < total; ++e) b[e] = orc_round_fp16(b[e] * up) * down;
+            else
+                for (size_t e = 0; e < total; ++e) b[e] = orc_round_bf16(b[e]);
+        }
         float* tmp = a; a = b; b = tmp;
     }
     memcpy(Y, a, total * sizeof(float));
     free(v); free(a); free(b);
     return 0;
+}
+
+int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
+                         const int64_t* indices, const float* vals,
+                         int bf16_values, const float* X, float* Y) {
+    return orc_diffuse_lowp_f32(n, F, t, indptr, indices, vals, bf16_values, 0, 0, X, Y);
 }
 
 /* ---------------------------------------------------------------- FP64 -- */

@@ -64,6 +64,9 @@ def load():
         "orc_num_threads": (ctypes.c_int, []),
         "orc_diffuse_bf16_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int, P, P]),
         "orc_bf16_round": (ctypes.c_float, [ctypes.c_float]),
+        "orc_fp16_round": (ctypes.c_float, [ctypes.c_float]),
+        "orc_diffuse_lowp_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, P, P]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -121,6 +124,29 @@ def diffuse_bf16_f32(indptr, indices, vals, X, t, bf16_values=False):
     if rc:
         raise MemoryError("oracle allocation failed")
     return Y
+
+
+def diffuse_lowp_f32(indptr, indices, vals, X, t, fmt="bf16", bf16_values=False,
+                     scale_exp=0):
+    """16-bit-iterate restatement ("bf16" or scaled "fp16")."""
+    X = _f32(X)
+    n, F = X.shape
+    Y = np.empty_like(X)
+    ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    rc = load().orc_diffuse_lowp_f32(n, F, t, ip.ctypes.data, ix.ctypes.data,
+                                     v.ctypes.data, 1 if bf16_values else 0,
+                                     1 if fmt == "fp16" else 0, int(scale_exp),
+                                     X.ctypes.data, Y.ctypes.data)
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return Y
+
+
+def fp16_round(a):
+    """Nearest-even binary16 rounding of a float32 array via the oracle."""
+    a = _f32(a)
+    f = load().orc_fp16_round
+    return np.array([f(float(x)) for x in a.ravel()], np.float32).reshape(a.shape)
 
 
 def bf16_round(a):

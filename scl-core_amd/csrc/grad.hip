@@ -39,6 +39,47 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// Reduce-scatter of U partial dot products across a group of G lanes
+// (G >= U, U a power of two): after log2(U) halving exchanges each lane holds
+// the partial sums of one entry, then plain xor-reduction over the remaining
+// lane bits completes it.  Lane li ends with the total of entry
+// e = the lane's bits G/2, G/4, ..., G/U read as a binary number (most
+// significant first).  Costs U - 1 + log2(G/U) shuffles instead of
+// U log2(G).
+template <int G, int U>
+__device__ __forceinline__ float group_reduce_scatter(float (&d)[U], int li) {
+    float v[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) v[q] = d[q];
+    int c = U;
+#pragma unroll
+    for (int m = G / 2; m >= G / U; m >>= 1) {
+        const bool hi = (li & m) != 0;
+        const int h = c / 2;
+#pragma unroll
+        for (int q = 0; q < U / 2; ++q) {
+            if (q < h) {
+                const float send = hi ? v[q] : v[q + h];
+                const float keep = hi ? v[q + h] : v[q];
+                v[q] = keep + __shfl_xor(send, m, 64);
+            }
+        }
+        c = h;
+    }
+    float r = v[0];
+#pragma unroll
+    for (int m = G / (2 * U); m > 0; m >>= 1) r += __shfl_xor(r, m, 64);
+    return r;
+}
+
+template <int G, int U>
+__device__ __forceinline__ int scatter_entry(int li) {
+    int e = 0;
+#pragma unroll
+    for (int m = G / 2; m >= G / U; m >>= 1) e = (e << 1) | ((li & m) ? 1 : 0);
+    return e;
+}
+
 // One backward step over a feature range [f0, f0 + Fp) of rows of stride F.
 // NC chunks of G*VEC features per lane group cover the range.
 template <int VEC, int G, int U, int NC, bool EDGE>
@@ -112,14 +153,25 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_edge(
                 float s = 0.0f;
 #pragma unroll
                 for (int c = 0; c < NC; ++c) s += dotv(g[u][c], xj[c]);
-                d[u] = group_sum<G>(s);
+                d[u] = s;
             }
-            if (li == 0) {
+            if constexpr (G >= U) {
+                const float r = group_reduce_scatter<G, U>(d, li);
+                const int e = scatter_entry<G, U>(li);
+                if ((li & (G / U - 1)) == 0 && k + e < end) {
+                    float* p = dPT + (k + e);
+                    *p = accumulate ? *p + r : r;
+                }
+            } else {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (k + u < end) {
-                        float* p = dPT + (k + u);
-                        *p = accumulate ? *p + d[u] : d[u];
+                for (int u = 0; u < U; ++u) d[u] = group_sum<G>(d[u]);
+                if (li == 0) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (k + u < end) {
+                            float* p = dPT + (k + u);
+                            *p = accumulate ? *p + d[u] : d[u];
+                        }
                     }
                 }
             }
@@ -280,11 +332,7 @@ void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
     if (PT.n <= 0 || F <= 0) return;
     const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
                    reinterpret_cast<uintptr_t>(Xprev);
-    // one column per wave (indices on the scalar unit) whenever 64 lanes
-    // can each own features, with the widest such access
-    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
-    const int vec = (ok4 && F >= 256) ? 4 : (ok2 && F >= 128) ? 2 : (F >= 64) ? 1
-                  : ok4 ? 4 : ok2 ? 2 : 1;
+    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
     const bool edge = Xprev != nullptr && dPT != nullptr;
     // feature passes of at most 8 chunks of 64 lanes; the edge term of the
     // later passes accumulates onto the first pass's value
@@ -322,9 +370,7 @@ void launch_edge_grad(const CsrView& P, const float* Gs, const float* Xprev, flo
                       bool accumulate, int64_t F, hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
     const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Xprev);
-    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
-    const int vec = (ok4 && F >= 256) ? 4 : (ok2 && F >= 128) ? 2 : (F >= 64) ? 1
-                  : ok4 ? 4 : ok2 ? 2 : 1;
+    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
     const int64_t per_pass = (int64_t)64 * vec * 8;
     for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
         const int64_t Fp = std::min<int64_t>(per_pass, F - f0);

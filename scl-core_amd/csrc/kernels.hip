@@ -297,14 +297,11 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
 SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
     SpmmShape s;
     const auto a = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y);
-    const bool ok4 = F % 4 == 0 && a % 16 == 0, ok2 = F % 2 == 0 && a % 8 == 0;
-    // prefer one row per wave (indices on the scalar unit) with the widest
-    // access that still gives all 64 lanes a feature; narrow rows fall back
-    // to several rows per wave
-    if (ok4 && F >= 256) s.vec = 4;
-    else if (ok2 && F >= 128) s.vec = 2;
-    else if (F >= 64) s.vec = 1;
-    else s.vec = ok4 ? 4 : (ok2 ? 2 : 1);
+    // the widest access the feature count and alignment allow: fewer,
+    // larger gathers beat one-row-per-wave layouts with narrow loads
+    if (F % 4 == 0 && a % 16 == 0) s.vec = 4;
+    else if (F % 2 == 0 && a % 8 == 0) s.vec = 2;
+    else s.vec = 1;
     s.group = pow2_ceil((F + s.vec - 1) / s.vec);
     s.unroll = 8;
     s.nontemporal = false;

@@ -77,10 +77,15 @@ void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t
 enum class LowpIo { F32_TO_BF16, BF16_TO_BF16, BF16_TO_F32, F32_TO_F32 };
 
 // One step Y = P X with fp32 accumulation, X and Y stored as the io pair
-// says, P's values read from vals_bf16 when not null (else P.val, fp32).
-// F must be even; rows stored in stored order, as in launch_spmm_f32.
+// says (16-bit side: bf16, or fp16 when fp16 is set), P's values read from
+// vals_bf16 when not null (else P.val, fp32).  A 16-bit input is multiplied
+// by in_scale after widening, a 16-bit output by out_scale before rounding
+// (powers of two, so both are exact).  F must be even.
 void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
-                      const void* X, void* Y, int64_t F, bool use_sched, hipStream_t s);
+                      const void* X, void* Y, int64_t F, bool use_sched, bool fp16,
+                      float in_scale, float out_scale, hipStream_t s);
+// partial[b] = max |x| over a grid-stride slice, for b < n_partial blocks.
+void launch_absmax(const float* x, int64_t m, float* partial, int n_partial, hipStream_t s);
 void launch_to_bf16(const float* src, uint16_t* dst, int64_t m, hipStream_t s);
 void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s);
 

@@ -38,14 +38,30 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {
     return (uint16_t)((u + r) >> 16);
 }
 
+// 16-bit storage formats of the iterates
+struct SBf16 {
+    __device__ static float widen(uint16_t h) { return bf16_to_f32(h); }
+    __device__ static uint16_t narrow(float f) { return f32_to_bf16(f); }
+};
+struct SFp16 {
+    __device__ static float widen(uint16_t h) {
+        return (float)__builtin_bit_cast(_Float16, h);
+    }
+    // round to nearest even (the hardware conversion), overflow -> inf
+    __device__ static uint16_t narrow(float f) {
+        return __builtin_bit_cast(uint16_t, (_Float16)f);
+    }
+};
+
 template <int BYTES> struct Raw;
 template <> struct Raw<4> { using T = uint32_t; };
 template <> struct Raw<8> { using T = uint32_t __attribute__((ext_vector_type(2))); };
 template <> struct Raw<16> { using T = uint32_t __attribute__((ext_vector_type(4))); };
 
-// Loads E elements of type IT starting at p and widens them to fp32.
-template <typename IT, int E>
-__device__ __forceinline__ void load_wide(const IT* p, float (&x)[E]) {
+// Loads E elements of type IT starting at p and widens them to fp32
+// (16-bit elements in storage format ST, multiplied by scale).
+template <typename IT, typename ST, int E>
+__device__ __forceinline__ void load_wide(const IT* p, float (&x)[E], float scale) {
     constexpr int B = E * (int)sizeof(IT);
     using R = typename Raw<B>::T;
     const R r = *reinterpret_cast<const R*>(p);
@@ -62,12 +78,12 @@ __device__ __forceinline__ void load_wide(const IT* p, float (&x)[E]) {
     } else {
 #pragma unroll
         for (int e = 0; e < E; ++e)
-            x[e] = bf16_to_f32((uint16_t)(w[e / 2] >> (16 * (e % 2))));
+            x[e] = ST::widen((uint16_t)(w[e / 2] >> (16 * (e % 2)))) * scale;
     }
 }
 
-template <typename OT, int E>
-__device__ __forceinline__ void store_narrow(OT* p, const float (&a)[E]) {
+template <typename OT, typename ST, int E>
+__device__ __forceinline__ void store_narrow(OT* p, const float (&a)[E], float scale) {
     constexpr int B = E * (int)sizeof(OT);
     using R = typename Raw<B>::T;
     uint32_t w[B / 4];
@@ -77,7 +93,8 @@ __device__ __forceinline__ void store_narrow(OT* p, const float (&a)[E]) {
     } else {
 #pragma unroll
         for (int q = 0; q < B / 4; ++q)
-            w[q] = (uint32_t)f32_to_bf16(a[2 * q]) | ((uint32_t)f32_to_bf16(a[2 * q + 1]) << 16);
+            w[q] = (uint32_t)ST::narrow(a[2 * q] * scale) |
+                   ((uint32_t)ST::narrow(a[2 * q + 1] * scale) << 16);
     }
     R r;
     if constexpr (B == 4) {
@@ -95,12 +112,12 @@ __device__ __forceinline__ float value_of(const VT* v, int k) {
     else return bf16_to_f32(v[k]);
 }
 
-template <typename IT, typename OT, typename VT, int E, int G, int U>
+template <typename IT, typename OT, typename VT, typename ST, int E, int G, int U>
 __global__ __launch_bounds__(kBlock) void k_spmm_mixed(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const VT* __restrict__ val, const int32_t* __restrict__ sched,
     const IT* __restrict__ X, OT* __restrict__ Y, int32_t n, int32_t F,
-    int32_t n_rowblocks) {
+    int32_t n_rowblocks, float in_scale, float out_scale) {
     constexpr int RPW = 64 / G;
     constexpr int RPB = (kBlock / 64) * RPW;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -134,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_mixed(
         while (true) {
             float x[U][E];
 #pragma unroll
-            for (int u = 0; u < U; ++u) load_wide<IT, E>(Xf + (size_t)j[u] * F, x[u]);
+            for (int u = 0; u < U; ++u) load_wide<IT, ST, E>(Xf + (size_t)j[u] * F, x[u], in_scale);
             const int kn = k + U;
             const bool more = kn + U <= end;
             int jn[U];
@@ -169,14 +186,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_mixed(
             v[u] = value_of(val, kk);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) load_wide<IT, E>(Xf + (size_t)j[u] * F, x[u]);
+        for (int u = 0; u < U; ++u) load_wide<IT, ST, E>(Xf + (size_t)j[u] * F, x[u], in_scale);
 #pragma unroll
         for (int u = 0; u < U; ++u)
             if (k + u < end)
 #pragma unroll
                 for (int e = 0; e < E; ++e) acc[e] = __builtin_fmaf(v[u], x[u][e], acc[e]);
     }
-    if (active) store_narrow<OT, E>(Y + (size_t)row * F + f, acc);
+    if (active) store_narrow<OT, ST, E>(Y + (size_t)row * F + f, acc, out_scale);
 }
 
 __global__ __launch_bounds__(kBlock) void k_to_bf16(const float* __restrict__ src,
@@ -193,9 +210,9 @@ __global__ __launch_bounds__(kBlock) void k_from_bf16(const uint16_t* __restrict
 
 constexpr int kLowpU = 8;
 
-template <typename IT, typename OT, typename VT>
+template <typename IT, typename OT, typename VT, typename ST>
 void launch_mixed(const CsrView& P, const VT* vals, const IT* X, OT* Y, int64_t F,
-                  bool use_sched, hipStream_t s) {
+                  bool use_sched, float in_scale, float out_scale, hipStream_t s) {
     // elements per lane: one 16-byte load when the row is wide enough for
     // 64 lanes, else narrower loads that still give every lane of the wave
     // its own features
@@ -218,9 +235,9 @@ void launch_mixed(const CsrView& P, const VT* vals, const IT* X, OT* Y, int64_t 
     const int64_t nb = n_rowblocks * n_slabs;
     const int32_t* sched = use_sched ? P.sched : nullptr;
 #define SCL_LAUNCH_MIXED(EE, GG)                                                       \
-    hipLaunchKernelGGL((k_spmm_mixed<IT, OT, VT, EE, GG, kLowpU>), dim3((unsigned)nb), \
-                       dim3(kBlock), 0, s, P.row_ptr, P.col, vals, sched, X, Y, P.n,   \
-                       (int32_t)F, (int32_t)n_rowblocks)
+    hipLaunchKernelGGL((k_spmm_mixed<IT, OT, VT, ST, EE, GG, kLowpU>), dim3((unsigned)nb), \
+                       dim3(kBlock), 0, s, P.row_ptr, P.col, vals, sched, X, Y, P.n,       \
+                       (int32_t)F, (int32_t)n_rowblocks, in_scale, out_scale)
 #define SCL_LAUNCH_MIXED_G(EE)                       \
     switch (G) {                                     \
         case 16: SCL_LAUNCH_MIXED(EE, 16); break;    \
@@ -257,28 +274,66 @@ void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s)
     check_launch("bf16 convert launch");
 }
 
-void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
-                      const void* X, void* Y, int64_t F, bool use_sched, hipStream_t s) {
-    if (P.n <= 0 || F <= 0) return;
-    const bool vb = vals_bf16 != nullptr;
+template <typename ST>
+void launch_lowp_fmt(const CsrView& P, const uint16_t* vb, LowpIo io, const void* X,
+                     void* Y, int64_t F, bool sched, float in_scale, float out_scale,
+                     hipStream_t s) {
+    using U16 = uint16_t;
     switch (io) {
         case LowpIo::F32_TO_BF16:
-            if (vb) launch_mixed<float, uint16_t, uint16_t>(P, vals_bf16, (const float*)X, (uint16_t*)Y, F, use_sched, s);
-            else launch_mixed<float, uint16_t, float>(P, P.val, (const float*)X, (uint16_t*)Y, F, use_sched, s);
+            if (vb) launch_mixed<float, U16, U16, ST>(P, vb, (const float*)X, (U16*)Y, F, sched, in_scale, out_scale, s);
+            else launch_mixed<float, U16, float, ST>(P, P.val, (const float*)X, (U16*)Y, F, sched, in_scale, out_scale, s);
             break;
         case LowpIo::BF16_TO_BF16:
-            if (vb) launch_mixed<uint16_t, uint16_t, uint16_t>(P, vals_bf16, (const uint16_t*)X, (uint16_t*)Y, F, use_sched, s);
-            else launch_mixed<uint16_t, uint16_t, float>(P, P.val, (const uint16_t*)X, (uint16_t*)Y, F, use_sched, s);
+            if (vb) launch_mixed<U16, U16, U16, ST>(P, vb, (const U16*)X, (U16*)Y, F, sched, in_scale, out_scale, s);
+            else launch_mixed<U16, U16, float, ST>(P, P.val, (const U16*)X, (U16*)Y, F, sched, in_scale, out_scale, s);
             break;
         case LowpIo::BF16_TO_F32:
-            if (vb) launch_mixed<uint16_t, float, uint16_t>(P, vals_bf16, (const uint16_t*)X, (float*)Y, F, use_sched, s);
-            else launch_mixed<uint16_t, float, float>(P, P.val, (const uint16_t*)X, (float*)Y, F, use_sched, s);
+            if (vb) launch_mixed<U16, float, U16, ST>(P, vb, (const U16*)X, (float*)Y, F, sched, in_scale, out_scale, s);
+            else launch_mixed<U16, float, float, ST>(P, P.val, (const U16*)X, (float*)Y, F, sched, in_scale, out_scale, s);
             break;
         case LowpIo::F32_TO_F32:
-            if (vb) launch_mixed<float, float, uint16_t>(P, vals_bf16, (const float*)X, (float*)Y, F, use_sched, s);
-            else launch_mixed<float, float, float>(P, P.val, (const float*)X, (float*)Y, F, use_sched, s);
+            if (vb) launch_mixed<float, float, U16, ST>(P, vb, (const float*)X, (float*)Y, F, sched, in_scale, out_scale, s);
+            else launch_mixed<float, float, float, ST>(P, P.val, (const float*)X, (float*)Y, F, sched, in_scale, out_scale, s);
             break;
     }
+}
+
+void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
+                      const void* X, void* Y, int64_t F, bool use_sched, bool fp16,
+                      float in_scale, float out_scale, hipStream_t s) {
+    if (P.n <= 0 || F <= 0) return;
+    if (fp16)
+        launch_lowp_fmt<SFp16>(P, vals_bf16, io, X, Y, F, use_sched, in_scale, out_scale, s);
+    else
+        launch_lowp_fmt<SBf16>(P, vals_bf16, io, X, Y, F, use_sched, in_scale, out_scale, s);
+}
+
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_absmax_partial(const float* __restrict__ x,
+                                                           int64_t m,
+                                                           float* __restrict__ partial) {
+    __shared__ float sh[kBlock / 64];
+    float v = 0.0f;
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < m;
+         p += (int64_t)gridDim.x * kBlock)
+        v = fmaxf(v, fabsf(x[p]));
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = sh[0];
+        for (int q = 1; q < kBlock / 64; ++q) t = fmaxf(t, sh[q]);
+        partial[blockIdx.x] = t;
+    }
+}
+
+}  // namespace
+
+void launch_absmax(const float* x, int64_t m, float* partial, int n_partial, hipStream_t s) {
+    hipLaunchKernelGGL(k_absmax_partial, dim3(n_partial), dim3(kBlock), 0, s, x, m, partial);
+    check_launch("absmax launch");
 }
 
 }  // namespace scl_amd

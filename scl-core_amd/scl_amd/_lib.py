@@ -90,8 +90,10 @@ SIGNATURES = {
     "scl_gpu_spmm_get_shape": (c_err, [P, c_index, ctypes.c_int, P, P, P, P]),
     "scl_gpu_spmm_autotune": (c_err, [P, P, P, c_index, c_index, ctypes.c_int, P, P]),
     "scl_gpu_sparse_set_values_device": (c_err, [P, P, P]),
-    "scl_gpu_diffuse_matrix_bf16_device": (
-        c_err, [P, P, P, P, c_index, c_index, c_index, ctypes.c_int, P]),
+    "scl_gpu_diffuse_matrix_lowp_device": (
+        c_err, [P, P, P, P, c_index, c_index, c_index, ctypes.c_int, ctypes.c_int, P]),
+    "scl_gpu_lowp_scale_exp": (
+        c_err, [P, P, c_index, c_index, c_index, ctypes.POINTER(ctypes.c_int), P]),
     "scl_gpu_diffuse_forward_ckpt_device": (
         c_err, [P, P, P, P, P, c_index, c_index, c_index, c_index, P]),
     "scl_gpu_edge_gradient_device": (

@@ -38,36 +38,46 @@ def diffuse(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> torch.Tensor:
     return Y
 
 
-def diffuse_bf16(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
-                 bf16_values: bool = False, schedule: bool = True) -> torch.Tensor:
-    """Returns P^t X (fp32) with bf16 intermediate iterates and fp32 accumulation."""
+def lowp_scale_exp(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> int:
+    """Power-of-two exponent that keeps fp16 iterates of P^t X in range."""
     _check_dense(X)
     n, F = X.shape
-    Y = torch.empty_like(X)
-    work = torch.empty((2, n, F), dtype=torch.bfloat16, device=X.device) \
-        if n_steps >= 2 else None
-    flags = (1 if bf16_values else 0) | (0 if schedule else 2)
+    k = ctypes.c_int(0)
     with torch.cuda.device(X.device):
-        _lib.check(_lib.load().scl_gpu_diffuse_matrix_bf16_device(
+        _lib.check(_lib.load().scl_gpu_lowp_scale_exp(
+            P.handle, X.data_ptr(), n, F, n_steps, ctypes.byref(k), _stream_ptr(X)))
+    return k.value
+
+
+def diffuse_lowp(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
+                 fmt: str = "bf16", bf16_values: bool = False, schedule: bool = True,
+                 scale_exp=None) -> torch.Tensor:
+    """Returns P^t X (fp32) with 16-bit intermediate iterates ("bf16" or
+    "fp16"; fp16 scaled by 2^scale_exp, computed when not given) and fp32
+    accumulation."""
+    _check_dense(X)
+    n, F = X.shape
+    fp16 = fmt == "fp16"
+    if fmt not in ("bf16", "fp16"):
+        raise ValueError("fmt must be 'bf16' or 'fp16'")
+    if scale_exp is None:
+        scale_exp = lowp_scale_exp(P, X, n_steps) if fp16 else 0
+    Y = torch.empty_like(X)
+    work = torch.empty((2, n, F), dtype=torch.float16 if fp16 else torch.bfloat16,
+                       device=X.device) if n_steps >= 2 else None
+    flags = (1 if bf16_values else 0) | (0 if schedule else 2) | (4 if fp16 else 0)
+    with torch.cuda.device(X.device):
+        _lib.check(_lib.load().scl_gpu_diffuse_matrix_lowp_device(
             P.handle, X.data_ptr(), Y.data_ptr(),
             None if work is None else work.data_ptr(), n, F, n_steps, flags,
-            _stream_ptr(X)))
+            int(scale_exp), _stream_ptr(X)))
     return Y
 
 
-def edge_gradient(P: SparseMatrixBase, G: torch.Tensor, Xprev: torch.Tensor,
-                  dP: torch.Tensor = None, accumulate: bool = False) -> torch.Tensor:
-    """dP[k] (+)= <G[i,:], Xprev[j,:]> for every nonzero k = (i, j) of P."""
-    _check_dense(G)
-    _check_dense(Xprev)
-    n, F = G.shape
-    if dP is None:
-        dP = torch.empty(max(P.nnz, 1), dtype=torch.float32, device=G.device)
-    with torch.cuda.device(G.device):
-        _lib.check(_lib.load().scl_gpu_edge_gradient_device(
-            P.handle, G.data_ptr(), Xprev.data_ptr(), dP.data_ptr(), n, F,
-            1 if accumulate else 0, _stream_ptr(G)))
-    return dP
+def diffuse_bf16(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
+                 bf16_values: bool = False, schedule: bool = True) -> torch.Tensor:
+    """P^t X with bf16 intermediate iterates (see diffuse_lowp)."""
+    return diffuse_lowp(P, X, n_steps, "bf16", bf16_values, schedule)
 
 
 def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False

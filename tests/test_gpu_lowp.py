@@ -1,7 +1,8 @@
-"""GPU parity of the bf16-iterate diffusion against its CPU restatement
-(oracle orc_diffuse_bf16_f32: the same fp32 FMA chains in stored order, the
-same nearest-even bf16 rounding of every intermediate iterate), bit for bit,
-plus the error against FP64 stated for the uniform15 workload shape."""
+"""GPU parity of the 16-bit-iterate diffusion (bf16, and fp16 with a
+power-of-two scale) against its CPU restatement (oracle orc_diffuse_lowp_f32:
+the same fp32 FMA chains in stored order, the same nearest-even rounding of
+every intermediate iterate), bit for bit, plus the error against FP64 stated
+for the uniform15 workload shape."""
 from __future__ import annotations
 
 import numpy as np
@@ -27,6 +28,23 @@ def test_bf16_path_matches_restatement(gpu, F, t, vb):
     assert np.array_equal(Y.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("F,t,vb", [(2, 3, False), (64, 4, True), (256, 4, False),
+                                    (300, 3, True), (1000, 2, False)])
+def test_fp16_path_matches_restatement(gpu, F, t, vb):
+    n = 20_000 if F <= 256 else 5_000
+    g = synth.uniform_knn(n, 15, seed=F + t + 1).normalized()
+    X = synth.normal(n, F, seed=F + 1) * np.float32(37.0)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd = torch.from_numpy(X).cuda()
+    k = torch_ops.lowp_scale_exp(P, Xd, t)
+    # 2^k * max|X| stays within 2^15 and is not needlessly small
+    assert 2.0 ** 14 < 2.0 ** k * np.abs(X).max() * 1.0000001 <= 2.0 ** 15
+    Y = torch_ops.diffuse_lowp(P, Xd, t, "fp16", bf16_values=vb, scale_exp=k)
+    ref = orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, t, "fp16",
+                               bf16_values=vb, scale_exp=k)
+    assert np.array_equal(Y.cpu().numpy(), ref)
+
+
 def test_bf16_error_vs_fp64_uniform15(gpu):
     """n = 200,000 (the 1M-cell shape scaled down 5x), k = 15 uniform
     neighbours, F = 256 N(0,1), t = 4."""
@@ -39,18 +57,20 @@ def test_bf16_error_vs_fp64_uniform15(gpu):
     y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X[:, cols], t)
     y64q = orc.diffuse_f64(g.indptr, g.indices, g.data, orc.bf16_round(X[:, cols]), t)
     rows = []
-    for vb in (False, True):
-        Y = torch_ops.diffuse_bf16(P, Xd, t, bf16_values=vb)[:, torch.from_numpy(cols).cuda()]
+    for fmt, vb in (("bf16", False), ("bf16", True), ("fp16", False), ("fp16", True)):
+        Y = torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb)[:, torch.from_numpy(cols).cuda()]
         Yh = Y.cpu().numpy()
-        rows.append((vb, orc.rel_err(Yh, y64), orc.median_rel_err(Yh, y64)))
+        rows.append((fmt, vb, orc.rel_err(Yh, y64), orc.median_rel_err(Yh, y64)))
     Y32 = torch_ops.diffuse(P, Xd, t)[:, torch.from_numpy(cols).cuda()].cpu().numpy()
     e32 = orc.rel_err(Y32, y64)
     Xq = torch.from_numpy(orc.bf16_round(X)).cuda()
     Yq = torch_ops.diffuse_bf16(P, Xq, t)[:, torch.from_numpy(cols).cuda()].cpu().numpy()
     eq = orc.rel_err(Yq, y64q)
     print(f"\nfp32 kernel vs FP64: {e32:.2e}")
-    for vb, emax, emed in rows:
-        print(f"bf16 iterates, {'bf16' if vb else 'fp32'} values: max {emax:.2e} median {emed:.2e}")
+    for fmt, vb, emax, emed in rows:
+        print(f"{fmt} iterates, {'bf16' if vb else 'fp32'} values: max {emax:.2e} median {emed:.2e}")
     print(f"bf16 iterates on bf16-rounded input vs FP64 of that input: {eq:.2e}")
     assert e32 <= 1e-5
-    assert all(emax < 1e-2 for _, emax, _ in rows)
+    assert all(emax < 1e-2 for _, _, emax, _ in rows)
+    # fp16 iterates with fp32 values meet the 1e-3 bar
+    assert rows[2][2] <= 1e-3

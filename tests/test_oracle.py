@@ -272,3 +272,37 @@ def test_bf16_restatement_reduces_to_fp32_when_nothing_rounds():
     y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X, 4)
     e = orc.rel_err(orc.diffuse_bf16_f32(g.indptr, g.indices, g.data, X, 4), y64)
     assert 1e-5 < e < 3 * 2.0 ** -8
+
+
+def test_fp16_rounding_matches_numpy():
+    rng = np.random.Generator(np.random.PCG64(19))
+    x = np.concatenate([
+        rng.standard_normal(20_000).astype(np.float32),
+        (rng.standard_normal(5_000) * 1e-6).astype(np.float32),     # subnormal fp16
+        (rng.standard_normal(2_000) * 3e4).astype(np.float32),      # near overflow
+        np.float32([65504, 65519, 65520, 7e4, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26,
+                    1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11, -0.0, 0.0])])
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    got = orc.fp16_round(x)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_fp16_restatement_scaling_is_exact_and_reduces():
+    g = synth.uniform_knn(600, 8, seed=6).normalized()
+    X = synth.normal(600, 4, seed=6)
+    # t = 1 or nothing to round: equals the FP32 restatement for any scale
+    assert np.array_equal(orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 1, "fp16", scale_exp=7),
+                          orc.diffuse_matrix_f32(g.indptr, g.indices, g.data, X, 1))
+    # t = 2 by hand with a scale of 2^5
+    mid = orc.spmm_step_f32(g.indptr, g.indices, g.data, X)
+    mid = (orc.fp16_round(mid * np.float32(32.0)) / np.float32(32.0)).astype(np.float32)
+    exp = orc.spmm_step_f32(g.indptr, g.indices, g.data, mid)
+    got = orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 2, "fp16", scale_exp=5)
+    assert np.array_equal(got, exp)
+    # fp16 iterates are ~8x closer to FP64 than bf16 iterates
+    y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X, 4)
+    e16 = orc.rel_err(orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 4, "fp16",
+                                           scale_exp=14), y64)
+    eb = orc.rel_err(orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 4, "bf16"), y64)
+    assert e16 < eb / 4 and e16 < 1e-3
