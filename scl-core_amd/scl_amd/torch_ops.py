@@ -80,6 +80,21 @@ def diffuse_bf16(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
     return diffuse_lowp(P, X, n_steps, "bf16", bf16_values, schedule)
 
 
+def edge_gradient(P: SparseMatrixBase, G: torch.Tensor, Xprev: torch.Tensor,
+                  dP: torch.Tensor = None, accumulate: bool = False) -> torch.Tensor:
+    """dP[k] (+)= <G[i,:], Xprev[j,:]> for every nonzero k = (i, j) of P."""
+    _check_dense(G)
+    _check_dense(Xprev)
+    n, F = G.shape
+    if dP is None:
+        dP = torch.empty(max(P.nnz, 1), dtype=torch.float32, device=G.device)
+    with torch.cuda.device(G.device):
+        _lib.check(_lib.load().scl_gpu_edge_gradient_device(
+            P.handle, G.data_ptr(), Xprev.data_ptr(), dP.data_ptr(), n, F,
+            1 if accumulate else 0, _stream_ptr(G)))
+    return dP
+
+
 def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
              ) -> float:
     """Picks the fastest SpMM launch shape for P and X's width; returns ms/launch."""

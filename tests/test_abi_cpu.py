@@ -196,3 +196,13 @@ def test_block_diag_stacks_handles():
     arr = (ctypes.c_void_p * 2)(A.handle.value, C.handle.value)
     assert lib.scl_sparse_block_diag(arr, 2, ctypes.byref(out)) == 21  # TYPE_MISMATCH
     assert lib.scl_sparse_block_diag(arr, 0, ctypes.byref(out)) == 10
+
+
+def test_python_layer_surface():
+    """The torch-facing helpers exist (they are exercised by the GPU tests)."""
+    from scl_amd import autograd, batch, torch_ops
+    for name in ("diffuse", "spmm", "edge_gradient", "diffuse_lowp", "diffuse_bf16",
+                 "lowp_scale_exp", "autotune", "get_shape", "set_shape"):
+        assert callable(getattr(torch_ops, name)), name
+    assert callable(autograd.diffuse) and callable(batch.block_diag)
+    assert hasattr(batch, "DiffusionPlan")
