@@ -283,6 +283,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--legs", default="c2,c3,c4,c5")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="vec,group,unroll,flags for the headline SpMM instead of the "
+                         "autotuner (flags as SCL_GPU_SHAPE_*)")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="also run one plain 1 GiB device copy (the reference kernel "
+                         "for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the "
@@ -313,7 +319,20 @@ def main() -> int:
     lib = scl_amd.load()
     stream = torch.cuda.current_stream(dev)
     tune_ms = None
-    if not args.no_autotune:
+    if args.calibration_copy:
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read, 1 GiB written")
+        del a, b
+    if args.shape:
+        v, gr, u, fl = (int(x) for x in args.shape.split(","))
+        torch_ops.set_shape(A, F, v, gr, u, nontemporal=bool(fl & 1),
+                            schedule=not (fl & 2), xcd_rows=bool(fl & 4),
+                            scalar_rows=bool(fl & 8))
+        log(f"shape set to {torch_ops.get_shape(A, F)}")
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         tune_ms = torch_ops.autotune(A, Xd)
         log(f"autotune {time.perf_counter() - t0:.1f}s -> {torch_ops.get_shape(A, F)}")

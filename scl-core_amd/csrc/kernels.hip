@@ -322,7 +322,9 @@ std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const voi
             for (int u : {4, 8, 16})
                 for (int nt = 0; nt < 2; ++nt)
                     for (int xr = 0; xr < 2; ++xr)
-                        for (int sr = 0; sr < ((vec == 4 && (g == 16 || g == 32)) ? 2 : 1); ++sr)
+                        // scalar multi-row only with two rows per wave: four
+                        // rows per wave measured 5-10x slower everywhere
+                        for (int sr = 0; sr < ((vec == 4 && g == 32) ? 2 : 1); ++sr)
                             for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
                                 SpmmShape s;
                                 s.vec = vec;
