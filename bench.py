@@ -115,6 +115,8 @@ def leg_c3(steps: int) -> dict:
     X = torch.from_numpy(synth.normal(n, F, seed=3)).to(dev)
     T = torch.from_numpy(synth.normal(n, F, seed=103)).to(dev)
     torch_ops.autotune(P, X)
+    torch_ops.autotune(P, X, transpose=True)   # the plain path's adjoint SpMM
+    torch_ops.autotune(P, X, adjoint=True)     # the fused path's adjoint step
     plan = batch.DiffusionPlan(P, n, F, t, every=every)
     plan.X.copy_(X)
     plan.T.copy_(T)
@@ -219,6 +221,7 @@ def leg_c5(steps: int) -> dict:
     X = torch.from_numpy(synth.normal(N, F, seed=5)).to(dev)
     T = torch.from_numpy(synth.normal(N, F, seed=105)).to(dev)
     torch_ops.autotune(B, X)
+    torch_ops.autotune(B, X, adjoint=True)
     plan = batch.DiffusionPlan(B, N, F, t, every=every)
     plan.X.copy_(X)
     plan.T.copy_(T)

@@ -74,7 +74,9 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 
 /* ---------------------------------------------------------------- shapes --
  * Launch shape of the row-split SpMM for one operator (transpose == 0) or its
- * transpose (transpose != 0) and one feature count:
+ * transpose (transpose == 1), or of the fused adjoint + edge-gradient step of
+ * the reverse pass (transpose == 2: vec, group and SCL_GPU_SHAPE_NO_SCHEDULE;
+ * unroll is 8 and the other flags are ignored), and one feature count:
  *   vec     floats per lane per access: 1, 2 or 4 (must divide n_features)
  *   group   lanes sharing one row: a power of two in 1..64; one launch covers
  *           group * vec features of every row per workgroup column (a slab)
@@ -88,7 +90,9 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           SCL_GPU_SHAPE_SCALAR_ROWS: with group 16 or 32, the 64/group
  *           rows of a wave keep their indices and values on the scalar unit
  *           (each lane selects its row's) instead of loading them per lane
- * Any shape gives bit-identical results; only the speed differs. */
+ * Any SpMM shape gives bit-identical results; only the speed differs.  The
+ * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
+ * are added in a shape-dependent order (fp32 rounding differences only). */
 #define SCL_GPU_SHAPE_NONTEMPORAL 1
 #define SCL_GPU_SHAPE_NO_SCHEDULE 2
 #define SCL_GPU_SHAPE_XCD_ROWS 4
@@ -109,7 +113,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
 /* Times the candidate shapes for this operator and feature count on the
  * caller's buffers (Y_dev is overwritten), keeps the fastest for later calls
  * and returns its time per launch in best_ms (nullable).  Synchronises
- * `stream`. */
+ * `stream`.  transpose == 2 tunes the fused adjoint + edge-gradient step
+ * (X_dev stands in for both of its inputs). */
 SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
                                              const scl_real_t* X_dev,
                                              scl_real_t* Y_dev,

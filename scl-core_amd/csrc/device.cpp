@@ -177,6 +177,7 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->fingerprint = fp;
             slot->shapes.clear();
             slot->t_shapes.clear();
+            slot->adj_shapes.clear();
         }
     }
     if (slot->pins > 0 && slot->t_row_ptr) {

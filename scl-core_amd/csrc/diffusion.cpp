@@ -338,6 +338,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     SCL_API_CHECK_NULL(transition, "Transition matrix is null");
     SCL_API_CHECK(n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
                   "n_features must be positive");
+    SCL_API_CHECK(transpose >= 0 && transpose <= 2, SCL_ERROR_INVALID_ARGUMENT,
+                  "transpose must be 0 (P), 1 (P^T) or 2 (fused adjoint)");
     SpmmShape sh;
     sh.vec = vec;
     sh.group = group;
@@ -346,15 +348,26 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.schedule = (flags & kShapeNoSchedule) == 0;
     sh.xcd_rows = (flags & kShapeXcdRows) != 0;
     sh.scalar_rows = (flags & kShapeScalarRows) != 0;
-    SCL_API_CHECK(spmm_shape_valid(sh, n_features, nullptr, nullptr),
-                  SCL_ERROR_INVALID_ARGUMENT,
-                  "invalid shape: vec in {1,2,4} dividing n_features, group a "
-                  "power of two in 1..64, unroll 4, 8 or 16");
+    if (transpose == 2) {
+        sh.unroll = 8;
+        sh.nontemporal = false;
+        sh.xcd_rows = false;
+        sh.scalar_rows = false;
+        SCL_API_CHECK(adjoint_shape_valid(sh, n_features, nullptr, nullptr, nullptr),
+                      SCL_ERROR_INVALID_ARGUMENT,
+                      "invalid adjoint shape: vec in {1,2,4} dividing n_features, "
+                      "group a power of two in 1..64");
+    } else {
+        SCL_API_CHECK(spmm_shape_valid(sh, n_features, nullptr, nullptr),
+                      SCL_ERROR_INVALID_ARGUMENT,
+                      "invalid shape: vec in {1,2,4} dividing n_features, group a "
+                      "power of two in 1..64, unroll 4, 8 or 16");
+    }
     SCL_API_TRY
         const int dev = current_device();
         std::lock_guard<std::mutex> lk(transition->mu);
         DeviceCSR& d = device_csr(transition, dev);
-        (transpose ? d.t_shapes : d.shapes)[n_features] = sh;
+        (transpose == 2 ? d.adj_shapes : transpose ? d.t_shapes : d.shapes)[n_features] = sh;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }
@@ -369,13 +382,29 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
                   "Output pointer is null");
     SCL_API_CHECK(n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
                   "n_features must be positive");
+    SCL_API_CHECK(transpose >= 0 && transpose <= 2, SCL_ERROR_INVALID_ARGUMENT,
+                  "transpose must be 0 (P), 1 (P^T) or 2 (fused adjoint)");
     SCL_API_TRY
         const int dev = current_device();
         std::lock_guard<std::mutex> lk(transition->mu);
         DeviceCSR& d = device_csr(transition, dev);
         // the choice for 16-byte aligned buffers
-        const SpmmShape sh = choose_shape(d, transpose != 0, n_features,
-                                          nullptr, nullptr);
+        SpmmShape sh;
+        if (transpose == 2) {
+            const auto it = d.adj_shapes.find(n_features);
+            if (it != d.adj_shapes.end()) {
+                sh = it->second;
+            } else {  // the launcher's default: widest vec, one row per group
+                sh.vec = n_features % 4 == 0 ? 4 : n_features % 2 == 0 ? 2 : 1;
+                sh.group = 64;
+            }
+            sh.unroll = 8;
+            sh.nontemporal = false;
+            sh.xcd_rows = false;
+            sh.scalar_rows = false;
+        } else {
+            sh = choose_shape(d, transpose != 0, n_features, nullptr, nullptr);
+        }
         *vec = sh.vec;
         *group = sh.group;
         *unroll = sh.unroll;
@@ -401,6 +430,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
                   "Dimensions must be positive");
     SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev,
                   SCL_ERROR_INVALID_ARGUMENT, "X and Y must be distinct");
+    SCL_API_CHECK(transpose >= 0 && transpose <= 2, SCL_ERROR_INVALID_ARGUMENT,
+                  "transpose must be 0 (P), 1 (P^T) or 2 (fused adjoint)");
     SCL_API_TRY
         check_operator(transition, n_nodes, n_features);
         const int dev = current_device();
@@ -409,8 +440,25 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         DeviceCSR& d = device_csr(transition, dev);
         if (transpose) ensure_transpose(transition, d, s);
         const CsrView v = transpose ? transposed_view_of(d) : view_of(d);
-        const auto cands = spmm_candidate_shapes(n_features, X_dev, Y_dev,
-                                                 v.sched != nullptr);
+        // the fused adjoint is timed with its edge term on: X stands in for
+        // both G_s and X_{s-1}, and the per-nonzero output goes to scratch
+        float* dpt = nullptr;
+        if (transpose == 2 && d.nnz > 0)
+            dpt = static_cast<float*>(device_workspace(dev, 4, (size_t)d.nnz * sizeof(float)));
+        const auto cands = transpose == 2
+                               ? adjoint_candidate_shapes(n_features, X_dev, Y_dev, nullptr,
+                                                          v.sched != nullptr)
+                               : spmm_candidate_shapes(n_features, X_dev, Y_dev,
+                                                       v.sched != nullptr);
+        SCL_API_CHECK(!cands.empty(), SCL_ERROR_INVALID_ARGUMENT,
+                      "no launch shape fits these buffers");
+        auto run = [&](const SpmmShape& c) {
+            if (transpose == 2)
+                launch_adjoint_edge(v, X_dev, dpt ? X_dev : nullptr, Y_dev, dpt, false,
+                                    n_features, s, &c);
+            else
+                launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);
+        };
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
@@ -418,9 +466,9 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         SpmmShape best_shape = cands.front();
         try {
             for (const SpmmShape& c : cands) {
-                launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);  // warm
+                run(c);  // warm
                 hip_check(hipEventRecord(e0, s), "event");
-                for (int r = 0; r < 2; ++r) launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);
+                for (int r = 0; r < 2; ++r) run(c);
                 hip_check(hipEventRecord(e1, s), "event");
                 hip_check(hipEventSynchronize(e1), "event");
                 float ms = 0.0f;
@@ -437,7 +485,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        (transpose ? d.t_shapes : d.shapes)[n_features] = best_shape;
+        (transpose == 2 ? d.adj_shapes : transpose ? d.t_shapes : d.shapes)[n_features] =
+            best_shape;
         if (best_ms) *best_ms = best / 2.0f;
         SCL_API_RETURN_OK;
     SCL_API_CATCH

@@ -190,7 +190,8 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
         const int dev = current_device();
         hipStream_t s = static_cast<hipStream_t>(stream);
         CsrView fv, av;
-        SpmmShape sh;
+        SpmmShape sh, adj;
+        bool has_adj = false;
         const int32_t* t_src = nullptr;
         {
             std::lock_guard<std::mutex> lk(transition->mu);
@@ -200,7 +201,13 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
             av = adj_view(d);
             t_src = d.t_src;
             sh = fwd_shape(d, F, X_dev, work_dev);
+            const auto ai = d.adj_shapes.find(F);
+            if (ai != d.adj_shapes.end()) {
+                adj = ai->second;
+                has_adj = true;
+            }
         }
+        const SpmmShape* adj_sh = has_adj ? &adj : nullptr;
         const size_t plane = (size_t)n * (size_t)F;
         const size_t plane_b = round_up(plane * sizeof(float));
         char* w = static_cast<char*>(work_dev);
@@ -228,7 +235,8 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
             for (int64_t st = b; st > a; --st) {
                 const float* Xprev = (st - 1 == a) ? base : rslot(st - 1 - a);
                 float* Gn = (st == 1) ? dX_dev : gbuf[st % 2];
-                launch_adjoint_edge(av, G, want_dp ? Xprev : nullptr, Gn, dPT, !first, F, s);
+                launch_adjoint_edge(av, G, want_dp ? Xprev : nullptr, Gn, dPT, !first, F, s,
+                                    adj_sh);
                 first = false;
                 G = Gn;
             }

@@ -32,56 +32,93 @@ __device__ __forceinline__ float dotv(f4 a, f4 b) { return dot4(a, b); }
 __device__ __forceinline__ float dotv(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
 __device__ __forceinline__ float dotv(float a, float b) { return a * b; }
 
+// Cross-lane sums.  Inside a 16-lane row, stage b pairs lane li with
+// li ^ (2^(b+1) - 1) through DPP (quad_perm, row_half_mirror, row_mirror),
+// which costs no LDS traffic; lanes 16 and 32 apart are paired with
+// __shfl_xor.  Successive stages use linearly independent masks, so a chain
+// over all of a group's bits sums every lane of the group exactly once.
+template <int B>
+__device__ __forceinline__ float dpp_partner(float v) {
+    static_assert(B >= 0 && B < 4, "DPP stages cover bits 0..3");
+    constexpr int ctrl = B == 0 ? 0xB1 : B == 1 ? 0x1B : B == 2 ? 0x141 : 0x140;
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF,
+                                           false));
+}
+
+// Lanes with bit B clear end with a(self) + a(partner), lanes with bit B set
+// with b(self) + b(partner).
+template <int B>
+__device__ __forceinline__ float pair_reduce(float a, float b, bool hi) {
+    const float send = hi ? a : b;
+    const float keep = hi ? b : a;
+    return keep + dpp_partner<B>(send);
+}
+
+// Sum over bits B-1 .. 0 of the lane index (B <= 4).
+template <int B>
+__device__ __forceinline__ float sum_bits_below(float v) {
+    if constexpr (B > 0) return sum_bits_below<B - 1>(v + dpp_partner<B - 1>(v));
+    else return v;
+}
+
 template <int G>
-__device__ __forceinline__ float group_sum(float v) {
-#pragma unroll
-    for (int m = G / 2; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+__device__ __forceinline__ constexpr int log2i() {
+    return G <= 1 ? 0 : 1 + log2i<G / 2>();
+}
+
+// Sum over lanes 16 and 32 apart, for groups wider than one row.
+template <int G>
+__device__ __forceinline__ float sum_across_rows(float v) {
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
     return v;
 }
 
-// Reduce-scatter of U partial dot products across a group of G lanes
-// (G >= U, U a power of two): after log2(U) halving exchanges each lane holds
-// the partial sums of one entry, then plain xor-reduction over the remaining
-// lane bits completes it.  Lane li ends with the total of entry
-// e = the lane's bits G/2, G/4, ..., G/U read as a binary number (most
-// significant first).  Costs U - 1 + log2(G/U) shuffles instead of
-// U log2(G).
-template <int G, int U>
-__device__ __forceinline__ float group_reduce_scatter(float (&d)[U], int li) {
-    float v[U];
-#pragma unroll
-    for (int q = 0; q < U; ++q) v[q] = d[q];
-    int c = U;
-#pragma unroll
-    for (int m = G / 2; m >= G / U; m >>= 1) {
-        const bool hi = (li & m) != 0;
-        const int h = c / 2;
-#pragma unroll
-        for (int q = 0; q < U / 2; ++q) {
-            if (q < h) {
-                const float send = hi ? v[q] : v[q + h];
-                const float keep = hi ? v[q + h] : v[q];
-                v[q] = keep + __shfl_xor(send, m, 64);
-            }
-        }
-        c = h;
-    }
-    float r = v[0];
-#pragma unroll
-    for (int m = G / (2 * U); m > 0; m >>= 1) r += __shfl_xor(r, m, 64);
-    return r;
+// Sum over the G lanes of a group (G a power of two, groups aligned).
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    constexpr int rb = log2i<G>() < 4 ? log2i<G>() : 4;
+    return sum_across_rows<G>(sum_bits_below<rb>(v));
 }
 
-template <int G, int U>
-__device__ __forceinline__ int scatter_entry(int li) {
-    int e = 0;
+template <int B, int C, int U>
+__device__ __forceinline__ void scatter_stages(float (&v)[U], int li) {
+    if constexpr (C > 1) {
+        const bool hi = ((li >> B) & 1) != 0;
 #pragma unroll
-    for (int m = G / 2; m >= G / U; m >>= 1) e = (e << 1) | ((li & m) ? 1 : 0);
-    return e;
+        for (int q = 0; q < C / 2; ++q) v[q] = pair_reduce<B>(v[q], v[q + C / 2], hi);
+        scatter_stages<B - 1, C / 2, U>(v, li);
+    }
 }
+
+// Reduce-scatter of U partial dot products across a group of G lanes
+// (G >= U, both powers of two).  Within a row, the group's top log2(U) row
+// bits each halve the set of entries a lane carries; the remaining low bits
+// and the cross-row bits are then summed outright.  Lane li ends with the
+// total of entry scatter_entry(li).
+template <int G, int U>
+struct Scatter {
+    static constexpr int kRowBits = log2i<G>() < 4 ? log2i<G>() : 4;
+    static constexpr int kLow = kRowBits - log2i<U>();  // bits summed outright
+    static_assert(kLow >= 0, "reduce-scatter needs U <= min(G, 16)");
+    __device__ static int entry(int li) { return (li >> kLow) & (U - 1); }
+    __device__ static bool writer(int li) {
+        return (li & ((1 << kLow) - 1)) == 0 && (G <= 16 || (li >> 4) == 0);
+    }
+    __device__ static float reduce(float (&d)[U], int li) {
+        float v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = d[q];
+        scatter_stages<kRowBits - 1, U, U>(v, li);
+        return sum_across_rows<G>(sum_bits_below<kLow>(v[0]));
+    }
+};
 
 // One backward step over a feature range [f0, f0 + Fp) of rows of stride F.
-// NC chunks of G*VEC features per lane group cover the range.
+// NC chunks of G*VEC features per lane group cover the range.  The source
+// rows and weights of the next U entries, and the running dPT values of the
+// current ones, are fetched while the current gathers are in flight.
 template <int VEC, int G, int U, int NC, bool EDGE>
 __global__ __launch_bounds__(kBlock) void k_adjoint_edge(
     const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
@@ -124,55 +161,80 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_edge(
 #pragma unroll
         for (int e = 0; e < VEC; ++e) acc[c][e] = 0.0f;
 
-    for (int k = beg; k < end; k += U) {
-        int i[U];
-        float w[U];
+    // entry of the U-chunk this lane writes, and whether it writes at all
+    constexpr bool kScatter = G >= U;
+    using SC = Scatter<kScatter ? G : U, U>;
+    const int my_e = kScatter ? SC::entry(li) : 0;
+    const bool writer = kScatter ? SC::writer(li) : li == 0;
+
+    int i[U];
+    float w[U];
+    if (beg < end) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int kk = min(k + u, end - 1);
+            const int kk = min(beg + u, end - 1);
             i[u] = t_col[kk];
             w[u] = t_val[kk];
         }
+    }
+    for (int k = beg; k < end; k += U) {
         VT g[U][NC];
 #pragma unroll
         for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int c = 0; c < NC; ++c)
                 g[u][c] = *reinterpret_cast<const VT*>(Gs + (size_t)i[u] * F + off[c]);
+        // running dP values of this chunk (kScatter: one per writer lane)
+        float old[kScatter ? 1 : U] = {};
+        if constexpr (EDGE) {
+            if (accumulate && writer) {
+                if constexpr (kScatter) {
+                    old[0] = dPT[min(k + my_e, end - 1)];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) old[u] = dPT[min(k + u, end - 1)];
+                }
+            }
+        }
+        float wc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) wc[u] = w[u];
+        // next chunk's indices (clamped into this row; unused past its end)
+        const int kn = k + U;
+        if (kn < end) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(kn + u, end - 1);
+                i[u] = t_col[kk];
+                w[u] = t_val[kk];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (k + u < end) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) fma_vec(w[u], g[u][c], acc[c]);
+                for (int c = 0; c < NC; ++c) fma_vec(wc[u], g[u][c], acc[c]);
             }
         }
         if constexpr (EDGE) {
             float d[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                float s = 0.0f;
+                float sdot = dotv(g[u][0], xj[0]);
 #pragma unroll
-                for (int c = 0; c < NC; ++c) s += dotv(g[u][c], xj[c]);
-                d[u] = s;
+                for (int c = 1; c < NC; ++c) sdot += dotv(g[u][c], xj[c]);
+                d[u] = sdot;
             }
-            if constexpr (G >= U) {
-                const float r = group_reduce_scatter<G, U>(d, li);
-                const int e = scatter_entry<G, U>(li);
-                if ((li & (G / U - 1)) == 0 && k + e < end) {
-                    float* p = dPT + (k + e);
-                    *p = accumulate ? *p + r : r;
-                }
+            if constexpr (kScatter) {
+                const float r = SC::reduce(d, li);
+                if (writer && k + my_e < end) dPT[k + my_e] = accumulate ? old[0] + r : r;
             } else {
 #pragma unroll
                 for (int u = 0; u < U; ++u) d[u] = group_sum<G>(d[u]);
-                if (li == 0) {
+                if (writer) {
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        if (k + u < end) {
-                            float* p = dPT + (k + u);
-                            *p = accumulate ? *p + d[u] : d[u];
-                        }
-                    }
+                    for (int u = 0; u < U; ++u)
+                        if (k + u < end) dPT[k + u] = accumulate ? old[u] + d[u] : d[u];
                 }
             }
         }
@@ -206,12 +268,7 @@ AdjFn pick_adj(int group, int nc, bool edge) {
             default: return pick_edge<VEC, 32, 1>(edge);
         }
     }
-    switch (nc) {
-        case 1: return pick_edge<VEC, 64, 1>(edge);
-        case 2: return pick_edge<VEC, 64, 2>(edge);
-        case 4: return pick_edge<VEC, 64, 4>(edge);
-        default: return pick_edge<VEC, 64, 8>(edge);
-    }
+    return nc == 1 ? pick_edge<VEC, 64, 1>(edge) : pick_edge<VEC, 64, 2>(edge);
 }
 
 // Standalone edge gradient of one step over the forward pattern: for row i,
@@ -260,9 +317,9 @@ __global__ __launch_bounds__(kBlock) void k_edge_grad(
         float d[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            float sacc = 0.0f;
+            float sacc = dotv(gi[0], x[u][0]);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) sacc += dotv(gi[c], x[u][c]);
+            for (int c = 1; c < NC; ++c) sacc += dotv(gi[c], x[u][c]);
             d[u] = group_sum<G>(sacc);
         }
         if (li == 0) {
@@ -295,12 +352,8 @@ void edge_pick(int group, int nc, const CsrView& P, const float* Gs, const float
         case 32: edge_launch<VEC, 32, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
         default: break;
     }
-    switch (nc) {
-        case 1: edge_launch<VEC, 64, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
-        case 2: edge_launch<VEC, 64, 2>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
-        case 4: edge_launch<VEC, 64, 4>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
-        default: edge_launch<VEC, 64, 8>(P, Gs, Xprev, dP, acc, F, f0, Fp, s); return;
-    }
+    if (nc == 1) edge_launch<VEC, 64, 1>(P, Gs, Xprev, dP, acc, F, f0, Fp, s);
+    else edge_launch<VEC, 64, 2>(P, Gs, Xprev, dP, acc, F, f0, Fp, s);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sub(const float* __restrict__ a,
@@ -326,22 +379,74 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ i
 
 }  // namespace
 
+namespace {
+
+int widest_vec(int64_t F, uintptr_t a) {
+    return (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
+}
+
+bool pow2_in(int g, int lo, int hi) { return g >= lo && g <= hi && (g & (g - 1)) == 0; }
+
+}  // namespace
+
+bool adjoint_shape_valid(const SpmmShape& sh, int64_t F, const void* a, const void* b,
+                         const void* c) {
+    if (sh.vec != 1 && sh.vec != 2 && sh.vec != 4) return false;
+    if (!pow2_in(sh.group, 1, 64) || F % sh.vec != 0) return false;
+    const auto al = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
+                    reinterpret_cast<uintptr_t>(c);
+    return al % (4 * (uintptr_t)sh.vec) == 0;
+}
+
+std::vector<SpmmShape> adjoint_candidate_shapes(int64_t F, const void* a, const void* b,
+                                                const void* c, bool has_schedule) {
+    std::vector<SpmmShape> out;
+    for (int vec : {4, 2, 1}) {
+        const int64_t lanes = (F + vec - 1) / vec;
+        int gmax = 1;
+        while (gmax < lanes && gmax < 64) gmax <<= 1;
+        for (int g = 4; g <= gmax; g <<= 1) {
+            if (g < gmax / 4) continue;             // many passes over the pattern
+            if (vec < 4 && g != gmax) continue;     // narrow loads only as one slab
+            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
+                SpmmShape sh;
+                sh.vec = vec;
+                sh.group = g;
+                sh.schedule = sc != 0;
+                if (adjoint_shape_valid(sh, F, a, b, c)) out.push_back(sh);
+            }
+        }
+    }
+    return out;
+}
+
 void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
                          float* Gp, float* dPT, bool accumulate, int64_t F,
-                         hipStream_t s) {
+                         hipStream_t s, const SpmmShape* shape) {
     if (PT.n <= 0 || F <= 0) return;
-    const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
-                   reinterpret_cast<uintptr_t>(Xprev);
-    const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
     const bool edge = Xprev != nullptr && dPT != nullptr;
-    // feature passes of at most 8 chunks of 64 lanes; the edge term of the
-    // later passes accumulates onto the first pass's value
-    const int64_t per_pass = (int64_t)64 * vec * 8;
+    int vec, gsel;
+    const int32_t* sched = PT.sched;
+    if (shape && adjoint_shape_valid(*shape, F, Gs, Gp, Xprev)) {
+        vec = shape->vec;
+        gsel = shape->group;
+        if (!shape->schedule) sched = nullptr;
+    } else {
+        const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
+                       reinterpret_cast<uintptr_t>(Xprev);
+        vec = widest_vec(F, a);
+        gsel = 64;
+    }
+    // feature passes of at most 2 chunks of 64 lanes (more chunks per lane
+    // cost occupancy: 4 chunks need 180 VGPRs, 8 chunks 256), or of one
+    // group-wide slab for narrower groups; the edge term of the later passes
+    // accumulates onto the first pass's value
+    const int64_t per_pass = gsel < 64 ? (int64_t)gsel * vec : (int64_t)64 * vec * 2;
     for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
         const int64_t Fp = std::min<int64_t>(per_pass, F - f0);
         const int64_t lanes = (Fp + vec - 1) / vec;
         int group = 1;
-        while (group < lanes && group < 64) group <<= 1;
+        while (group < lanes && group < gsel) group <<= 1;
         int nc = 1;
         if (group == 64)
             while ((int64_t)nc * 64 < lanes) nc <<= 1;
@@ -351,7 +456,7 @@ void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
                  : vec == 2 ? pick_adj<2>(group, nc, edge)
                             : pick_adj<1>(group, nc, edge);
         hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(kBlock), 0, s, PT.row_ptr,
-                           PT.col, PT.val, PT.sched, Gs, Xprev, Gp, dPT,
+                           PT.col, PT.val, sched, Gs, Xprev, Gp, dPT,
                            (accumulate || f0 > 0) ? 1 : 0, PT.n, (int32_t)F,
                            (int32_t)f0, (int32_t)Fp);
         check_launch("adjoint launch");
@@ -371,7 +476,7 @@ void launch_edge_grad(const CsrView& P, const float* Gs, const float* Xprev, flo
     if (P.n <= 0 || F <= 0) return;
     const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Xprev);
     const int vec = (F % 4 == 0 && a % 16 == 0) ? 4 : (F % 2 == 0 && a % 8 == 0) ? 2 : 1;
-    const int64_t per_pass = (int64_t)64 * vec * 8;
+    const int64_t per_pass = (int64_t)64 * vec * 2;
     for (int64_t f0 = 0; f0 < F; f0 += per_pass) {
         const int64_t Fp = std::min<int64_t>(per_pass, F - f0);
         const int64_t lanes = (Fp + vec - 1) / vec;

@@ -98,6 +98,7 @@ struct DeviceCSR {
     // launch shapes chosen per feature count (autotuned or set by the
     // caller) for the operator and for its transpose
     std::map<int64_t, SpmmShape> shapes, t_shapes;
+    std::map<int64_t, SpmmShape> adj_shapes;  // fused adjoint + edge kernel
 
     ~DeviceCSR();
     void release_transpose();

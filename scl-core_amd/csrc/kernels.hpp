@@ -55,9 +55,22 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
 //   Gp[j, :] = sum_i p_ij Gs[i, :]               (fp32 FMA, stored order)
 //   dPT[p]  (+)= <Gs[i, :], Xprev[j, :]>          (when Xprev and dPT are set)
 // dPT is indexed by transposed position; accumulate adds onto its contents.
+// shape: vec, group and schedule on/off of the launch (null or invalid for F
+// and the buffers' alignment: the widest vec, one row per group, the
+// operator's schedule); unroll is fixed at 8.
 void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
                          float* Gp, float* dPT, bool accumulate, int64_t F,
-                         hipStream_t s);
+                         hipStream_t s, const SpmmShape* shape = nullptr);
+
+// Whether the fused adjoint kernel can run vec x group on F features with
+// buffers at a, b, c (null pointers are not checked for alignment).
+bool adjoint_shape_valid(const SpmmShape& sh, int64_t F, const void* a, const void* b,
+                         const void* c);
+
+// (vec, group, schedule on/off) the autotuner times for the fused adjoint
+// kernel.
+std::vector<SpmmShape> adjoint_candidate_shapes(int64_t F, const void* a, const void* b,
+                                                const void* c, bool has_schedule);
 
 // Edge gradient of one step over the forward pattern (CSR order):
 // dP[k] (+)= <Gs[i, :], Xprev[j, :]> for every nonzero k = (i, j).

@@ -95,24 +95,32 @@ def edge_gradient(P: SparseMatrixBase, G: torch.Tensor, Xprev: torch.Tensor,
     return dP
 
 
-def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
-             ) -> float:
-    """Picks the fastest SpMM launch shape for P and X's width; returns ms/launch."""
+def _direction(transpose: bool, adjoint: bool) -> int:
+    return 2 if adjoint else (1 if transpose else 0)
+
+
+def autotune(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False,
+             adjoint: bool = False) -> float:
+    """Picks the fastest launch shape for P and X's width; returns ms/launch.
+
+    transpose tunes the P^T SpMM; adjoint tunes the fused adjoint +
+    edge-gradient step of the reverse pass."""
     _check_dense(X)
     n, F = X.shape
     Y = torch.empty_like(X)
     best = ctypes.c_float(0.0)
     with torch.cuda.device(X.device):
         _lib.check(_lib.load().scl_gpu_spmm_autotune(
-            P.handle, X.data_ptr(), Y.data_ptr(), n, F, 1 if transpose else 0,
+            P.handle, X.data_ptr(), Y.data_ptr(), n, F, _direction(transpose, adjoint),
             _stream_ptr(X), ctypes.addressof(best)))
     return best.value
 
 
-def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False) -> dict:
+def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False,
+              adjoint: bool = False) -> dict:
     v, g, u, f = (ctypes.c_int() for _ in range(4))
     _lib.check(_lib.load().scl_gpu_spmm_get_shape(
-        P.handle, n_features, 1 if transpose else 0, ctypes.addressof(v),
+        P.handle, n_features, _direction(transpose, adjoint), ctypes.addressof(v),
         ctypes.addressof(g), ctypes.addressof(u), ctypes.addressof(f)))
     return {"vec": v.value, "group": g.value, "unroll": u.value,
             "nontemporal": bool(f.value & 1), "schedule": not (f.value & 2),
@@ -122,11 +130,11 @@ def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False) -> 
 def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
               unroll: int = 8, nontemporal: bool = False, schedule: bool = True,
               transpose: bool = False, xcd_rows: bool = False,
-              scalar_rows: bool = False) -> None:
+              scalar_rows: bool = False, adjoint: bool = False) -> None:
     flags = ((1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
              | (8 if scalar_rows else 0))
     _lib.check(_lib.load().scl_gpu_spmm_set_shape(
-        P.handle, n_features, 1 if transpose else 0, vec, group, unroll, flags))
+        P.handle, n_features, _direction(transpose, adjoint), vec, group, unroll, flags))
 
 
 def spmm(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False

@@ -120,3 +120,36 @@ def test_unfused_building_blocks_agree_with_fused(gpu):
     (0.5 * ((Y - Td) ** 2).sum()).backward()
     assert torch.equal(G, Xr.grad)
     assert torch.allclose(dP, vals.grad, rtol=1e-5, atol=1e-6 * vals.grad.abs().max().item())
+
+
+@pytest.mark.parametrize("F", [64, 128, 300])
+def test_every_adjoint_shape_agrees(gpu, F):
+    """Every (vec, group) of the fused adjoint kernel: dX bit-identical, dP
+    equal up to the order of its per-feature sums, and both within 1e-3 of
+    the FP64 gradient; autotuning picks one of them."""
+    n, t = 17_000, 3   # above the size that gets a locality schedule
+    g, X, T = _problem(n, 9, F, seed=F)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    _, _, dX64, dP64 = orc.grad_f64(g.indptr, g.indices, g.data, X, T, t)
+    ref = None
+    shapes = [(v, gr, sc) for v in (4, 2, 1) if F % v == 0 for gr in (4, 8, 16, 32, 64)
+              for sc in (True, False)]
+    for v, gr, sc in shapes:
+        torch_ops.set_shape(P, F, v, gr, schedule=sc, adjoint=True)
+        got = torch_ops.get_shape(P, F, adjoint=True)
+        assert (got["vec"], got["group"], got["schedule"]) == (v, gr, sc)
+        Xr = Xd.clone().requires_grad_(True)
+        vals = torch.from_numpy(g.data.copy()).cuda().requires_grad_(True)
+        Y = autograd.diffuse(P, Xr, t, values=vals, checkpoint_every=2)
+        (0.5 * ((Y - Td) ** 2).sum()).backward()
+        if ref is None:
+            ref = (Xr.grad.clone(), vals.grad.clone())
+        assert torch.equal(Xr.grad, ref[0]), (v, gr)
+        assert torch.allclose(vals.grad, ref[1], rtol=1e-5,
+                              atol=1e-6 * ref[1].abs().max().item()), (v, gr)
+        assert orc.rel_err(vals.grad.cpu().numpy(), dP64) <= 1e-3
+        assert orc.rel_err(Xr.grad.cpu().numpy(), dX64) <= 1e-3
+    torch_ops.autotune(P, Xd, adjoint=True)
+    best = torch_ops.get_shape(P, F, adjoint=True)
+    assert (best["vec"], best["group"], best["schedule"]) in shapes
