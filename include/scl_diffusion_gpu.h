@@ -89,7 +89,10 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           range of slabs);
  *           SCL_GPU_SHAPE_SCALAR_ROWS: with group 16 or 32, the 64/group
  *           rows of a wave keep their indices and values on the scalar unit
- *           (each lane selects its row's) instead of loading them per lane
+ *           (each lane selects its row's) instead of loading them per lane;
+ *           SCL_GPU_SHAPE_OCCUPANCY(c), c in 1..15: at most c workgroups of
+ *           256 lanes resident per CU (LDS is reserved to enforce it); fewer
+ *           rows in flight per XCD keep more of their shared neighbours in L2
  * Any SpMM shape gives bit-identical results; only the speed differs.  The
  * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
  * are added in a shape-dependent order (fp32 rounding differences only). */
@@ -97,6 +100,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_NO_SCHEDULE 2
 #define SCL_GPU_SHAPE_XCD_ROWS 4
 #define SCL_GPU_SHAPE_SCALAR_ROWS 8
+#define SCL_GPU_SHAPE_OCCUPANCY(c) (((c) & 0xF) << 8)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

@@ -348,11 +348,13 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.schedule = (flags & kShapeNoSchedule) == 0;
     sh.xcd_rows = (flags & kShapeXcdRows) != 0;
     sh.scalar_rows = (flags & kShapeScalarRows) != 0;
+    sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
     if (transpose == 2) {
         sh.unroll = 8;
         sh.nontemporal = false;
         sh.xcd_rows = false;
         sh.scalar_rows = false;
+        sh.wg_per_cu = 0;
         SCL_API_CHECK(adjoint_shape_valid(sh, n_features, nullptr, nullptr, nullptr),
                       SCL_ERROR_INVALID_ARGUMENT,
                       "invalid adjoint shape: vec in {1,2,4} dividing n_features, "
@@ -411,7 +413,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
         *flags = (sh.nontemporal ? kShapeNontemporal : 0) |
                  (sh.schedule ? 0 : kShapeNoSchedule) |
                  (sh.xcd_rows ? kShapeXcdRows : 0) |
-                 (sh.scalar_rows ? kShapeScalarRows : 0);
+                 (sh.scalar_rows ? kShapeScalarRows : 0) |
+                 (sh.wg_per_cu << kShapeOccupancyShift);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

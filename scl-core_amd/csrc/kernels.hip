@@ -291,7 +291,15 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     if (!(s.vec == 1 || s.vec == 2 || s.vec == 4)) return false;
     if (F % s.vec != 0 || a % (4 * (uintptr_t)s.vec) != 0) return false;
     if (s.group < 1 || s.group > 64 || (s.group & (s.group - 1)) != 0) return false;
+    if (s.wg_per_cu < 0 || s.wg_per_cu > kShapeOccupancyMask) return false;
     return s.unroll == 4 || s.unroll == 8 || s.unroll == 16;
+}
+
+int occupancy_lds_bytes(int wg_per_cu) {
+    if (wg_per_cu <= 0) return 0;
+    // 160 KiB of LDS per gfx950 CU, allocated in 512-byte granules
+    constexpr int kLdsPerCu = 160 * 1024;
+    return (kLdsPerCu / wg_per_cu) / 512 * 512;
 }
 
 SpmmShape spmm_default_shape(int64_t F, const void* X, const void* Y) {
@@ -325,17 +333,22 @@ std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const voi
                         // scalar multi-row only with two rows per wave: four
                         // rows per wave measured 5-10x slower everywhere
                         for (int sr = 0; sr < ((vec == 4 && g == 32) ? 2 : 1); ++sr)
-                            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
-                                SpmmShape s;
-                                s.vec = vec;
-                                s.group = g;
-                                s.unroll = u;
-                                s.nontemporal = nt != 0;
-                                s.xcd_rows = xr != 0;
-                                s.scalar_rows = sr != 0;
-                                s.schedule = sc != 0;
-                                out.push_back(s);
-                            }
+                            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc)
+                                // fewer resident waves shrink the set of rows
+                                // whose gathers share L2 at once (wide rows)
+                                for (int occ : {0, 4, 5}) {
+                                    if (occ && !(vec == 4 && g == 64 && u <= 8)) continue;
+                                    SpmmShape s;
+                                    s.vec = vec;
+                                    s.group = g;
+                                    s.unroll = u;
+                                    s.nontemporal = nt != 0;
+                                    s.xcd_rows = xr != 0;
+                                    s.scalar_rows = sr != 0;
+                                    s.schedule = sc != 0;
+                                    s.wg_per_cu = occ;
+                                    out.push_back(s);
+                                }
         }
     }
     if (out.empty()) out.push_back(spmm_default_shape(F, X, Y));
@@ -353,7 +366,8 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : n_rowblocks * n_slabs;
     if (nb > INT32_MAX) throw std::runtime_error("spmm grid too large");
     const int32_t* sched = shape.schedule ? P.sched : nullptr;
-    hipLaunchKernelGGL(pick_spmm(shape), dim3((unsigned)nb), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(pick_spmm(shape), dim3((unsigned)nb), dim3(kBlock),
+                       (unsigned)occupancy_lds_bytes(shape.wg_per_cu), s,
                        P.row_ptr, P.col, P.val, sched, X, Y, P.n, (int32_t)F,
                        (int32_t)n_rowblocks, (int32_t)rb_per_xcd);
     check_launch("spmm launch");

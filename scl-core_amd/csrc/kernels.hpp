@@ -28,6 +28,8 @@ struct SpmmShape {
     bool schedule = true;      // visit rows in the operator's locality order
     bool xcd_rows = false;     // each XCD owns a row range (else a slab range)
     bool scalar_rows = false;  // group 16/32: rows' indices on the scalar unit
+    int wg_per_cu = 0;  // cap on resident workgroups per CU (0: none), set by
+                        // reserving LDS the kernel does not use
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
@@ -35,6 +37,11 @@ constexpr int kShapeNontemporal = 1;
 constexpr int kShapeNoSchedule = 2;
 constexpr int kShapeXcdRows = 4;
 constexpr int kShapeScalarRows = 8;
+// bits 8..11: workgroups-per-CU cap (0 = none, else 1..15)
+constexpr int kShapeOccupancyShift = 8;
+constexpr int kShapeOccupancyMask = 0xF;
+// LDS reserved per workgroup so that at most cap workgroups fit on a CU.
+int occupancy_lds_bytes(int wg_per_cu);
 
 // True when the shape can run on F features with these X / Y pointers.
 bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* Y);

@@ -244,6 +244,8 @@ def test_every_launch_shape_is_bit_identical(gpu, F):
                 for xr in (False, True):
                     for u in (4, 16):
                         torch_ops.set_shape(A, F, vec, group, u, nontemporal=u == 16,
-                                            schedule=not xr, xcd_rows=xr, scalar_rows=sr)
+                                            schedule=not xr, xcd_rows=xr, scalar_rows=sr,
+                                            wg_per_cu=4 if u == 4 else 0)
+                        assert torch_ops.get_shape(A, F)["wg_per_cu"] == (4 if u == 4 else 0)
                         Y = torch_ops.spmm(A, Xd).cpu().numpy()
                         assert np.array_equal(Y, ref), (vec, group, sr, xr, u)
