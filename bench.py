@@ -21,8 +21,9 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
       0.5||Y-T||^2 + fused adjoint/edge-gradient reverse pass (one hipGraph),
       vs the same work as plain kernels (6 SpMM + 6 transposed SpMM + 6 edge-
       gradient passes with every iterate kept)
-  c4  uniform15_1m, F=256, t=4: bf16 iterates (fp32 or bf16 P values) vs
-      the fp32 kernel
+  c4  uniform15_1m, F=256, t=4: bf16 iterates (fp32 or bf16 P values),
+      fp16 iterates (2^k scaled; X in fp32 or stored in fp16 too) vs the
+      fp32 kernel
   c5  256 graphs, n_i log-uniform in [2000, 50000], k_i in 10..30, F=64,
       t=5, fwd+bwd: one hipGraph over the block-diagonal stack vs the 256
       graphs one after another with plain (non-graph) launches
@@ -186,15 +187,23 @@ def leg_c4(steps: int) -> dict:
            "fp32_GBps": round(t * step_bytes(n, g.nnz, F) / f32_ms / 1e6, 1)}
     scale = float(Y32.abs().max())
     k16 = torch_ops.lowp_scale_exp(P, X, t)
-    for fmt, vb in (("bf16", False), ("bf16", True), ("fp16", False)):
+    for fmt, vb, ri in (("bf16", False, False), ("bf16", True, False), ("fp16", False, False),
+                        ("fp16", False, True)):
         kk = k16 if fmt == "fp16" else 0
-        Yb = torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk)
-        ms = timed(lambda: torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk),
+        Yb = torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk, round_input=ri)
+        ms = timed(lambda: torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk,
+                                                  round_input=ri),
                    steps, stream)
         vbytes = 2 if vb else 4
-        bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, 2) + (t - 2) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
-                  + step_bytes(n, g.nnz, F, vbytes, 2, 4))
-        key = f"{fmt}_iterates_{'bf16' if vb else 'fp32'}_values"
+        if ri:  # X -> 16 bits once, then t steps gathering 16-bit rows
+            bytes_ = (n * F * 6 + (t - 1) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
+                      + step_bytes(n, g.nnz, F, vbytes, 2, 4))
+        else:
+            bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, 2)
+                      + (t - 2) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
+                      + step_bytes(n, g.nnz, F, vbytes, 2, 4))
+        key = (f"{fmt}_iterates_{'bf16' if vb else 'fp32'}_values"
+               + ("_16bit_input" if ri else ""))
         out[key] = {"ms": round(ms, 3), "GBps": round(bytes_ / ms / 1e6, 1),
                     "speedup_vs_fp32": round(f32_ms / ms, 3),
                     "max_rel_diff_vs_fp32_kernel": float((Yb - Y32).abs().max()) / scale}

@@ -135,12 +135,18 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
  *   SCL_GPU_LOWP_FP16         iterates as fp16 scaled by 2^scale_exp (else bf16,
  *                             scale_exp must be 0)
  *   SCL_GPU_LOWP_BF16_VALUES  read P's values rounded to bf16 (else fp32)
- *   SCL_GPU_LOWP_NO_SCHEDULE  natural row order
+ *   SCL_GPU_LOWP_NO_SCHEDULE  natural row order (the default follows the
+ *                             fp32 SpMM shape tuned or set for n_features)
+ *   SCL_GPU_LOWP_ROUND_INPUT  store X in 16 bits too (one rounding of
+ *                             X * 2^scale_exp up front), so that the first
+ *                             step also gathers 16-bit rows
  * n_features must be even; work_dev holds 2 planes of n x F 16-bit values
- * (n * F * 4 bytes) and is required when n_steps >= 2. */
+ * (n * F * 4 bytes) and is required when n_steps >= 2 or with
+ * SCL_GPU_LOWP_ROUND_INPUT. */
 #define SCL_GPU_LOWP_BF16_VALUES 1
 #define SCL_GPU_LOWP_NO_SCHEDULE 2
 #define SCL_GPU_LOWP_FP16 4
+#define SCL_GPU_LOWP_ROUND_INPUT 8
 SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     void* work_dev, scl_index_t n_nodes, scl_index_t n_features,

@@ -207,8 +207,8 @@ float orc_fp16_round(float f) { return orc_round_fp16(f); }
 
 int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
-                         int bf16_values, int fp16, int scale_exp, const float* X,
-                         float* Y) {
+                         int bf16_values, int fp16, int scale_exp, int round_input,
+                         const float* X, float* Y) {
     const size_t total = (size_t)n * (size_t)F;
     const int64_t nnz = indptr[n];
     float* v = (float*)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(float));
@@ -218,6 +218,13 @@ int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
     const float up = ldexpf(1.0f, scale_exp), down = ldexpf(1.0f, -scale_exp);
     for (int64_t k = 0; k < nnz; ++k) v[k] = bf16_values ? orc_round_bf16(vals[k]) : vals[k];
     memcpy(a, X, total * sizeof(float));
+    if (round_input) {
+        /* X stored as round16(X * 2^k) and read back as that * 2^-k */
+        if (fp16)
+            for (size_t e = 0; e < total; ++e) a[e] = orc_round_fp16(a[e] * up) * down;
+        else
+            for (size_t e = 0; e < total; ++e) a[e] = orc_round_bf16(a[e]);
+    }
     for (int64_t s = 1; s <= t; ++s) {
         orc_spmm_strided_f32(n, F, indptr, indices, v, a, b);
         if (s < t) {
@@ -237,7 +244,7 @@ int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
 int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
                          int bf16_values, const float* X, float* Y) {
-    return orc_diffuse_lowp_f32(n, F, t, indptr, indices, vals, bf16_values, 0, 0, X, Y);
+    return orc_diffuse_lowp_f32(n, F, t, indptr, indices, vals, bf16_values, 0, 0, 0, X, Y);
 }
 
 /* ---------------------------------------------------------------- FP64 -- */

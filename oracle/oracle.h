@@ -90,11 +90,12 @@ float orc_bf16_round(float f);
 /* The same with fp16 iterates when fp16 != 0: each intermediate iterate is
  * stored as round_fp16(y * 2^scale_exp) and read back times 2^-scale_exp
  * (round to nearest even, overflow to inf, subnormals kept); bf16 iterates
- * otherwise (scale_exp ignored). */
+ * otherwise (scale_exp ignored).  round_input != 0 stores X the same way
+ * before the first step. */
 int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
-                         int bf16_values, int fp16, int scale_exp, const float* X,
-                         float* Y);
+                         int bf16_values, int fp16, int scale_exp, int round_input,
+                         const float* X, float* Y);
 /* fp32 -> nearest-even binary16, returned as fp32. */
 float orc_fp16_round(float f);
 

@@ -66,7 +66,8 @@ def load():
         "orc_bf16_round": (ctypes.c_float, [ctypes.c_float]),
         "orc_fp16_round": (ctypes.c_float, [ctypes.c_float]),
         "orc_diffuse_lowp_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int,
-                                                 ctypes.c_int, ctypes.c_int, P, P]),
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 P, P]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -127,8 +128,9 @@ def diffuse_bf16_f32(indptr, indices, vals, X, t, bf16_values=False):
 
 
 def diffuse_lowp_f32(indptr, indices, vals, X, t, fmt="bf16", bf16_values=False,
-                     scale_exp=0):
-    """16-bit-iterate restatement ("bf16" or scaled "fp16")."""
+                     scale_exp=0, round_input=False):
+    """16-bit-iterate restatement ("bf16" or scaled "fp16"); round_input
+    stores X in the same format before the first step."""
     X = _f32(X)
     n, F = X.shape
     Y = np.empty_like(X)
@@ -136,6 +138,7 @@ def diffuse_lowp_f32(indptr, indices, vals, X, t, fmt="bf16", bf16_values=False,
     rc = load().orc_diffuse_lowp_f32(n, F, t, ip.ctypes.data, ix.ctypes.data,
                                      v.ctypes.data, 1 if bf16_values else 0,
                                      1 if fmt == "fp16" else 0, int(scale_exp),
+                                     1 if round_input else 0,
                                      X.ctypes.data, Y.ctypes.data)
     if rc:
         raise MemoryError("oracle allocation failed")

@@ -106,6 +106,10 @@ void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
                       float in_scale, float out_scale, hipStream_t s);
 // partial[b] = max |x| over a grid-stride slice, for b < n_partial blocks.
 void launch_absmax(const float* x, int64_t m, float* partial, int n_partial, hipStream_t s);
+// dst = round16(src * scale) (bf16 or fp16, nearest even), m floats.
+void launch_narrow_f32(const float* src, uint16_t* dst, int64_t m, bool fp16, float scale,
+                       hipStream_t s);
+
 void launch_to_bf16(const float* src, uint16_t* dst, int64_t m, hipStream_t s);
 void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s);
 

@@ -81,8 +81,10 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
                   SCL_ERROR_INVALID_ARGUMENT, "Dimensions must be positive");
     SCL_API_CHECK(n_features % 2 == 0, SCL_ERROR_INVALID_ARGUMENT,
                   "16-bit storage needs an even n_features");
-    SCL_API_CHECK(n_steps == 1 || work_dev != nullptr, SCL_ERROR_NULL_POINTER,
-                  "work buffer is required for n_steps >= 2");
+    const bool round_input = (flags & SCL_GPU_LOWP_ROUND_INPUT) != 0;
+    SCL_API_CHECK((n_steps == 1 && !round_input) || work_dev != nullptr,
+                  SCL_ERROR_NULL_POINTER,
+                  "work buffer is required for n_steps >= 2 or a rounded input");
     SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev, SCL_ERROR_INVALID_ARGUMENT,
                   "X and Y must be distinct");
     const bool fp16 = (flags & SCL_GPU_LOWP_FP16) != 0;
@@ -95,9 +97,13 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
         hipStream_t s = static_cast<hipStream_t>(stream);
         CsrView v;
         const uint16_t* vb = nullptr;
+        bool sched = (flags & SCL_GPU_LOWP_NO_SCHEDULE) == 0;
         {
             std::lock_guard<std::mutex> lk(transition->mu);
             DeviceCSR& d = device_csr(transition, dev);
+            // the visiting order the fp32 SpMM was tuned to for this width
+            const auto ti = d.shapes.find(n_features);
+            if (ti != d.shapes.end() && !ti->second.schedule) sched = false;
             if (flags & SCL_GPU_LOWP_BF16_VALUES) {
                 if (!d.val_bf16) {
                     void* p = nullptr;
@@ -114,13 +120,24 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
             }
             v = CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
         }
-        const bool sched = (flags & SCL_GPU_LOWP_NO_SCHEDULE) == 0;
         const float up = std::ldexp(1.0f, scale_exp), down = std::ldexp(1.0f, -scale_exp);
         const size_t plane = (size_t)n_nodes * (size_t)n_features;
         uint16_t* w[2] = {static_cast<uint16_t*>(work_dev),
                           work_dev ? static_cast<uint16_t*>(work_dev) + plane : nullptr};
         const int64_t F = n_features;
-        if (n_steps == 1) {
+        if (round_input) {
+            // X itself is stored in 16 bits once; every step then gathers
+            // 16-bit rows
+            launch_narrow_f32(X_dev, w[0], (int64_t)plane, fp16, up, s);
+            int cur = 0;
+            for (int64_t st = 1; st < n_steps; ++st) {
+                launch_spmm_lowp(v, vb, LowpIo::BF16_TO_BF16, w[cur], w[cur ^ 1], F, sched,
+                                 fp16, down, up, s);
+                cur ^= 1;
+            }
+            launch_spmm_lowp(v, vb, LowpIo::BF16_TO_F32, w[cur], Y_dev, F, sched, fp16, down,
+                             1.0f, s);
+        } else if (n_steps == 1) {
             launch_spmm_lowp(v, vb, LowpIo::F32_TO_F32, X_dev, Y_dev, F, sched, fp16, 1.0f,
                              1.0f, s);
         } else {

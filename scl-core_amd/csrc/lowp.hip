@@ -202,6 +202,26 @@ __global__ __launch_bounds__(kBlock) void k_to_bf16(const float* __restrict__ sr
     if (p < m) dst[p] = f32_to_bf16(src[p]);
 }
 
+// dst = narrow(src * scale), 4 elements per lane (m % 4 == 0, 16-byte
+// aligned src, 8-byte aligned dst) or one per lane.
+template <typename ST, int E>
+__global__ __launch_bounds__(kBlock) void k_narrow(const float* __restrict__ src,
+                                                   uint16_t* __restrict__ dst, int64_t m,
+                                                   float scale) {
+    const int64_t p = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * E;
+    if (p >= m) return;
+    if constexpr (E == 4) {
+        const f4 x = *reinterpret_cast<const f4*>(src + p);
+        const uint32_t lo = (uint32_t)ST::narrow(x.x * scale) |
+                            ((uint32_t)ST::narrow(x.y * scale) << 16);
+        const uint32_t hi = (uint32_t)ST::narrow(x.z * scale) |
+                            ((uint32_t)ST::narrow(x.w * scale) << 16);
+        *reinterpret_cast<uint2*>(dst + p) = make_uint2(lo, hi);
+    } else {
+        dst[p] = ST::narrow(src[p] * scale);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_from_bf16(const uint16_t* __restrict__ src,
                                                       float* __restrict__ dst, int64_t m) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -265,6 +285,24 @@ void launch_to_bf16(const float* src, uint16_t* dst, int64_t m, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((m + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, src, dst, m);
     check_launch("bf16 convert launch");
+}
+
+void launch_narrow_f32(const float* src, uint16_t* dst, int64_t m, bool fp16, float scale,
+                       hipStream_t s) {
+    if (m <= 0) return;
+    const bool wide = m % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(dst) % 8 == 0;
+    const int E = wide ? 4 : 1;
+    const int64_t lanes = (m + E - 1) / E;
+    const dim3 grid((unsigned)((lanes + kBlock - 1) / kBlock));
+    if (fp16) {
+        if (wide) hipLaunchKernelGGL((k_narrow<SFp16, 4>), grid, dim3(kBlock), 0, s, src, dst, m, scale);
+        else hipLaunchKernelGGL((k_narrow<SFp16, 1>), grid, dim3(kBlock), 0, s, src, dst, m, scale);
+    } else {
+        if (wide) hipLaunchKernelGGL((k_narrow<SBf16, 4>), grid, dim3(kBlock), 0, s, src, dst, m, scale);
+        else hipLaunchKernelGGL((k_narrow<SBf16, 1>), grid, dim3(kBlock), 0, s, src, dst, m, scale);
+    }
+    check_launch("16-bit convert launch");
 }
 
 void launch_from_bf16(const uint16_t* src, float* dst, int64_t m, hipStream_t s) {

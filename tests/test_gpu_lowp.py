@@ -45,6 +45,24 @@ def test_fp16_path_matches_restatement(gpu, F, t, vb):
     assert np.array_equal(Y.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("fmt,F,t,vb", [("bf16", 2, 1, False), ("bf16", 64, 3, True),
+                                        ("fp16", 256, 4, False), ("fp16", 300, 1, True),
+                                        ("fp16", 1000, 2, False)])
+def test_rounded_input_matches_restatement(gpu, fmt, F, t, vb):
+    """X stored in 16 bits before the first step: bit-identical to the
+    restatement, also for t = 1 and for F not a multiple of 4."""
+    n = 20_000 if F <= 256 else 5_000
+    g = synth.uniform_knn(n, 15, seed=F + t + 2).normalized()
+    X = synth.normal(n, F, seed=F + 2) * np.float32(11.0)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd = torch.from_numpy(X).cuda()
+    k = torch_ops.lowp_scale_exp(P, Xd, t) if fmt == "fp16" else 0
+    Y = torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb, scale_exp=k, round_input=True)
+    ref = orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, t, fmt, bf16_values=vb,
+                               scale_exp=k, round_input=True)
+    assert np.array_equal(Y.cpu().numpy(), ref)
+
+
 def test_bf16_error_vs_fp64_uniform15(gpu):
     """n = 200,000 (the 1M-cell shape scaled down 5x), k = 15 uniform
     neighbours, F = 256 N(0,1), t = 4."""
@@ -57,10 +75,13 @@ def test_bf16_error_vs_fp64_uniform15(gpu):
     y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X[:, cols], t)
     y64q = orc.diffuse_f64(g.indptr, g.indices, g.data, orc.bf16_round(X[:, cols]), t)
     rows = []
-    for fmt, vb in (("bf16", False), ("bf16", True), ("fp16", False), ("fp16", True)):
-        Y = torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb)[:, torch.from_numpy(cols).cuda()]
+    for fmt, vb, ri in (("bf16", False, False), ("bf16", True, False), ("fp16", False, False),
+                        ("fp16", True, False), ("fp16", False, True)):
+        Y = torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb,
+                                   round_input=ri)[:, torch.from_numpy(cols).cuda()]
         Yh = Y.cpu().numpy()
-        rows.append((fmt, vb, orc.rel_err(Yh, y64), orc.median_rel_err(Yh, y64)))
+        rows.append((fmt + (" rounded input" if ri else ""), vb, orc.rel_err(Yh, y64),
+                     orc.median_rel_err(Yh, y64)))
     Y32 = torch_ops.diffuse(P, Xd, t)[:, torch.from_numpy(cols).cuda()].cpu().numpy()
     e32 = orc.rel_err(Y32, y64)
     Xq = torch.from_numpy(orc.bf16_round(X)).cuda()
@@ -72,5 +93,6 @@ def test_bf16_error_vs_fp64_uniform15(gpu):
     print(f"bf16 iterates on bf16-rounded input vs FP64 of that input: {eq:.2e}")
     assert e32 <= 1e-5
     assert all(emax < 1e-2 for _, _, emax, _ in rows)
-    # fp16 iterates with fp32 values meet the 1e-3 bar
+    # fp16 iterates with fp32 values meet the 1e-3 bar, also with X in fp16
     assert rows[2][2] <= 1e-3
+    assert rows[4][2] <= 1e-3

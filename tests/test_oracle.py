@@ -306,3 +306,24 @@ def test_fp16_restatement_scaling_is_exact_and_reduces():
                                            scale_exp=14), y64)
     eb = orc.rel_err(orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 4, "bf16"), y64)
     assert e16 < eb / 4 and e16 < 1e-3
+
+
+def test_rounded_input_restatement_by_hand():
+    """round_input = one more rounding, of X itself, before step 1: equal to
+    the plain 16-bit restatement run on the rounded X, and to t = 1 done by
+    hand."""
+    g = synth.uniform_knn(500, 7, seed=8).normalized()
+    X = synth.normal(500, 6, seed=8) * np.float32(5.0)
+    for fmt, k in (("bf16", 0), ("fp16", 9)):
+        s = np.float32(2.0 ** k)
+        Xq = (orc.fp16_round(X * s) / s).astype(np.float32) if fmt == "fp16" else orc.bf16_round(X)
+        assert not np.array_equal(Xq, X)
+        for t in (1, 3):
+            got = orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, t, fmt, scale_exp=k,
+                                       round_input=True)
+            assert np.array_equal(got, orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, Xq,
+                                                            t, fmt, scale_exp=k))
+        assert np.array_equal(
+            orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 1, fmt, scale_exp=k,
+                                 round_input=True),
+            orc.spmm_step_f32(g.indptr, g.indices, g.data, Xq))
