@@ -25,7 +25,8 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
       fp16 iterates (2^k scaled; X in fp32 or stored in fp16 too) vs the
       fp32 kernel
   c5  256 graphs, n_i log-uniform in [2000, 50000], k_i in 10..30, F=64,
-      t=5, fwd+bwd: one hipGraph over the block-diagonal stack vs the 256
+      t=5, fwd+bwd storing all t iterates: one hipGraph over the
+      block-diagonal stack vs the 256
       graphs one after another with plain (non-graph) launches
 """
 from __future__ import annotations
@@ -216,7 +217,7 @@ def leg_c5(steps: int) -> dict:
     import numpy as np
     import torch
     from scl_amd import CsrMatrix, batch, synth, torch_ops, _lib
-    F, t, every = 64, 5, 2
+    F, t, every = 64, 5, 1   # every iterate stored, no recompute
     sizes, ks = synth.batch_sizes(256, seed=5)
     graphs = [synth.uniform_knn(int(n), int(k), seed=5000 + i).normalized()
               for i, (n, k) in enumerate(zip(sizes, ks))]

@@ -39,10 +39,11 @@ def test_block_diag_forward_matches_each_graph(gpu):
                               orc.diffuse_matrix_f32(g.indptr, g.indices, g.data, Xi, t)), i
 
 
-def test_plan_replays_match_eager(gpu):
+@pytest.mark.parametrize("every", [1, 2])
+def test_plan_replays_match_eager(gpu, every):
     graphs, mats = make_batch(16, seed=7)
     B = batch.block_diag(mats)
-    N, F, t, every = B.shape[0], 64, 5, 2
+    N, F, t = B.shape[0], 64, 5
     X = torch.from_numpy(synth.normal(N, F, seed=1)).cuda()
     T = torch.from_numpy(synth.normal(N, F, seed=2)).cuda()
     # eager reference through autograd
