@@ -66,9 +66,9 @@ SCL_EXPORT scl_error_t scl_gpu_compute_transition_matrix_ex(
     double* residual_out);
 
 /* Host-only (needs no GPU): the locality row order the device uses for
- * operators of >= 16384 rows -- a Cuthill-McKee traversal of the pattern,
- * sorted by decreasing row length inside windows of 64 rows.  order_out
- * receives a permutation of 0..primary-1. */
+ * operators of >= 16384 rows -- greedy: each next row is the unplaced row
+ * with the most pattern edges (either direction) to the last 256 placed
+ * rows.  order_out receives a permutation of 0..primary-1. */
 SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
                                                  scl_index_t* order_out);
 
