@@ -28,6 +28,8 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
       t=5, fwd+bwd storing all t iterates: one hipGraph over the
       block-diagonal stack vs the 256
       graphs one after another with plain (non-graph) launches
+  mp  a skewed pattern (Pareto row lengths, hub rows of 200k nonzeros),
+      n=1M, F=64: merge-path SpMM vs the autotuned row-split SpMM
 """
 from __future__ import annotations
 
@@ -212,6 +214,48 @@ def leg_c4(steps: int) -> dict:
     return out
 
 
+def leg_mp(steps: int) -> dict:
+    """Skewed pattern (Pareto row lengths, 4 hub rows of 200k nonzeros),
+    n=1M, F=64: merge-path SpMM vs the autotuned row-split SpMM."""
+    import numpy as np
+    import torch
+    from scl_amd import CsrMatrix, torch_ops
+    n, F = 1_000_000, 64
+    rng = np.random.Generator(np.random.PCG64(6))
+    deg = np.minimum((rng.pareto(1.1, n) * 4).astype(np.int64) + 1, 20_000)
+    deg[rng.integers(0, n, 4)] = 200_000
+    rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+    key = np.unique(rows * n + rng.integers(0, n, len(rows)))
+    rows, cols = key // n, key % n
+    ip = np.zeros(n + 1, np.int64)
+    np.add.at(ip, rows + 1, 1)
+    ip = np.cumsum(ip)
+    vals = rng.random(len(cols)).astype(np.float32)
+    P = CsrMatrix.from_arrays(ip, cols, vals, (n, n))
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    X = torch.from_numpy(rng.standard_normal((n, F), dtype=np.float32)).to(dev)
+    row_ms = torch_ops.autotune(P, X)
+    row_shape = torch_ops.get_shape(P, F)
+    Yr = torch_ops.spmm(P, X)
+    row_ms = timed(lambda: torch_ops.spmm(P, X), steps, stream)
+    out = {"workload": "pareto_1m_f64", "nnz": int(ip[-1]), "max_row": int(deg.max()),
+           "rowsplit_ms": round(row_ms, 3), "rowsplit_shape": row_shape}
+    best = None
+    for m in (1, 3, 5):
+        torch_ops.set_shape(P, F, 4, 64, 8, merge=m)
+        Ym = torch_ops.spmm(P, X)
+        ms = timed(lambda: torch_ops.spmm(P, X), steps, stream)
+        err = float((Ym - Yr).abs().max()) / float(Yr.abs().max())
+        out[f"merge{m}_ms"] = round(ms, 3)
+        out[f"merge{m}_max_rel_diff"] = err
+        if best is None or ms < best[0]:
+            best = (ms, m)
+    out["speedup_merge_vs_rowsplit"] = round(row_ms / best[0], 2)
+    out["GBps_best"] = round(step_bytes(n, int(ip[-1]), F) / min(best[0], row_ms) / 1e6, 1)
+    return out
+
+
 def leg_c5(steps: int) -> dict:
     """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
     import numpy as np
@@ -294,7 +338,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="c2,c3,c4,c5")
+    ap.add_argument("--legs", default="c2,c3,c4,c5,mp")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="vec,group,unroll,flags for the headline SpMM instead of the "
@@ -387,7 +431,7 @@ def main() -> int:
     legs = {}
     del Xd, Yd, Wd, A
     torch.cuda.empty_cache()
-    for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5)):
+    for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5), ("mp", leg_mp)):
         if name not in args.legs.split(","):
             continue
         log(f"leg {name}")

@@ -92,8 +92,19 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           (each lane selects its row's) instead of loading them per lane;
  *           SCL_GPU_SHAPE_OCCUPANCY(c), c in 1..15: at most c workgroups of
  *           256 lanes resident per CU (LDS is reserved to enforce it); fewer
- *           rows in flight per XCD keep more of their shared neighbours in L2
- * Any SpMM shape gives bit-identical results; only the speed differs.  The
+ *           rows in flight per XCD keep more of their shared neighbours in L2;
+ *           SCL_GPU_SHAPE_MERGE_PATH(m), m in 1..8 (group 64 only): merge-path
+ *           SpMM -- the path of row ends and nonzeros is cut into pieces of
+ *           256 << (m - 1) items, one per wave, so work is even whatever the
+ *           row lengths (rows in natural order; the schedule flag is
+ *           ignored).  A row cut into p pieces is summed as p stored-order
+ *           chains added in order: p - 1 extra fp32 roundings, so such rows
+ *           are not bit-identical to the row-split shapes.  Setting it
+ *           allocates the per-wave scratch (calls on one operator with a
+ *           merge-path shape must not overlap on two streams).  The
+ *           autotuner never picks it
+ * Any row-split SpMM shape gives bit-identical results; only the speed
+ * differs.  The
  * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
  * are added in a shape-dependent order (fp32 rounding differences only). */
 #define SCL_GPU_SHAPE_NONTEMPORAL 1
@@ -101,6 +112,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_XCD_ROWS 4
 #define SCL_GPU_SHAPE_SCALAR_ROWS 8
 #define SCL_GPU_SHAPE_OCCUPANCY(c) (((c) & 0xF) << 8)
+#define SCL_GPU_SHAPE_MERGE_PATH(m) (((m) & 0xF) << 12)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

@@ -83,6 +83,37 @@ void* device_workspace(int device, int slot, size_t bytes) {
     return w.first;
 }
 
+void ensure_merge_scratch(DeviceCSR& d, int64_t waves, int64_t F) {
+    const int64_t floats = waves * F;
+    if (waves <= d.merge.waves && floats <= d.merge.floats) return;
+    if (d.pins > 0)
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "a diffusion plan holds this operator: merge-path scratch cannot grow");
+    const int64_t w = std::max(waves, d.merge.waves);
+    const int64_t f = std::max(floats, d.merge.floats);
+    dev_free(d.merge.split);
+    dev_free(d.merge.carry_row);
+    dev_free(d.merge.carry_val);
+    d.merge = MergeScratch{};
+    void* a = nullptr;
+    void* b = nullptr;
+    void* c = nullptr;
+    if (hipMalloc(&a, (size_t)(2 * (w + 1)) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&b, (size_t)w * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c, (size_t)std::max<int64_t>(f, 1) * sizeof(float)) != hipSuccess) {
+        dev_free(a);
+        dev_free(b);
+        dev_free(c);
+        (void)hipGetLastError();
+        fail(SCL_ERROR_OUT_OF_MEMORY, "merge-path scratch allocation failed");
+    }
+    d.merge.split = static_cast<int32_t*>(a);
+    d.merge.carry_row = static_cast<int32_t*>(b);
+    d.merge.carry_val = static_cast<float*>(c);
+    d.merge.waves = w;
+    d.merge.floats = f;
+}
+
 DeviceCSR::~DeviceCSR() {
     int cur = -1;
     if (device >= 0 && hipGetDevice(&cur) == hipSuccess && cur != device)
@@ -92,6 +123,9 @@ DeviceCSR::~DeviceCSR() {
     dev_free(val);
     dev_free(sched);
     dev_free(val_bf16);
+    dev_free(merge.split);
+    dev_free(merge.carry_row);
+    dev_free(merge.carry_val);
     release_transpose();
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
 }

@@ -20,11 +20,12 @@ namespace {
 constexpr int64_t kInt32Max = std::numeric_limits<int32_t>::max();
 
 CsrView view_of(const DeviceCSR& d) {
-    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
+    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge};
 }
 
 CsrView transposed_view_of(const DeviceCSR& d) {
-    return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched};
+    return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched,
+                   &d.merge};
 }
 
 // Checks the operator against the dense operand: the reference never checks
@@ -349,6 +350,10 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.xcd_rows = (flags & kShapeXcdRows) != 0;
     sh.scalar_rows = (flags & kShapeScalarRows) != 0;
     sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
+    sh.merge = (flags >> kShapeMergeShift) & kShapeMergeMask;
+    SCL_API_CHECK(sh.merge <= kMergeMax && (sh.merge == 0 || transpose != 2),
+                  SCL_ERROR_INVALID_ARGUMENT,
+                  "merge-path code must be 0..8, and 0 for the fused adjoint");
     if (transpose == 2) {
         sh.unroll = 8;
         sh.nontemporal = false;
@@ -369,6 +374,9 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
         const int dev = current_device();
         std::lock_guard<std::mutex> lk(transition->mu);
         DeviceCSR& d = device_csr(transition, dev);
+        if (sh.merge > 0)
+            ensure_merge_scratch(d, merge_waves(transpose ? d.n_sec : d.n, d.nnz, sh.merge),
+                                 n_features);
         (transpose == 2 ? d.adj_shapes : transpose ? d.t_shapes : d.shapes)[n_features] = sh;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
@@ -414,7 +422,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
                  (sh.schedule ? 0 : kShapeNoSchedule) |
                  (sh.xcd_rows ? kShapeXcdRows : 0) |
                  (sh.scalar_rows ? kShapeScalarRows : 0) |
-                 (sh.wg_per_cu << kShapeOccupancyShift);
+                 (sh.wg_per_cu << kShapeOccupancyShift) |
+                 (sh.merge << kShapeMergeShift);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

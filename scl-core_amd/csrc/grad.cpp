@@ -36,10 +36,11 @@ void check_square(const scl_sparse_matrix* P, int64_t n, int64_t F) {
 int64_t n_checkpoints(int64_t t, int64_t every) { return (t + every - 1) / every - 1; }
 
 CsrView fwd_view(const DeviceCSR& d) {
-    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
+    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge};
 }
 CsrView adj_view(const DeviceCSR& d) {
-    return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched};
+    return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched,
+                   &d.merge};
 }
 
 SpmmShape fwd_shape(DeviceCSR& d, int64_t F, const void* a, const void* b) {

@@ -99,6 +99,9 @@ struct DeviceCSR {
     // caller) for the operator and for its transpose
     std::map<int64_t, SpmmShape> shapes, t_shapes;
     std::map<int64_t, SpmmShape> adj_shapes;  // fused adjoint + edge kernel
+    // scratch of merge-path launches (both directions share it; grown when a
+    // merge-path shape is set, never while plans hold pointers into it)
+    MergeScratch merge;
 
     ~DeviceCSR();
     void release_transpose();
@@ -140,6 +143,8 @@ hipStream_t library_stream(int device);
 std::mutex& device_mutex(int device);
 // Grow-only device buffer `slot` of a device, at least `bytes` long.
 void* device_workspace(int device, int slot, size_t bytes);
+// Grows d.merge to hold `waves` waves of F features.
+void ensure_merge_scratch(DeviceCSR& d, int64_t waves, int64_t F);
 
 // Operators with at least this many rows get a locality schedule.
 constexpr int64_t kScheduleMinRows = 16384;

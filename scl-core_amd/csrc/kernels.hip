@@ -139,6 +139,132 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
 }
 
 
+// ------------------------------------------------------------ merge path --
+// The merge path of a CSR matrix interleaves its n row ends with its nnz
+// nonzeros: nonzero k is consumed while k < row_ptr[i + 1] for the current
+// row i, then row i's end.  Cutting the path into pieces of D items gives
+// every wave the same amount of work whatever the row lengths.  A wave walks
+// its piece with all 64 lanes on one row at a time (lanes over features,
+// indices wave-uniform on the scalar unit), writes every row whose end lies
+// in its piece, and leaves the partial sum of the row it does not finish in
+// carry_val.  A second pass adds a row's carries, in path order, to the
+// value written where the row ends.  A row inside one piece is the same FMA
+// chain as in the row-split kernels; a row cut into p pieces is p chains
+// added in order (p - 1 extra fp32 roundings).
+
+// (row, nonzero) where path diagonal d = w * items starts: the smallest i in
+// [max(0, d - nnz), min(d, n)] with row_ptr[i + 1] > d - i - 1.
+__global__ __launch_bounds__(kBlock) void k_merge_split(const int32_t* __restrict__ row_ptr,
+                                                        int32_t n, int32_t nnz, int32_t items,
+                                                        int32_t waves,
+                                                        int32_t* __restrict__ split) {
+    const int w = blockIdx.x * kBlock + threadIdx.x;
+    if (w > waves) return;
+    const int64_t total = (int64_t)n + nnz;
+    const int d = (int)min((int64_t)w * items, total);
+    int lo = max(d - nnz, 0), hi = min(d, n);
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (row_ptr[mid + 1] <= d - mid - 1) lo = mid + 1;
+        else hi = mid;
+    }
+    split[2 * w] = lo;
+    split[2 * w + 1] = d - lo;
+}
+
+template <int VEC, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_spmm_merge(
+    const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
+    const float* __restrict__ val, const int32_t* __restrict__ split,
+    const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
+    int32_t waves, int32_t wgs, int32_t* __restrict__ carry_row,
+    float* __restrict__ carry_val) {
+    using VT = typename Vec<VEC>::T;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int slab = L / wgs;
+    const int wg = L - slab * wgs;
+    const int lane = threadIdx.x & 63;
+    const int w = wg * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= waves) return;
+    const int i0 = __builtin_amdgcn_readfirstlane(split[2 * w]);
+    const int k0 = __builtin_amdgcn_readfirstlane(split[2 * w + 1]);
+    const int i1 = __builtin_amdgcn_readfirstlane(split[2 * w + 2]);
+    const int k1 = __builtin_amdgcn_readfirstlane(split[2 * w + 3]);
+    const int f = slab * (64 * VEC) + lane * VEC;
+    const bool active = f < F;
+    const float* Xf = X + (active ? f : 0);
+
+    float acc[VEC];
+#pragma unroll
+    for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+    int r = i0;  // current row; its end offset e (none past the last row)
+    int e = r < n ? row_ptr[r + 1] : INT32_MAX;
+    auto finish_row = [&]() {
+        if (active) store_vec<VEC, NT>(Y + (size_t)r * F + f, acc);
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[c] = 0.0f;
+        ++r;
+        e = r < n ? row_ptr[r + 1] : INT32_MAX;
+    };
+    for (int k = k0; k < k1; k += U) {
+        int j[U];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u, k1 - 1);
+            j[u] = col[kk];
+            v[u] = val[kk];
+        }
+        VT x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)j[u] * F);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k + u < k1) {
+                while (k + u >= e) finish_row();
+                fma_vec(v[u], x[u], acc);
+            }
+        }
+    }
+    // rows ending inside the piece after its last nonzero (empty rows too)
+    while (r < i1) finish_row();
+    // row i1, when some of its nonzeros lie in [k0, k1), continues in the
+    // next piece
+    const bool carry = i1 < n && k1 > max(k0, row_ptr[min(i1, n - 1)]);
+    if (slab == 0 && lane == 0) carry_row[w] = carry ? i1 : -1;
+    if (carry && active) store_vec<VEC, false>(carry_val + (size_t)w * F + f, acc);
+}
+
+// Adds the carries of each cut row to the value written where the row ends:
+// Y[r] = ((c_w + c_{w+1}) + ...) + Y[r] over the consecutive waves w.. that
+// carry r.  One wave per carrying wave; only the first of a run works.
+template <int VEC>
+__global__ __launch_bounds__(kBlock) void k_merge_fixup(const int32_t* __restrict__ carry_row,
+                                                        const float* __restrict__ carry_val,
+                                                        float* __restrict__ Y, int32_t waves,
+                                                        int32_t F) {
+    const int w = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w >= waves) return;
+    const int r = carry_row[w];
+    if (r < 0 || (w > 0 && carry_row[w - 1] == r)) return;
+    int w_end = w + 1;
+    while (w_end < waves && carry_row[w_end] == r) ++w_end;
+    for (int f = lane * VEC; f < F; f += 64 * VEC) {
+        float s[VEC];
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) s[c] = carry_val[(size_t)w * F + f + c];
+        for (int q = w + 1; q < w_end; ++q) {
+#pragma unroll
+            for (int c = 0; c < VEC; ++c) s[c] += carry_val[(size_t)q * F + f + c];
+        }
+        float* y = Y + (size_t)r * F + f;
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) y[c] = s[c] + y[c];
+    }
+}
+
 // Several rows per wave (RPW = 2 or 4, G = 64 / RPW lanes each) with the
 // rows' column indices and values kept on the scalar unit: every lane picks
 // its row's (j, p) with a select instead of loading them itself, so a
@@ -292,6 +418,8 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     if (F % s.vec != 0 || a % (4 * (uintptr_t)s.vec) != 0) return false;
     if (s.group < 1 || s.group > 64 || (s.group & (s.group - 1)) != 0) return false;
     if (s.wg_per_cu < 0 || s.wg_per_cu > kShapeOccupancyMask) return false;
+    // merge path: lanes over features, one piece of the path per wave
+    if (s.merge < 0 || s.merge > kMergeMax || (s.merge > 0 && s.group != 64)) return false;
     return s.unroll == 4 || s.unroll == 8 || s.unroll == 16;
 }
 
@@ -355,9 +483,59 @@ std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const voi
     return out;
 }
 
+int64_t merge_waves(int64_t n, int64_t nnz, int m) {
+    const int64_t items = (int64_t)256 << (m - 1);
+    return std::max<int64_t>((n + nnz + items - 1) / items, 1);
+}
+
+namespace {
+
+template <int VEC, bool NT>
+void launch_merge_vec(const CsrView& P, const float* X, float* Y, int64_t F, int64_t waves,
+                      int32_t items, int lds, hipStream_t s) {
+    const MergeScratch& m = *P.merge;
+    hipLaunchKernelGGL(k_merge_split, dim3((unsigned)((waves + 1 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, P.row_ptr, P.n, (int32_t)P.nnz, items,
+                       (int32_t)waves, m.split);
+    const int64_t wgs = (waves + kBlock / 64 - 1) / (kBlock / 64);
+    const int64_t slabs = (F + 64 * VEC - 1) / (64 * VEC);
+    if (wgs * slabs > INT32_MAX) throw std::runtime_error("merge-path grid too large");
+    hipLaunchKernelGGL((k_spmm_merge<VEC, 8, NT>), dim3((unsigned)(wgs * slabs)),
+                       dim3(kBlock), (unsigned)lds, s, P.row_ptr, P.col, P.val, m.split, X,
+                       Y, P.n, (int32_t)F, (int32_t)waves, (int32_t)wgs, m.carry_row,
+                       m.carry_val);
+    hipLaunchKernelGGL((k_merge_fixup<VEC>), dim3((unsigned)wgs), dim3(kBlock), 0, s,
+                       m.carry_row, m.carry_val, Y, (int32_t)waves, (int32_t)F);
+}
+
+void launch_spmm_merge(const CsrView& P, const float* X, float* Y, int64_t F,
+                       const SpmmShape& shape, hipStream_t s) {
+    if ((int64_t)P.n + P.nnz >= INT32_MAX)
+        throw std::runtime_error("merge path: rows + nonzeros must fit int32");
+    const int64_t waves = merge_waves(P.n, P.nnz, shape.merge);
+    if (!P.merge || P.merge->waves < waves || P.merge->floats < waves * F)
+        throw std::runtime_error("merge-path scratch missing or too small for this launch");
+    const int32_t items = 256 << (shape.merge - 1);
+    const int lds = occupancy_lds_bytes(shape.wg_per_cu);
+#define SCL_MERGE(V)                                                                  \
+    (shape.nontemporal ? launch_merge_vec<V, true>(P, X, Y, F, waves, items, lds, s)   \
+                       : launch_merge_vec<V, false>(P, X, Y, F, waves, items, lds, s))
+    if (shape.vec == 4) SCL_MERGE(4);
+    else if (shape.vec == 2) SCL_MERGE(2);
+    else SCL_MERGE(1);
+#undef SCL_MERGE
+    check_launch("merge-path spmm launch");
+}
+
+}  // namespace
+
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
                      const SpmmShape& shape, hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
+    if (shape.merge > 0) {
+        launch_spmm_merge(P, X, Y, F, shape, s);
+        return;
+    }
     const int rpb = (kBlock / 64) * (64 / shape.group);
     const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
     const int64_t n_slabs = (F + (int64_t)shape.group * shape.vec - 1) /

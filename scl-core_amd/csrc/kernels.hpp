@@ -9,6 +9,16 @@
 namespace scl_amd {
 
 // Operator view handed to the launchers: CSR with int32 indices.
+// Device scratch of the merge-path SpMM: per wave, its (row, nonzero) start
+// on the merge path and the partial sum of the row it leaves unfinished.
+struct MergeScratch {
+    int32_t* split = nullptr;     // 2 * (waves + 1): (row, nonzero) pairs
+    int32_t* carry_row = nullptr; // waves: unfinished row, or -1
+    float* carry_val = nullptr;   // waves * F: its partial sum
+    int64_t waves = 0;            // capacity in waves
+    int64_t floats = 0;           // capacity of carry_val
+};
+
 struct CsrView {
     int32_t n;            // rows of the applied operator
     int32_t n_sec;        // columns
@@ -17,6 +27,7 @@ struct CsrView {
     const int32_t* col;
     const float* val;
     const int32_t* sched;  // row visiting order or null
+    const MergeScratch* merge = nullptr;  // for merge-path shapes
 };
 
 // Launch shape of the row-split SpMM (see kernels.hip).
@@ -30,6 +41,8 @@ struct SpmmShape {
     bool scalar_rows = false;  // group 16/32: rows' indices on the scalar unit
     int wg_per_cu = 0;  // cap on resident workgroups per CU (0: none), set by
                         // reserving LDS the kernel does not use
+    int merge = 0;      // 0: row-split; m > 0: merge path, 256 << (m - 1)
+                        // path items (row ends + nonzeros) per wave
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
@@ -42,6 +55,12 @@ constexpr int kShapeOccupancyShift = 8;
 constexpr int kShapeOccupancyMask = 0xF;
 // LDS reserved per workgroup so that at most cap workgroups fit on a CU.
 int occupancy_lds_bytes(int wg_per_cu);
+// bits 12..15: merge-path code (0 = row-split, 1..8)
+constexpr int kShapeMergeShift = 12;
+constexpr int kShapeMergeMask = 0xF;
+constexpr int kMergeMax = 8;
+// Waves a merge-path launch with code m uses for n rows and nnz nonzeros.
+int64_t merge_waves(int64_t n, int64_t nnz, int m);
 
 // True when the shape can run on F features with these X / Y pointers.
 bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* Y);
