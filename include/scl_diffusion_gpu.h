@@ -102,7 +102,15 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           are not bit-identical to the row-split shapes.  Setting it
  *           allocates the per-wave scratch (calls on one operator with a
  *           merge-path shape must not overlap on two streams).  The
- *           autotuner never picks it
+ *           autotuner never picks it;
+ *           SCL_GPU_SHAPE_ROW_GROUPS(r), r = 4 or 8 (group 64, transpose 0):
+ *           one wave computes r consecutive rows of the visiting order,
+ *           walking the sorted union of their column indices and gathering
+ *           each X row once for all the rows that hold it; every row keeps
+ *           its sorted-order FMA chain (bit-identical).  Needs strictly
+ *           increasing column indices in every row and < 2^24 columns;
+ *           setting it builds the grouped form on the host (one pass over
+ *           the pattern), which follows later value changes
  * Any row-split SpMM shape gives bit-identical results; only the speed
  * differs.  The
  * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
@@ -113,6 +121,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_SCALAR_ROWS 8
 #define SCL_GPU_SHAPE_OCCUPANCY(c) (((c) & 0xF) << 8)
 #define SCL_GPU_SHAPE_MERGE_PATH(m) (((m) & 0xF) << 12)
+#define SCL_GPU_SHAPE_ROW_GROUPS(r) ((((r) / 4) & 0x3) << 16)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

@@ -3,7 +3,10 @@
 // sparse handles (int32 indices, float values, optional transposed pattern).
 #include "internal.hpp"
 
+#include <algorithm>
+#include <array>
 #include <limits>
+#include <vector>
 
 namespace scl_amd {
 
@@ -126,8 +129,25 @@ DeviceCSR::~DeviceCSR() {
     dev_free(merge.split);
     dev_free(merge.carry_row);
     dev_free(merge.carry_val);
+    release_grouped();
     release_transpose();
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+}
+
+void DeviceCSR::release_grouped() {
+    for (GroupedCsr& g : grouped) {
+        dev_free(g.gptr);
+        dev_free(g.gword);
+        dev_free(g.gvptr);
+        dev_free(g.gval);
+        dev_free(g.gperm);
+        dev_free(g.grow);
+        g = GroupedCsr{};
+    }
+    // shapes that read the grouped form go with it
+    for (auto* m : {&shapes, &t_shapes})
+        for (auto it = m->begin(); it != m->end();)
+            it = it->second.rows_per_group ? m->erase(it) : std::next(it);
 }
 
 void DeviceCSR::release_transpose() {
@@ -214,6 +234,23 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->adj_shapes.clear();
         }
     }
+    if (slot->grouped[0].R || slot->grouped[1].R) {
+        // same pattern: re-gather the grouped values; else drop the forms
+        const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
+        bool keep = same_shape;
+        for (const GroupedCsr& g : slot->grouped)
+            if (g.R && g.fingerprint != fp) keep = false;
+        if (keep) {
+            for (GroupedCsr& g : slot->grouped)
+                if (g.R) launch_gather_f32(g.gperm, slot->val, g.gval, nnz, nullptr);
+            hip_check(hipStreamSynchronize(nullptr), "refresh grouped values");
+        } else {
+            if (slot->pins > 0)
+                fail(SCL_ERROR_INVALID_ARGUMENT,
+                     "the operator's pattern changed while a captured plan uses it");
+            slot->release_grouped();
+        }
+    }
     if (slot->pins > 0 && slot->t_row_ptr) {
         // a plan holds the transposed arrays: refresh their values in place
         launch_gather_f32(slot->t_src, slot->val, slot->t_val, nnz, nullptr);
@@ -261,6 +298,98 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
     }
     if (q >= kScheduleMinRows) d.t_sched = upload_schedule(locality_schedule(q, trp.data(), tcol.data()));
     d.t_generation = d.generation;
+}
+
+
+void refresh_grouped_values(DeviceCSR& d, hipStream_t s) {
+    for (GroupedCsr& g : d.grouped)
+        if (g.R) launch_gather_f32(g.gperm, d.val, g.gval, d.nnz, s);
+}
+
+bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
+    GroupedCsr& g = d.grouped[R == 8 ? 1 : 0];
+    const int64_t p = h->primary(), q = h->secondary(), nnz = h->nnz;
+    const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
+    if (g.R == R && g.fingerprint == fp) return true;
+    if (q >= (int64_t)1 << 24) return false;
+    for (int64_t i = 0; i < p; ++i)
+        for (int64_t k = h->indptr[i] + 1; k < h->indptr[i + 1]; ++k)
+            if (h->indices[k] <= h->indices[k - 1]) return false;
+    if (d.pins > 0)
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "a diffusion plan holds this operator: its grouped form cannot be rebuilt");
+    // groups follow the device's visiting order (the locality schedule)
+    std::vector<int32_t> order((size_t)p);
+    if (d.sched) {
+        hip_check(hipMemcpy(order.data(), d.sched, (size_t)p * 4, hipMemcpyDeviceToHost),
+                  "download schedule");
+    } else {
+        for (int64_t i = 0; i < p; ++i) order[i] = (int32_t)i;
+    }
+    const int64_t n_groups = (p + R - 1) / R;
+    std::vector<int32_t> gptr((size_t)n_groups + 1, 0), gvptr((size_t)n_groups + 1, 0);
+    std::vector<int32_t> grow((size_t)n_groups * R, -1), gperm;
+    std::vector<uint32_t> gword;
+    gperm.reserve((size_t)nnz);
+    gword.reserve((size_t)nnz);
+    // (column, slot, position) of the group's nonzeros, ordered by column then
+    // slot: one entry per distinct column, values in slot order
+    std::vector<std::array<int64_t, 3>> items;
+    for (int64_t gi = 0; gi < n_groups; ++gi) {
+        items.clear();
+        for (int r = 0; r < R; ++r) {
+            const int64_t rr = gi * R + r;
+            if (rr >= p) break;
+            const int32_t row = order[rr];
+            grow[gi * R + r] = row;
+            for (int64_t k = h->indptr[row]; k < h->indptr[row + 1]; ++k)
+                items.push_back({(int64_t)h->indices[k], r, k});
+        }
+        std::sort(items.begin(), items.end());
+        for (size_t a = 0; a < items.size();) {
+            uint32_t mask = 0;
+            size_t b = a;
+            for (; b < items.size() && items[b][0] == items[a][0]; ++b) {
+                mask |= 1u << items[b][1];
+                gperm.push_back((int32_t)items[b][2]);
+            }
+            gword.push_back((uint32_t)items[a][0] | (mask << 24));
+            a = b;
+        }
+        gptr[gi + 1] = (int32_t)gword.size();
+        gvptr[gi + 1] = (int32_t)gperm.size();
+    }
+    // replace the old form (if any) by the new one
+    dev_free(g.gptr);
+    dev_free(g.gword);
+    dev_free(g.gvptr);
+    dev_free(g.gval);
+    dev_free(g.gperm);
+    dev_free(g.grow);
+    g = GroupedCsr{};
+    const int64_t ne = (int64_t)gword.size();
+    g.gptr = dev_alloc<int32_t>(gptr.size(), "grouped gptr");
+    g.gvptr = dev_alloc<int32_t>(gvptr.size(), "grouped gvptr");
+    g.grow = dev_alloc<int32_t>(grow.size(), "grouped rows");
+    g.gword = dev_alloc<uint32_t>((size_t)std::max<int64_t>(ne, 1), "grouped words");
+    g.gperm = dev_alloc<int32_t>((size_t)std::max<int64_t>(nnz, 1), "grouped perm");
+    g.gval = dev_alloc<float>((size_t)nnz + 64, "grouped values");
+    hip_check(hipMemcpy(g.gptr, gptr.data(), gptr.size() * 4, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(g.gvptr, gvptr.data(), gvptr.size() * 4, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(g.grow, grow.data(), grow.size() * 4, hipMemcpyHostToDevice), "upload");
+    if (ne > 0)
+        hip_check(hipMemcpy(g.gword, gword.data(), (size_t)ne * 4, hipMemcpyHostToDevice), "upload");
+    if (nnz > 0)
+        hip_check(hipMemcpy(g.gperm, gperm.data(), (size_t)nnz * 4, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemset(g.gval, 0, ((size_t)nnz + 64) * sizeof(float)), "clear grouped values");
+    launch_gather_f32(g.gperm, d.val, g.gval, nnz, nullptr);
+    hip_check(hipStreamSynchronize(nullptr), "grouped values");
+    g.R = R;
+    g.n_groups = n_groups;
+    g.n_entries = ne;
+    g.fingerprint = fp;
+    g.schedule = d.sched != nullptr;
+    return true;
 }
 
 }  // namespace scl_amd

@@ -20,7 +20,8 @@ namespace {
 constexpr int64_t kInt32Max = std::numeric_limits<int32_t>::max();
 
 CsrView view_of(const DeviceCSR& d) {
-    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge};
+    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge,
+                   d.grouped};
 }
 
 CsrView transposed_view_of(const DeviceCSR& d) {
@@ -42,6 +43,10 @@ void check_operator(const scl_sparse_matrix* P, int64_t n_nodes, int64_t F) {
     if (F > kInt32Max)
         fail(SCL_ERROR_INVALID_ARGUMENT, "n_features too large");
 }
+
+// Operators smaller than this are not worth a grouped form in the autotuner
+// (the form costs a host pass over the pattern).
+constexpr int64_t kGroupedMinRows = 4096;
 
 // Launch shape for F features: the one tuned or set for this operator, when
 // it fits the buffers, else the default.  Caller holds the handle's mutex.
@@ -103,6 +108,7 @@ void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
     double* scratch = deg + n;
     launch_row_sums_f64(d.row_ptr, d.val, n, deg, s);
     launch_normalize_rows(d.row_ptr, d.col, d.val, n, deg, symmetric, s);
+    refresh_grouped_values(d, s);
     if (A->nnz > 0)
         hip_check(hipMemcpyAsync(A->data, d.val, (size_t)A->nnz * 4,
                                  hipMemcpyDeviceToHost, s),
@@ -351,6 +357,10 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.scalar_rows = (flags & kShapeScalarRows) != 0;
     sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
     sh.merge = (flags >> kShapeMergeShift) & kShapeMergeMask;
+    const int gcode = (flags >> kShapeGroupShift) & kShapeGroupMask;
+    SCL_API_CHECK(gcode <= 2 && (gcode == 0 || transpose == 0), SCL_ERROR_INVALID_ARGUMENT,
+                  "grouped form: code 1 (4 rows) or 2 (8 rows), forward direction only");
+    sh.rows_per_group = gcode ? 4 * gcode : 0;
     SCL_API_CHECK(sh.merge <= kMergeMax && (sh.merge == 0 || transpose != 2),
                   SCL_ERROR_INVALID_ARGUMENT,
                   "merge-path code must be 0..8, and 0 for the fused adjoint");
@@ -377,6 +387,10 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
         if (sh.merge > 0)
             ensure_merge_scratch(d, merge_waves(transpose ? d.n_sec : d.n, d.nnz, sh.merge),
                                  n_features);
+        if (sh.rows_per_group > 0 && !ensure_grouped(transition, d, sh.rows_per_group))
+            fail(SCL_ERROR_INVALID_ARGUMENT,
+                 "grouped form needs strictly increasing column indices in every row "
+                 "and fewer than 2^24 columns");
         (transpose == 2 ? d.adj_shapes : transpose ? d.t_shapes : d.shapes)[n_features] = sh;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
@@ -423,7 +437,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
                  (sh.xcd_rows ? kShapeXcdRows : 0) |
                  (sh.scalar_rows ? kShapeScalarRows : 0) |
                  (sh.wg_per_cu << kShapeOccupancyShift) |
-                 (sh.merge << kShapeMergeShift);
+                 (sh.merge << kShapeMergeShift) |
+                 ((sh.rows_per_group / 4) << kShapeGroupShift);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }
@@ -457,11 +472,29 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         float* dpt = nullptr;
         if (transpose == 2 && d.nnz > 0)
             dpt = static_cast<float*>(device_workspace(dev, 4, (size_t)d.nnz * sizeof(float)));
-        const auto cands = transpose == 2
-                               ? adjoint_candidate_shapes(n_features, X_dev, Y_dev, nullptr,
-                                                          v.sched != nullptr)
-                               : spmm_candidate_shapes(n_features, X_dev, Y_dev,
-                                                       v.sched != nullptr);
+        auto cands = transpose == 2
+                         ? adjoint_candidate_shapes(n_features, X_dev, Y_dev, nullptr,
+                                                    v.sched != nullptr)
+                         : spmm_candidate_shapes(n_features, X_dev, Y_dev, v.sched != nullptr);
+        if (transpose == 0 && kGroupedMinRows <= d.n) {
+            // row groups share gathers of common neighbours: try 4 and 8
+            // rows per wave when the pattern allows the grouped form
+            for (int R : {4, 8}) {
+                if (!ensure_grouped(transition, d, R)) break;
+                for (int vec : {4, 2})
+                    for (int occ : {0, 4})
+                        for (int nt = 0; nt < 2; ++nt) {
+                            SpmmShape c;
+                            c.vec = vec;
+                            c.group = 64;
+                            c.rows_per_group = R;
+                            c.wg_per_cu = occ;
+                            c.nontemporal = nt != 0;
+                            if (spmm_shape_valid(c, n_features, X_dev, Y_dev))
+                                cands.push_back(c);
+                        }
+            }
+        }
         SCL_API_CHECK(!cands.empty(), SCL_ERROR_INVALID_ARGUMENT,
                       "no launch shape fits these buffers");
         auto run = [&](const SpmmShape& c) {

@@ -36,7 +36,8 @@ void check_square(const scl_sparse_matrix* P, int64_t n, int64_t F) {
 int64_t n_checkpoints(int64_t t, int64_t every) { return (t + every - 1) / every - 1; }
 
 CsrView fwd_view(const DeviceCSR& d) {
-    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge};
+    return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge,
+                   d.grouped};
 }
 CsrView adj_view(const DeviceCSR& d) {
     return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched,
@@ -77,6 +78,7 @@ SCL_EXPORT scl_error_t scl_gpu_sparse_set_values_device(scl_sparse_t matrix,
                                      hipMemcpyDeviceToDevice, s),
                       "copy values");
             if (keep_t) launch_gather_f32(d.t_src, d.val, d.t_val, matrix->nnz, s);
+            refresh_grouped_values(d, s);
             hip_check(hipMemcpyAsync(matrix->data, d.val, (size_t)matrix->nnz * 4,
                                      hipMemcpyDeviceToHost, s),
                       "download values");

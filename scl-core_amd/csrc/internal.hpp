@@ -102,8 +102,12 @@ struct DeviceCSR {
     // scratch of merge-path launches (both directions share it; grown when a
     // merge-path shape is set, never while plans hold pointers into it)
     MergeScratch merge;
+    // row-grouped forms of the forward operator, R = 4 and R = 8 (built when
+    // a grouped shape is set or tuned; values refreshed with the operator's)
+    GroupedCsr grouped[2];
 
     ~DeviceCSR();
+    void release_grouped();
     void release_transpose();
 };
 
@@ -145,6 +149,13 @@ std::mutex& device_mutex(int device);
 void* device_workspace(int device, int slot, size_t bytes);
 // Grows d.merge to hold `waves` waves of F features.
 void ensure_merge_scratch(DeviceCSR& d, int64_t waves, int64_t F);
+// Builds d.grouped[R == 8] for the handle's current pattern when missing or
+// stale; false when the pattern cannot be grouped (a row with unsorted or
+// repeated column indices, or >= 2^24 columns).
+bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R);
+// Re-gathers the grouped forms' values from d.val (after the device values
+// changed in place), stream-ordered on s.
+void refresh_grouped_values(DeviceCSR& d, hipStream_t s);
 
 // Operators with at least this many rows get a locality schedule.
 constexpr int64_t kScheduleMinRows = 16384;

@@ -139,6 +139,76 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
 }
 
 
+// ---------------------------------------------------------- row groups --
+// One wave computes R rows (a group of consecutive rows in visiting order)
+// for one slab of 64 * VEC features.  It walks the union of the rows' column
+// indices in increasing order and gathers each X[j, slab] once for all the
+// rows holding column j (the mask in the entry's top 8 bits), so neighbour
+// rows shared inside the group cost one gather instead of several.  Each row
+// still meets its nonzeros in sorted order with one fp32 FMA each: the same
+// chain as the row-split kernels, bit for bit.  The values of a chunk of U
+// entries (at most U * R <= 64 of them) are loaded one per lane and handed
+// to the FMAs with readlane, so no scalar load sits inside the row loop.
+template <int VEC, int R, int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_spmm_grouped(
+    const int32_t* __restrict__ gptr, const uint32_t* __restrict__ gword,
+    const int32_t* __restrict__ gvptr, const float* __restrict__ gval,
+    const int32_t* __restrict__ grow, const float* __restrict__ X, float* __restrict__ Y,
+    int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd) {
+    static_assert(U * R <= 64, "a chunk's values must fit one per lane");
+    using VT = typename Vec<VEC>::T;
+    int slab, gb;
+    if (!block_coords(n_gblocks, rb_per_xcd, slab, gb)) return;
+    const int lane = threadIdx.x & 63;
+    const int g = gb * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (g >= n_groups) return;
+    const int f = slab * (64 * VEC) + lane * VEC;
+    const bool active = f < F;
+    const float* Xf = X + (active ? f : 0);
+    const int e1 = __builtin_amdgcn_readfirstlane(gptr[g + 1]);
+    int e = __builtin_amdgcn_readfirstlane(gptr[g]);
+    int vo = __builtin_amdgcn_readfirstlane(gvptr[g]);
+
+    float acc[R][VEC];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int c = 0; c < VEC; ++c) acc[r][c] = 0.0f;
+
+    for (; e < e1; e += U) {
+        uint32_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = gword[min(e + u, e1 - 1)];
+        VT x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * F);
+        // this chunk's values, one per lane (gval is padded by 64)
+        const int vl = __builtin_bit_cast(int, gval[vo + lane]);
+        int c = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (e + u < e1) {
+                const uint32_t m = w[u] >> 24;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (m & (1u << r)) {
+                        const float v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(vl, c));
+                        ++c;
+                        fma_vec(v, x[u], acc[r]);
+                    }
+                }
+            }
+        }
+        vo += c;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = __builtin_amdgcn_readfirstlane(grow[(size_t)g * R + r]);
+        if (row >= 0 && active) store_vec<VEC, NT>(Y + (size_t)row * F + f, acc[r]);
+    }
+}
+
 // ------------------------------------------------------------ merge path --
 // The merge path of a CSR matrix interleaves its n row ends with its nnz
 // nonzeros: nonzero k is consumed while k < row_ptr[i + 1] for the current
@@ -420,6 +490,10 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     if (s.wg_per_cu < 0 || s.wg_per_cu > kShapeOccupancyMask) return false;
     // merge path: lanes over features, one piece of the path per wave
     if (s.merge < 0 || s.merge > kMergeMax || (s.merge > 0 && s.group != 64)) return false;
+    // grouped form: lanes over features, 4 or 8 rows per wave, not with merge
+    if (s.rows_per_group != 0 &&
+        (!(s.rows_per_group == 4 || s.rows_per_group == 8) || s.group != 64 || s.merge))
+        return false;
     return s.unroll == 4 || s.unroll == 8 || s.unroll == 16;
 }
 
@@ -529,11 +603,57 @@ void launch_spmm_merge(const CsrView& P, const float* X, float* Y, int64_t F,
 
 }  // namespace
 
+namespace {
+
+template <int VEC, int R, bool NT>
+void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F,
+                        bool xcd_rows, int lds, hipStream_t s) {
+    constexpr int U = 64 / R;
+    const int64_t n_gblocks = (G.n_groups + kBlock / 64 - 1) / (kBlock / 64);
+    const int64_t n_slabs = (F + 64 * VEC - 1) / (64 * VEC);
+    const int64_t rb_per_xcd = xcd_rows ? (n_gblocks + kNumXcd - 1) / kNumXcd : 0;
+    const int64_t nb = xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : n_gblocks * n_slabs;
+    if (nb > INT32_MAX) throw std::runtime_error("grouped spmm grid too large");
+    hipLaunchKernelGGL((k_spmm_grouped<VEC, R, U, NT>), dim3((unsigned)nb), dim3(kBlock),
+                       (unsigned)lds, s, G.gptr, G.gword, G.gvptr, G.gval, G.grow, X, Y,
+                       (int32_t)G.n_groups, (int32_t)F, (int32_t)n_gblocks,
+                       (int32_t)rb_per_xcd);
+}
+
+void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
+                         const SpmmShape& shape, hipStream_t s) {
+    const int R = shape.rows_per_group;
+    const GroupedCsr* G = P.grouped ? &P.grouped[R == 8 ? 1 : 0] : nullptr;
+    if (!G || G->R != R || G->n_groups * R < P.n)
+        throw std::runtime_error("grouped form of the operator missing for this launch");
+    const int lds = occupancy_lds_bytes(shape.wg_per_cu);
+#define SCL_GROUPED(V, RR)                                                              \
+    (shape.nontemporal ? launch_grouped_vec<V, RR, true>(*G, X, Y, F, shape.xcd_rows, lds, s) \
+                       : launch_grouped_vec<V, RR, false>(*G, X, Y, F, shape.xcd_rows, lds, s))
+    if (R == 8) {
+        if (shape.vec == 4) SCL_GROUPED(4, 8);
+        else if (shape.vec == 2) SCL_GROUPED(2, 8);
+        else SCL_GROUPED(1, 8);
+    } else {
+        if (shape.vec == 4) SCL_GROUPED(4, 4);
+        else if (shape.vec == 2) SCL_GROUPED(2, 4);
+        else SCL_GROUPED(1, 4);
+    }
+#undef SCL_GROUPED
+    check_launch("grouped spmm launch");
+}
+
+}  // namespace
+
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
                      const SpmmShape& shape, hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
     if (shape.merge > 0) {
         launch_spmm_merge(P, X, Y, F, shape, s);
+        return;
+    }
+    if (shape.rows_per_group > 0) {
+        launch_spmm_grouped(P, X, Y, F, shape, s);
         return;
     }
     const int rpb = (kBlock / 64) * (64 / shape.group);

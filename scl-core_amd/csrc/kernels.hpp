@@ -19,6 +19,25 @@ struct MergeScratch {
     int64_t floats = 0;           // capacity of carry_val
 };
 
+// Row-grouped form of an operator (the "grouped" SpMM shapes): rows are
+// taken R at a time in visiting order; a group's entries are the union of its
+// rows' column indices in increasing order, each with the mask of the rows
+// holding it.  gval lists the values entry by entry (row slots in order), so
+// every row still meets its own nonzeros in stored (sorted) order.
+struct GroupedCsr {
+    int R = 0;                    // rows per group (4 or 8); 0: not built
+    int64_t n_groups = 0;
+    int64_t n_entries = 0;
+    int32_t* gptr = nullptr;      // n_groups + 1: first entry of each group
+    uint32_t* gword = nullptr;    // n_entries: column | mask << 24
+    int32_t* gvptr = nullptr;     // n_groups + 1: first value of each group
+    float* gval = nullptr;        // nnz + 64 (padded for whole-wave loads)
+    int32_t* gperm = nullptr;     // nnz: gval[i] = val[gperm[i]]
+    int32_t* grow = nullptr;      // n_groups * R: the rows, -1 past the end
+    uint64_t fingerprint = 0;     // pattern it was built for
+    bool schedule = false;        // built over the locality order
+};
+
 struct CsrView {
     int32_t n;            // rows of the applied operator
     int32_t n_sec;        // columns
@@ -28,6 +47,7 @@ struct CsrView {
     const float* val;
     const int32_t* sched;  // row visiting order or null
     const MergeScratch* merge = nullptr;  // for merge-path shapes
+    const GroupedCsr* grouped = nullptr;  // [0]: R = 4, [1]: R = 8
 };
 
 // Launch shape of the row-split SpMM (see kernels.hip).
@@ -43,6 +63,7 @@ struct SpmmShape {
                         // reserving LDS the kernel does not use
     int merge = 0;      // 0: row-split; m > 0: merge path, 256 << (m - 1)
                         // path items (row ends + nonzeros) per wave
+    int rows_per_group = 0;  // 0: one row per lane group; 4 or 8: grouped form
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
@@ -59,6 +80,9 @@ int occupancy_lds_bytes(int wg_per_cu);
 constexpr int kShapeMergeShift = 12;
 constexpr int kShapeMergeMask = 0xF;
 constexpr int kMergeMax = 8;
+// bits 16..17: grouped form (0 = none, 1 = 4 rows, 2 = 8 rows per group)
+constexpr int kShapeGroupShift = 16;
+constexpr int kShapeGroupMask = 0x3;
 // Waves a merge-path launch with code m uses for n rows and nnz nonzeros.
 int64_t merge_waves(int64_t n, int64_t nnz, int m);
 

@@ -249,3 +249,47 @@ def test_every_launch_shape_is_bit_identical(gpu, F):
                         assert torch_ops.get_shape(A, F)["wg_per_cu"] == (4 if u == 4 else 0)
                         Y = torch_ops.spmm(A, Xd).cpu().numpy()
                         assert np.array_equal(Y, ref), (vec, group, sr, xr, u)
+
+
+@pytest.mark.parametrize("F", [1, 50, 256, 2000])
+def test_row_grouped_shapes_are_bit_identical(gpu, F):
+    """Grouped form (4 or 8 rows per wave over the union of their columns):
+    every row keeps its sorted-order FMA chain, so the result is the
+    restatement's bit for bit -- with and without the locality schedule,
+    through t steps, and after the operator's values change in place."""
+    import torch
+    from scl_amd import torch_ops
+    for n in (20_000, 3_001):              # with a schedule; without, ragged end
+        g = synth.knn_graph(n, 12, seed=F + n)
+        A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+        ip, ix, p = A.export()
+        X = synth.normal(n, F, seed=F)
+        Xd = torch.from_numpy(X).cuda()
+        vmax = 4 if F % 4 == 0 else (2 if F % 2 == 0 else 1)
+        for R in (4, 8):
+            for vec in sorted({1, vmax}):
+                torch_ops.set_shape(A, F, vec, 64, 8, rows_per_group=R,
+                                    nontemporal=vec == 1, wg_per_cu=4 if R == 8 else 0)
+                assert torch_ops.get_shape(A, F)["rows_per_group"] == R
+                Y = torch_ops.spmm(A, Xd).cpu().numpy()
+                assert np.array_equal(Y, orc.spmm_step_f32(ip, ix, p, X)), (n, R, vec)
+        # values change in place (normalisation): the grouped values follow
+        diffusion.compute_transition_matrix(A)
+        ip, ix, p = A.export()
+        Y = torch_ops.diffuse(A, Xd, 3).cpu().numpy()
+        assert torch_ops.get_shape(A, F)["rows_per_group"] == 8
+        assert np.array_equal(Y, orc.diffuse_matrix_f32(ip, ix, p, X, 3))
+
+
+def test_row_grouped_needs_sorted_rows(gpu):
+    from scl_amd import torch_ops
+    ip = np.array([0, 2, 3], np.int64)
+    A = CsrMatrix.from_arrays(ip, np.array([1, 0, 1], np.int64),
+                              np.ones(3, np.float32), (2, 2))   # row 0 unsorted
+    with pytest.raises(Exception):
+        torch_ops.set_shape(A, 8, 4, 64, 8, rows_per_group=4)
+    B = CsrMatrix.from_arrays(ip, np.array([0, 1, 1], np.int64),
+                              np.ones(3, np.float32), (2, 2))
+    torch_ops.set_shape(B, 8, 4, 64, 8, rows_per_group=4)
+    with pytest.raises(Exception):   # forward direction only
+        torch_ops.set_shape(B, 8, 4, 64, 8, rows_per_group=4, transpose=True)
