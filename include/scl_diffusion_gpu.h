@@ -107,7 +107,9 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           one wave computes r consecutive rows of the visiting order,
  *           walking the sorted union of their column indices and gathering
  *           each X row once for all the rows that hold it; every row keeps
- *           its sorted-order FMA chain (bit-identical).  Needs strictly
+ *           its sorted-order FMA chain (bit-identical); unroll is the
+ *           union entries gathered per chunk (4 rows: 8 or 16, 8 rows: 4 or
+ *           8; larger values are clamped).  Needs strictly
  *           increasing column indices in every row and < 2^24 columns;
  *           setting it builds the grouped form on the host (one pass over
  *           the pattern), which follows later value changes

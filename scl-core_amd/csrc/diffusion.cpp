@@ -482,17 +482,19 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
             for (int R : {4, 8}) {
                 if (!ensure_grouped(transition, d, R)) break;
                 for (int vec : {4, 2})
-                    for (int occ : {0, 4})
-                        for (int nt = 0; nt < 2; ++nt) {
-                            SpmmShape c;
-                            c.vec = vec;
-                            c.group = 64;
-                            c.rows_per_group = R;
-                            c.wg_per_cu = occ;
-                            c.nontemporal = nt != 0;
-                            if (spmm_shape_valid(c, n_features, X_dev, Y_dev))
-                                cands.push_back(c);
-                        }
+                    for (int u : {R == 8 ? 4 : 8, R == 8 ? 8 : 16})
+                        for (int occ : {0, 4})
+                            for (int nt = 0; nt < 2; ++nt) {
+                                SpmmShape c;
+                                c.vec = vec;
+                                c.group = 64;
+                                c.unroll = u;
+                                c.rows_per_group = R;
+                                c.wg_per_cu = occ;
+                                c.nontemporal = nt != 0;
+                                if (spmm_shape_valid(c, n_features, X_dev, Y_dev))
+                                    cands.push_back(c);
+                            }
             }
         }
         SCL_API_CHECK(!cands.empty(), SCL_ERROR_INVALID_ARGUMENT,

@@ -605,10 +605,9 @@ void launch_spmm_merge(const CsrView& P, const float* X, float* Y, int64_t F,
 
 namespace {
 
-template <int VEC, int R, bool NT>
+template <int VEC, int R, int U, bool NT>
 void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F,
                         bool xcd_rows, int lds, hipStream_t s) {
-    constexpr int U = 64 / R;
     const int64_t n_gblocks = (G.n_groups + kBlock / 64 - 1) / (kBlock / 64);
     const int64_t n_slabs = (F + 64 * VEC - 1) / (64 * VEC);
     const int64_t rb_per_xcd = xcd_rows ? (n_gblocks + kNumXcd - 1) / kNumXcd : 0;
@@ -627,18 +626,25 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
     if (!G || G->R != R || G->n_groups * R < P.n)
         throw std::runtime_error("grouped form of the operator missing for this launch");
     const int lds = occupancy_lds_bytes(shape.wg_per_cu);
-#define SCL_GROUPED(V, RR)                                                              \
-    (shape.nontemporal ? launch_grouped_vec<V, RR, true>(*G, X, Y, F, shape.xcd_rows, lds, s) \
-                       : launch_grouped_vec<V, RR, false>(*G, X, Y, F, shape.xcd_rows, lds, s))
+    // U entries per chunk with U * R <= 64 (a chunk's values fit one per lane)
+#define SCL_GROUPED(V, RR, UU)                                                          \
+    (shape.nontemporal                                                                  \
+         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape.xcd_rows, lds, s)     \
+         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape.xcd_rows, lds, s))
+#define SCL_GROUPED_V(RR, UU)                          \
+    do {                                               \
+        if (shape.vec == 4) SCL_GROUPED(4, RR, UU);    \
+        else if (shape.vec == 2) SCL_GROUPED(2, RR, UU); \
+        else SCL_GROUPED(1, RR, UU);                   \
+    } while (0)
     if (R == 8) {
-        if (shape.vec == 4) SCL_GROUPED(4, 8);
-        else if (shape.vec == 2) SCL_GROUPED(2, 8);
-        else SCL_GROUPED(1, 8);
+        if (shape.unroll >= 8) SCL_GROUPED_V(8, 8);
+        else SCL_GROUPED_V(8, 4);
     } else {
-        if (shape.vec == 4) SCL_GROUPED(4, 4);
-        else if (shape.vec == 2) SCL_GROUPED(2, 4);
-        else SCL_GROUPED(1, 4);
+        if (shape.unroll >= 16) SCL_GROUPED_V(4, 16);
+        else SCL_GROUPED_V(4, 8);
     }
+#undef SCL_GROUPED_V
 #undef SCL_GROUPED
     check_launch("grouped spmm launch");
 }
