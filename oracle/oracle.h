@@ -141,6 +141,25 @@ double orc_median_rel_err_f32_f64(size_t len, const float* a, const double* b);
 /* Thread count the restatement runs with (omp_get_max_threads). */
 int orc_num_threads(void);
 
+/* ---- vector-side consumers (walks.c): diffusion.hpp:670-1194 ---------- */
+int orc_lazy_walk_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                      float* x, int64_t n_steps, float laziness);
+int orc_heat_kernel_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                        float* sig, float t, int64_t n_steps);
+int64_t orc_rwr_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                    const int64_t* seeds, int64_t n_seeds, float* scores, float alpha,
+                    int64_t max_iter, float tol);
+int orc_dpt_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                int64_t root, float* pt, int64_t max_iter, float tol);
+int orc_dpt_multi_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                      const int64_t* roots, int64_t n_roots, float* pt, int64_t max_iter);
+void orc_dmap_init_f32(int64_t n, int64_t K, float* Q);
+int orc_dmap_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                 float* emb, int64_t K, int64_t n_iter);
+int orc_mgs_f32(int64_t n, int64_t K, float* Q);
+int orc_diffusion_distance_f32(int64_t n, const int64_t* ip, const int64_t* ix,
+                               const float* v, float* D, int64_t n_steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,15 @@ def load():
         "orc_diffuse_lowp_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  P, P]),
+        "orc_lazy_walk_f32": (ctypes.c_int, [I64, P, P, P, P, I64, ctypes.c_float]),
+        "orc_heat_kernel_f32": (ctypes.c_int, [I64, P, P, P, P, ctypes.c_float, I64]),
+        "orc_rwr_f32": (I64, [I64, P, P, P, P, I64, P, ctypes.c_float, I64, ctypes.c_float]),
+        "orc_dpt_f32": (ctypes.c_int, [I64, P, P, P, I64, P, I64, ctypes.c_float]),
+        "orc_dpt_multi_f32": (ctypes.c_int, [I64, P, P, P, P, I64, P, I64]),
+        "orc_dmap_init_f32": (None, [I64, I64, P]),
+        "orc_dmap_f32": (ctypes.c_int, [I64, P, P, P, P, I64, I64]),
+        "orc_mgs_f32": (ctypes.c_int, [I64, I64, P]),
+        "orc_diffusion_distance_f32": (ctypes.c_int, [I64, P, P, P, P, I64]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -248,3 +257,91 @@ def median_rel_err(a, b) -> float:
 
 def num_threads() -> int:
     return load().orc_num_threads()
+
+
+# ---------------------------------------------------- vector-side consumers --
+
+def _csr(indptr, indices, vals):
+    return _i64(indptr), _i64(indices), _f32(vals)
+
+
+def lazy_walk_f32(indptr, indices, vals, x, n_steps, laziness):
+    ip, ix, v = _csr(indptr, indices, vals)
+    y = _f32(x).copy()
+    if load().orc_lazy_walk_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                y.ctypes.data, n_steps, laziness):
+        raise MemoryError
+    return y
+
+
+def heat_kernel_f32(indptr, indices, vals, t, n_steps):
+    ip, ix, v = _csr(indptr, indices, vals)
+    sig = np.empty(len(ip) - 1, np.float32)
+    if load().orc_heat_kernel_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                  sig.ctypes.data, t, n_steps):
+        raise MemoryError
+    return sig
+
+
+def rwr_f32(indptr, indices, vals, seeds, alpha, max_iter, tol):
+    """Returns (scores, iterations run)."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    sd = _i64(seeds)
+    out = np.empty(len(ip) - 1, np.float32)
+    it = load().orc_rwr_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                            sd.ctypes.data, len(sd), out.ctypes.data, alpha, max_iter, tol)
+    if it < 0:
+        raise MemoryError
+    return out, it
+
+
+def dpt_f32(indptr, indices, vals, root, max_iter, tol):
+    ip, ix, v = _csr(indptr, indices, vals)
+    out = np.empty(len(ip) - 1, np.float32)
+    if load().orc_dpt_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, root,
+                          out.ctypes.data, max_iter, tol):
+        raise MemoryError
+    return out
+
+
+def dpt_multi_f32(indptr, indices, vals, roots, max_iter):
+    ip, ix, v = _csr(indptr, indices, vals)
+    r = _i64(roots)
+    out = np.empty(len(ip) - 1, np.float32)
+    if load().orc_dpt_multi_f32(len(ip) - 1, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                r.ctypes.data, len(r), out.ctypes.data, max_iter):
+        raise MemoryError
+    return out
+
+
+def dmap_init_f32(n, K):
+    Q = np.empty((n, K), np.float32)
+    load().orc_dmap_init_f32(n, K, Q.ctypes.data)
+    return Q
+
+
+def dmap_f32(indptr, indices, vals, K, n_iter):
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    out = np.empty((n, K), np.float32)
+    if load().orc_dmap_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, out.ctypes.data,
+                           K, n_iter):
+        raise MemoryError
+    return out
+
+
+def mgs_f32(Q):
+    Q = _f32(Q).copy()
+    if load().orc_mgs_f32(Q.shape[0], Q.shape[1], Q.ctypes.data):
+        raise MemoryError
+    return Q
+
+
+def diffusion_distance_f32(indptr, indices, vals, n_steps):
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    D = np.empty((n, n), np.float32)
+    if load().orc_diffusion_distance_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                         D.ctypes.data, n_steps):
+        raise MemoryError
+    return D
