@@ -175,4 +175,28 @@ void launch_stationarity_residual(const int32_t* t_row_ptr, const int32_t* t_col
                                   const double* deg, double* partial,
                                   double* out, hipStream_t s);
 
+// ---- vector-side consumers (walks.hip) ----------------------------------
+// x = fma(a, x, b * t) elementwise.
+void launch_blend(float* x, const float* t, int64_t n, float a, float b, hipStream_t s);
+// y = fma(alpha, y, beta * r); *total = sum |y - x| (device scalar).
+void launch_rwr_update(float* y, const float* x, const float* r, int64_t n, float alpha,
+                       float beta, float* part, float* total, hipStream_t s);
+// One DPT bookkeeping step; *max_unvisited = max(1 - vis) (device scalar).
+void launch_dpt_update(const float* pn, float* vis, float* hit, int64_t n, float it,
+                       float* part, float* max_unvisited, hipStream_t s);
+// hit /= vis where vis > 0.5, pt = vis > 0.5 ? hit / max : 1.
+void launch_dpt_finish(const float* vis, float* hit, float* pt, int64_t n, float* part,
+                       float* max_time, hipStream_t s);
+// x = value * e_at (x = 0 when at is outside [0, n)).
+void launch_set_unit(float* x, int64_t n, int64_t at, float value, hipStream_t s);
+void launch_add(float* acc, const float* x, int64_t n, hipStream_t s);
+void launch_scale(float* x, int64_t n, float a, hipStream_t s);
+// Diffusion-map start vectors and modified Gram-Schmidt (one workgroup).
+void launch_dmap_init(float* Q, int64_t n, int64_t K, hipStream_t s);
+void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s);
+// A = I; B = A^T; pairwise row distances D of an n x n row-major matrix.
+void launch_identity(float* A, int64_t n, hipStream_t s);
+void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s);
+void launch_pairwise_rows(const float* A, float* D, int64_t n, hipStream_t s);
+
 }  // namespace scl_amd

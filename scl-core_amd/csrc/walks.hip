@@ -1,0 +1,373 @@
+// walks.hip -- gfx950 kernels of the vector-side diffusion consumers (lazy
+// walk, heat kernel signature, random walk with restart, diffusion
+// pseudotime, diffusion maps, diffusion distance).  The SpMV / SpMM steps run
+// on the row-split kernels of kernels.hip; these are the element-wise updates
+// between steps, their reductions, the Gram-Schmidt pass and the pairwise
+// distances.  Contraction is off: every fused multiply-add below is an
+// explicit fmaf, matching the CPU restatement operation for operation.
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace scl_amd {
+
+using namespace dev;
+
+namespace {
+
+constexpr int kParts = 1024;  // first-level partials of a reduction
+
+__device__ __forceinline__ float block_reduce(float v, bool is_max) {
+    __shared__ float red[kBlock / 64];
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        const float o = __shfl_xor(v, m, 64);
+        v = is_max ? fmaxf(v, o) : v + o;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = red[0];
+        for (int w = 1; w < kBlock / 64; ++w) r = is_max ? fmaxf(r, red[w]) : r + red[w];
+        red[0] = r;
+    }
+    __syncthreads();
+    const float r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// x[i] = fma(a, x[i], b * t[i])
+__global__ __launch_bounds__(kBlock) void k_blend(float* __restrict__ x,
+                                                  const float* __restrict__ t, int64_t n,
+                                                  float a, float b) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const float bt = b * t[i];
+        x[i] = __builtin_fmaf(a, x[i], bt);
+    }
+}
+
+// y[i] = fma(alpha, y[i], beta * r[i]); partial sums of |y[i] - x[i]|
+__global__ __launch_bounds__(kBlock) void k_rwr_update(float* __restrict__ y,
+                                                       const float* __restrict__ x,
+                                                       const float* __restrict__ r,
+                                                       int64_t n, float alpha, float beta,
+                                                       float* __restrict__ part) {
+    float s = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const float br = beta * r[i];
+        const float yi = __builtin_fmaf(alpha, y[i], br);
+        y[i] = yi;
+        s += fabsf(yi - x[i]);
+    }
+    s = block_reduce(s, false);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// One DPT step's bookkeeping (diffusion.hpp:710-733) and partial maxima of
+// 1 - visited.
+__global__ __launch_bounds__(kBlock) void k_dpt_update(const float* __restrict__ pn,
+                                                       float* __restrict__ vis,
+                                                       float* __restrict__ hit, int64_t n,
+                                                       float it, float* __restrict__ part) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        float v = vis[i];
+        const float p = pn[i];
+        if (v < 0.99f && p > 1e-15f) {
+            const float rem = 1.0f - v;
+            const float np = p * rem;
+            hit[i] = __builtin_fmaf(it, np, hit[i]);
+            v = v + np;
+            v = v < 1.0f ? v : 1.0f;
+            vis[i] = v;
+        }
+        m = fmaxf(m, 1.0f - v);
+    }
+    m = block_reduce(m, true);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// hit /= vis where vis > 0.5; partial maxima of those hitting times
+__global__ __launch_bounds__(kBlock) void k_dpt_normalise(const float* __restrict__ vis,
+                                                          float* __restrict__ hit, int64_t n,
+                                                          float* __restrict__ part) {
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        if (vis[i] > 0.5f) {
+            const float h = hit[i] / vis[i];
+            hit[i] = h;
+            m = fmaxf(m, h);
+        }
+    }
+    m = block_reduce(m, true);
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// pt = vis > 0.5 ? hit * inv_max : 1, with inv_max from the device maximum
+__global__ __launch_bounds__(kBlock) void k_dpt_finish(const float* __restrict__ vis,
+                                                       const float* __restrict__ hit,
+                                                       const float* __restrict__ max_time,
+                                                       float* __restrict__ pt, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float mt = *max_time;
+    const float inv = mt > 0.0f ? 1.0f / mt : 1.0f;
+    pt[i] = vis[i] > 0.5f ? hit[i] * inv : 1.0f;
+}
+
+// Second level: one block folds the partials into out[0].
+__global__ __launch_bounds__(kBlock) void k_fold(const float* __restrict__ part, int m,
+                                                 int is_max, float* __restrict__ out) {
+    float v = 0.0f;
+    for (int i = threadIdx.x; i < m; i += kBlock)
+        v = is_max ? fmaxf(v, part[i]) : v + part[i];
+    v = block_reduce(v, is_max != 0);
+    if (threadIdx.x == 0) out[0] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_set_unit(float* __restrict__ x, int64_t n,
+                                                     int64_t at, float value) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = (i == at) ? value : 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add(float* __restrict__ acc,
+                                                const float* __restrict__ x, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) acc[i] = acc[i] + x[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale(float* __restrict__ x, int64_t n, float a) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) x[i] = x[i] * a;
+}
+
+// Q[i][c] = cos(ph t) + 0.1 sin(ph t 2.7), t = i / n, ph = (c + 1) pi
+// (diffusion.hpp:922-929)
+__global__ __launch_bounds__(kBlock) void k_dmap_init(float* __restrict__ Q, int64_t n,
+                                                      int64_t K) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n * K) return;
+    const int64_t i = e / K, c = e - i * K;
+    const float t = (float)i / (float)n;
+    const float ph = (float)(c + 1) * 3.14159265358979323846f;
+    const float s = 0.1f * sinf(ph * t * 2.7f);
+    Q[e] = cosf(ph * t) + s;
+}
+
+// Modified Gram-Schmidt of the K columns of row-major n x K Q in one
+// workgroup (orthogonalize_vectors, diffusion.hpp:370-446): projections onto
+// earlier columns, one extra pass when some |<w, q_p>| / |w| >= 0.7,
+// normalisation, unit vector e_(c mod n) for a degenerate column.  w is an
+// n-float workspace.  Sums are block tree sums.
+__global__ __launch_bounds__(kBlock) void k_mgs(float* __restrict__ Q, float* __restrict__ w,
+                                                int64_t n, int64_t K) {
+    const int tid = threadIdx.x;
+    for (int64_t c = 0; c < K; ++c) {
+        for (int64_t i = tid; i < n; i += kBlock) w[i] = Q[i * K + c];
+        __syncthreads();
+        for (int reorth = 0; reorth <= 2; ++reorth) {
+            for (int64_t p = 0; p < c; ++p) {
+                float dot = 0.0f, nrm = 0.0f;
+                for (int64_t i = tid; i < n; i += kBlock) {
+                    const float q = Q[i * K + p];
+                    dot = __builtin_fmaf(w[i], q, dot);
+                    nrm = __builtin_fmaf(q, q, nrm);
+                }
+                dot = block_reduce(dot, false);
+                nrm = block_reduce(nrm, false);
+                if (nrm > 1e-15f) {
+                    const float coeff = dot / nrm;
+                    for (int64_t i = tid; i < n; i += kBlock)
+                        w[i] = __builtin_fmaf(-coeff, Q[i * K + p], w[i]);
+                }
+                __syncthreads();
+            }
+            float nv = 0.0f;
+            for (int64_t i = tid; i < n; i += kBlock) nv = __builtin_fmaf(w[i], w[i], nv);
+            nv = block_reduce(nv, false);
+            if (reorth == 0 && c > 0) {
+                float max_cos = 0.0f;
+                for (int64_t p = 0; p < c; ++p) {
+                    float dot = 0.0f;
+                    for (int64_t i = tid; i < n; i += kBlock)
+                        dot = __builtin_fmaf(w[i], Q[i * K + p], dot);
+                    dot = block_reduce(dot, false);
+                    const float cs = nv > 1e-15f ? fabsf(dot) / sqrtf(nv) : 0.0f;
+                    max_cos = fmaxf(max_cos, cs);
+                }
+                if (max_cos < 0.7f) break;
+            } else {
+                break;
+            }
+        }
+        float nrm = 0.0f;
+        for (int64_t i = tid; i < n; i += kBlock) nrm = __builtin_fmaf(w[i], w[i], nrm);
+        nrm = block_reduce(nrm, false);
+        if (nrm > 1e-15f) {
+            const float inv = 1.0f / sqrtf(nrm);
+            for (int64_t i = tid; i < n; i += kBlock) Q[i * K + c] = w[i] * inv;
+        } else {
+            const int64_t u = c % n;
+            for (int64_t i = tid; i < n; i += kBlock) Q[i * K + c] = (i == u) ? 1.0f : 0.0f;
+        }
+        __syncthreads();
+    }
+}
+
+// A = I (n x n, row-major)
+__global__ __launch_bounds__(kBlock) void k_identity(float* __restrict__ A, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < n * n) A[e] = (e / n == e % n) ? 1.0f : 0.0f;
+}
+
+// B = A^T for n x n row-major A (16 x 16 tiles through LDS).
+__global__ __launch_bounds__(kBlock) void k_transpose(const float* __restrict__ A,
+                                                      float* __restrict__ B, int64_t n) {
+    __shared__ float tile[16][17];
+    const int64_t bi = (int64_t)blockIdx.y * 16, bj = (int64_t)blockIdx.x * 16;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    if (bi + ty < n && bj + tx < n) tile[ty][tx] = A[(bi + ty) * n + bj + tx];
+    __syncthreads();
+    if (bj + ty < n && bi + tx < n) B[(bj + ty) * n + bi + tx] = tile[tx][ty];
+}
+
+// D[i][j] = D[j][i] = sqrt(sum_k (A[i][k] - A[j][k])^2) for i < j, summed in
+// k order (one thread per pair, 16 x 16 pairs per block, k in LDS chunks of
+// 32), D[i][i] = 0.  Only blocks on or above the diagonal work.
+__global__ __launch_bounds__(kBlock) void k_pairwise(const float* __restrict__ A,
+                                                     float* __restrict__ D, int64_t n) {
+    constexpr int T = 16, KC = 32;
+    __shared__ float ai[T][KC + 1], aj[T][KC + 1];
+    const int64_t bi = (int64_t)blockIdx.y * T, bj = (int64_t)blockIdx.x * T;
+    if (bj + T <= bi) return;  // wholly below the diagonal
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int64_t i = bi + ty, j = bj + tx;
+    float s = 0.0f;
+    for (int64_t k0 = 0; k0 < n; k0 += KC) {
+        for (int e = threadIdx.x; e < T * KC; e += kBlock) {
+            const int r = e / KC, c = e % KC;
+            const int64_t k = k0 + c;
+            ai[r][c] = (bi + r < n && k < n) ? A[(bi + r) * n + k] : 0.0f;
+            aj[r][c] = (bj + r < n && k < n) ? A[(bj + r) * n + k] : 0.0f;
+        }
+        __syncthreads();
+        const int kmax = (int)(n - k0 < KC ? n - k0 : KC);
+        for (int c = 0; c < kmax; ++c) {
+            const float d = ai[ty][c] - aj[tx][c];
+            s = __builtin_fmaf(d, d, s);
+        }
+        __syncthreads();
+    }
+    if (i < n && j < n) {
+        if (i == j) {
+            D[i * n + i] = 0.0f;
+        } else if (i < j) {
+            const float dist = sqrtf(s);
+            D[i * n + j] = dist;
+            D[j * n + i] = dist;
+        }
+    }
+}
+
+unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+unsigned parts_for(int64_t n) {
+    const int64_t b = (n + kBlock - 1) / kBlock;
+    return (unsigned)(b < kParts ? (b > 0 ? b : 1) : kParts);
+}
+
+}  // namespace
+
+void launch_blend(float* x, const float* t, int64_t n, float a, float b, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_blend, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, t, n, a, b);
+    check_launch("blend launch");
+}
+
+void launch_rwr_update(float* y, const float* x, const float* r, int64_t n, float alpha,
+                       float beta, float* part, float* total, hipStream_t s) {
+    const unsigned p = parts_for(n);
+    hipLaunchKernelGGL(k_rwr_update, dim3(p), dim3(kBlock), 0, s, y, x, r, n, alpha, beta,
+                       part);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, 0, total);
+    check_launch("rwr update launch");
+}
+
+void launch_dpt_update(const float* pn, float* vis, float* hit, int64_t n, float it,
+                       float* part, float* max_unvisited, hipStream_t s) {
+    const unsigned p = parts_for(n);
+    hipLaunchKernelGGL(k_dpt_update, dim3(p), dim3(kBlock), 0, s, pn, vis, hit, n, it, part);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, 1, max_unvisited);
+    check_launch("dpt update launch");
+}
+
+void launch_dpt_finish(const float* vis, float* hit, float* pt, int64_t n, float* part,
+                       float* max_time, hipStream_t s) {
+    const unsigned p = parts_for(n);
+    hipLaunchKernelGGL(k_dpt_normalise, dim3(p), dim3(kBlock), 0, s, vis, hit, n, part);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, 1, max_time);
+    hipLaunchKernelGGL(k_dpt_finish, dim3(blocks_for(n)), dim3(kBlock), 0, s, vis, hit,
+                       max_time, pt, n);
+    check_launch("dpt finish launch");
+}
+
+void launch_set_unit(float* x, int64_t n, int64_t at, float value, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_set_unit, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, n, at, value);
+    check_launch("fill launch");
+}
+
+void launch_add(float* acc, const float* x, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add, dim3(blocks_for(n)), dim3(kBlock), 0, s, acc, x, n);
+    check_launch("add launch");
+}
+
+void launch_scale(float* x, int64_t n, float a, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, n, a);
+    check_launch("scale launch");
+}
+
+void launch_dmap_init(float* Q, int64_t n, int64_t K, hipStream_t s) {
+    if (n * K <= 0) return;
+    hipLaunchKernelGGL(k_dmap_init, dim3(blocks_for(n * K)), dim3(kBlock), 0, s, Q, n, K);
+    check_launch("embedding init launch");
+}
+
+void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s) {
+    if (n <= 0 || K <= 0) return;
+    hipLaunchKernelGGL(k_mgs, dim3(1), dim3(kBlock), 0, s, Q, w, n, K);
+    check_launch("gram-schmidt launch");
+}
+
+void launch_identity(float* A, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_identity, dim3(blocks_for(n * n)), dim3(kBlock), 0, s, A, n);
+    check_launch("identity launch");
+}
+
+void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)((n + 15) / 16);
+    hipLaunchKernelGGL(k_transpose, dim3(g, g), dim3(kBlock), 0, s, A, B, n);
+    check_launch("transpose launch");
+}
+
+void launch_pairwise_rows(const float* A, float* D, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const unsigned g = (unsigned)((n + 15) / 16);
+    hipLaunchKernelGGL(k_pairwise, dim3(g, g), dim3(kBlock), 0, s, A, D, n);
+    check_launch("pairwise distance launch");
+}
+
+}  // namespace scl_amd

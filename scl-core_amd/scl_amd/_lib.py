@@ -75,6 +75,16 @@ SIGNATURES = {
     "scl_diffusion_diffuse_vector": (c_err, [P, P, c_size, c_index]),
     "scl_diffusion_diffuse_matrix": (c_err, [P, P, c_index, c_index, c_index]),
     "scl_diffusion_magic_impute": (c_err, [P, P, c_index, c_index, c_index]),
+    "scl_diffusion_compute_dpt": (c_err, [P, c_index, P, c_size, c_index, c_real]),
+    "scl_diffusion_compute_dpt_multi_root": (c_err, [P, P, c_size, P, c_size, c_index]),
+    "scl_diffusion_random_walk_with_restart": (
+        c_err, [P, P, c_size, P, c_size, c_real, c_index, c_real]),
+    "scl_diffusion_personalized_pagerank": (
+        c_err, [P, c_index, P, c_size, c_real, c_index, c_real]),
+    "scl_diffusion_diffusion_map_embedding": (c_err, [P, P, c_index, c_index, c_index]),
+    "scl_diffusion_heat_kernel_signature": (c_err, [P, P, c_size, c_real, c_index]),
+    "scl_diffusion_diffusion_distance": (c_err, [P, P, c_size, c_index]),
+    "scl_diffusion_lazy_random_walk": (c_err, [P, P, c_size, c_index, c_real]),
     # scl_diffusion_gpu.h
     "scl_gpu_device_count": (c_err, [ctypes.POINTER(ctypes.c_int)]),
     "scl_gpu_sparse_invalidate": (c_err, [P]),

@@ -72,3 +72,92 @@ def diffuse_vector(transition: SparseMatrixBase, x: np.ndarray, n_steps: int
     _lib.check(_lib.load().scl_diffusion_diffuse_vector(
         transition.handle, x.ctypes.data, x.shape[0], n_steps))
     return x
+
+
+# ------------------------------------------------ vector-side consumers --
+# Host-array wrappers of the scl_diffusion_* walks (0 selects the reference's
+# defaults, as at the C ABI).
+
+def _n(transition: SparseMatrixBase) -> int:
+    return transition.shape[0]
+
+
+def _idx(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def compute_dpt(transition: SparseMatrixBase, root_cell: int, max_iter: int = 0,
+                tol: float = 0.0) -> np.ndarray:
+    """Diffusion pseudotime from one root (scl_diffusion_compute_dpt)."""
+    out = np.empty(_n(transition), np.float32)
+    _lib.check(_lib.load().scl_diffusion_compute_dpt(
+        transition.handle, root_cell, out.ctypes.data, len(out), max_iter, tol))
+    return out
+
+
+def compute_dpt_multi_root(transition: SparseMatrixBase, root_cells, max_iter: int = 0
+                           ) -> np.ndarray:
+    """Mean pseudotime over several roots (scl_diffusion_compute_dpt_multi_root)."""
+    roots = _idx(root_cells)
+    out = np.empty(_n(transition), np.float32)
+    _lib.check(_lib.load().scl_diffusion_compute_dpt_multi_root(
+        transition.handle, roots.ctypes.data, len(roots), out.ctypes.data, len(out),
+        max_iter))
+    return out
+
+
+def random_walk_with_restart(transition: SparseMatrixBase, seed_nodes, alpha: float = 0.0,
+                             max_iter: int = 0, tol: float = 0.0) -> np.ndarray:
+    """scl_diffusion_random_walk_with_restart."""
+    seeds = _idx(seed_nodes)
+    out = np.empty(_n(transition), np.float32)
+    _lib.check(_lib.load().scl_diffusion_random_walk_with_restart(
+        transition.handle, seeds.ctypes.data, len(seeds), out.ctypes.data, len(out), alpha,
+        max_iter, tol))
+    return out
+
+
+def personalized_pagerank(transition: SparseMatrixBase, seed_node: int, alpha: float = 0.0,
+                          max_iter: int = 0, tol: float = 0.0) -> np.ndarray:
+    """scl_diffusion_personalized_pagerank."""
+    out = np.empty(_n(transition), np.float32)
+    _lib.check(_lib.load().scl_diffusion_personalized_pagerank(
+        transition.handle, seed_node, out.ctypes.data, len(out), alpha, max_iter, tol))
+    return out
+
+
+def diffusion_map_embedding(transition: SparseMatrixBase, n_components: int,
+                            n_iter: int = 0) -> np.ndarray:
+    """n x n_components embedding (scl_diffusion_diffusion_map_embedding)."""
+    n = _n(transition)
+    out = np.empty((n, n_components), np.float32)
+    _lib.check(_lib.load().scl_diffusion_diffusion_map_embedding(
+        transition.handle, out.ctypes.data, n, n_components, n_iter))
+    return out
+
+
+def heat_kernel_signature(transition: SparseMatrixBase, t: float = 1.0, n_steps: int = 0
+                          ) -> np.ndarray:
+    """scl_diffusion_heat_kernel_signature."""
+    out = np.empty(_n(transition), np.float32)
+    _lib.check(_lib.load().scl_diffusion_heat_kernel_signature(
+        transition.handle, out.ctypes.data, len(out), t, n_steps))
+    return out
+
+
+def diffusion_distance(transition: SparseMatrixBase, n_steps: int = 3) -> np.ndarray:
+    """n x n diffusion distances (scl_diffusion_diffusion_distance)."""
+    n = _n(transition)
+    out = np.empty((n, n), np.float32)
+    _lib.check(_lib.load().scl_diffusion_diffusion_distance(
+        transition.handle, out.ctypes.data, n, n_steps))
+    return out
+
+
+def lazy_random_walk(transition: SparseMatrixBase, x: np.ndarray, n_steps: int,
+                     laziness: float = 0.0) -> np.ndarray:
+    """x <- laziness x + (1 - laziness) P x, n_steps times, in place."""
+    x = _f32_inplace(x)
+    _lib.check(_lib.load().scl_diffusion_lazy_random_walk(
+        transition.handle, x.ctypes.data, x.shape[0], n_steps, laziness))
+    return x
