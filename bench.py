@@ -30,6 +30,9 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
       graphs one after another with plain (non-graph) launches
   mp  a skewed pattern (Pareto row lengths, hub rows of 200k nonzeros),
       n=1M, F=64: merge-path SpMM vs the autotuned row-split SpMM
+  wk  the vector-side entry points (lazy walk, heat kernel, RWR, DPT,
+      diffusion map, diffusion distance) through the C ABI vs the CPU
+      restatement
 """
 from __future__ import annotations
 
@@ -256,6 +259,60 @@ def leg_mp(steps: int) -> dict:
     return out
 
 
+def leg_wk(steps: int, with_cpu: bool = True) -> dict:
+    """The vector-side diffusion entry points (host arrays in and out, as at
+    the C ABI) on knn30_200k; with the CPU baseline on, the CPU restatement
+    of each is timed beside it on this host."""
+    from scl_amd import CsrMatrix, diffusion, synth
+    if with_cpu:
+        from oracle import oracle as orc
+    n = 200_000
+    g = synth.knn_graph(n, 30, seed=7).normalized()
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    P.prepare_device()
+    x = synth.normal(n, 1, seed=7)[:, 0]
+    ip, ix, v = g.indptr, g.indices, g.data
+    if not with_cpu:
+        orc = None
+    cases = {
+        "lazy_walk_t20": (lambda: diffusion.lazy_random_walk(P, x.copy(), 20, 0.5),
+                          lambda: orc.lazy_walk_f32(ip, ix, v, x, 20, 0.5)),
+        "heat_kernel_10": (lambda: diffusion.heat_kernel_signature(P, 1.0, 10),
+                           lambda: orc.heat_kernel_f32(ip, ix, v, 1.0, 10)),
+        "rwr_100": (lambda: diffusion.random_walk_with_restart(P, [0, 50, 900], 0.85, 100, 1e-30),
+                    lambda: orc.rwr_f32(ip, ix, v, [0, 50, 900], 0.85, 100, -1.0)),
+        "dpt_100": (lambda: diffusion.compute_dpt(P, 0, 100, 1e-30),
+                    lambda: orc.dpt_f32(ip, ix, v, 0, 100, 1e-30)),
+        "diffusion_map_k10_i20": (lambda: diffusion.diffusion_map_embedding(P, 10, 20),
+                                  lambda: orc.dmap_f32(ip, ix, v, 10, 20)),
+    }
+    out = {"workload": "knn30_200k walks (host arrays in/out)"}
+    if with_cpu:
+        out["cpu_threads"] = orc.num_threads()
+
+    def record(name, gpu_fn, cpu_fn, reps):
+        gpu_fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gpu_fn()
+        g_ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"gpu_ms": round(g_ms, 2)}
+        if with_cpu:
+            t0 = time.perf_counter()
+            cpu_fn()
+            c_ms = (time.perf_counter() - t0) * 1e3
+            out[name].update({"cpu_ms": round(c_ms, 1), "speedup": round(c_ms / g_ms, 1)})
+
+    for name, (gpu_fn, cpu_fn) in cases.items():
+        record(name, gpu_fn, cpu_fn, max(1, steps // 5))
+    # dense output: diffusion distance on a 4000-node graph
+    gs = synth.knn_graph(4000, 15, seed=7).normalized()
+    Ps = CsrMatrix.from_arrays(gs.indptr, gs.indices, gs.data, (4000, 4000))
+    record("distance_n4000_t3", lambda: diffusion.diffusion_distance(Ps, 3),
+           lambda: orc.diffusion_distance_f32(gs.indptr, gs.indices, gs.data, 3), 1)
+    return out
+
+
 def leg_c5(steps: int) -> dict:
     """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
     import numpy as np
@@ -338,7 +395,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="c2,c3,c4,c5,mp")
+    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="vec,group,unroll,flags for the headline SpMM instead of the "
@@ -431,12 +488,16 @@ def main() -> int:
     legs = {}
     del Xd, Yd, Wd, A
     torch.cuda.empty_cache()
-    for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5), ("mp", leg_mp)):
+    for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5), ("mp", leg_mp),
+                     ("wk", leg_wk)):
         if name not in args.legs.split(","):
             continue
         log(f"leg {name}")
         try:
-            legs[name] = fn(max(3, args.steps // 2))
+            if name == "wk":
+                legs[name] = fn(max(3, args.steps // 2), not args.no_cpu_baseline)
+            else:
+                legs[name] = fn(max(3, args.steps // 2))
         except Exception as e:  # a failed leg is reported, not fatal
             legs[name] = {"error": f"{type(e).__name__}: {e}"}
         torch.cuda.empty_cache()

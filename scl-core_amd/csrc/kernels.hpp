@@ -194,6 +194,13 @@ void launch_scale(float* x, int64_t n, float a, hipStream_t s);
 // Diffusion-map start vectors and modified Gram-Schmidt (one workgroup).
 void launch_dmap_init(float* Q, int64_t n, int64_t K, hipStream_t s);
 void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s);
+// The same over the whole GPU on a column-major K x n matrix (columns
+// contiguous); part: 2 * 1024 floats, S: 6 floats of device scratch.
+void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
+                         hipStream_t s);
+// B (cols x rows) = A (rows x cols)^T, both row-major.
+void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
+                           hipStream_t s);
 // A = I; B = A^T; pairwise row distances D of an n x n row-major matrix.
 void launch_identity(float* A, int64_t n, hipStream_t s);
 void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s);

@@ -41,6 +41,8 @@ constexpr int64_t kDefaultMaxIter = 100;
 constexpr int64_t kDefaultNIter = 50;
 constexpr int64_t kDefaultHksSteps = 10;
 constexpr float kDefaultLaziness = 0.5f;
+// rows from which the diffusion-map basis is orthogonalised across the GPU
+constexpr int64_t kWideGramSchmidtRows = 16384;
 
 // Device vector freed at scope exit.
 class DevBuf {
@@ -341,13 +343,27 @@ SCL_EXPORT scl_error_t scl_diffusion_diffusion_map_embedding(scl_sparse_t transi
                 fail(SCL_ERROR_INVALID_ARGUMENT, "n_components too large");
             const int64_t iters = n_iter == 0 ? kDefaultNIter : n_iter;
             DevBuf Q(n * K), Qn(n * K), w(n);
+            // one workgroup orthogonalises small bases; from kWideGramSchmidtRows
+            // rows the columns are transposed to contiguous vectors and every
+            // dot product spreads over the GPU
+            const bool wide = n >= kWideGramSchmidtRows;
+            DevBuf C(wide ? n * K : 0), part(wide ? 2 * 1024 : 0), sc(wide ? 8 : 0);
+            auto orthogonalise = [&](float* M) {
+                if (!wide) {
+                    launch_mgs(M, w.get(), n, K, S.s);
+                    return;
+                }
+                launch_transpose_rect(M, C.get(), n, K, S.s);
+                launch_gram_schmidt(C.get(), w.get(), n, K, part.get(), sc.get(), S.s);
+                launch_transpose_rect(C.get(), M, K, n, S.s);
+            };
             launch_dmap_init(Q.get(), n, K, S.s);
-            launch_mgs(Q.get(), w.get(), n, K, S.s);
+            orthogonalise(Q.get());
             float* cur = Q.get();
             float* nxt = Qn.get();
             for (int64_t it = 0; it < iters; ++it) {
                 S.spmm(cur, nxt, K);
-                launch_mgs(nxt, w.get(), n, K, S.s);
+                orthogonalise(nxt);
                 std::swap(cur, nxt);
             }
             S.download(embedding, cur, n * K);

@@ -230,6 +230,117 @@ __global__ __launch_bounds__(kBlock) void k_identity(float* __restrict__ A, int6
     if (e < n * n) A[e] = (e / n == e % n) ? 1.0f : 0.0f;
 }
 
+// --- multi-workgroup Gram-Schmidt over a column-major K x n matrix C ------
+// (columns contiguous).  Every global sum is two launches: per-block partials
+// then one fold, so a pass over the columns needs no host synchronisation.
+// The fold kernels keep the scalars the next kernels read in S:
+//   S[0] coeff of the current projection      S[1] 1 if it applies
+//   S[2] running max |<w, c_p>| / |w|        S[3] 1 if a second pass runs
+//   S[4] |w|^2 of the current column          S[5] 1 / |w| (0: degenerate)
+
+// partial sums of <w, c_p> and <c_p, c_p> (a = w, b = c_p), or of <w, w>
+__global__ __launch_bounds__(kBlock) void k_gs_dots(const float* __restrict__ a,
+                                                    const float* __restrict__ b, int64_t n,
+                                                    float* __restrict__ part,
+                                                    const float* __restrict__ gate) {
+    if (gate && *gate == 0.0f) return;
+    float d = 0.0f, q = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock) {
+        const float bi = b[i];
+        d = __builtin_fmaf(a[i], bi, d);
+        q = __builtin_fmaf(bi, bi, q);
+    }
+    d = block_reduce(d, false);
+    q = block_reduce(q, false);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = d;
+        part[2 * blockIdx.x + 1] = q;
+    }
+}
+
+// mode 0: projection coefficient of c_p (S[0], S[1])
+// mode 1: cos = |dot| / sqrt(S[4]) folded into the running max S[2]
+// mode 2: |w|^2 into S[4] (dot of w with itself: the first partial)
+// mode 3: normalisation factor S[5] from |w|^2
+// mode 4: S[3] = (S[2] >= 0.7): whether the second pass runs
+__global__ __launch_bounds__(kBlock) void k_gs_fold(const float* __restrict__ part, int m,
+                                                    int mode, float* __restrict__ S,
+                                                    const float* __restrict__ gate) {
+    if (gate && *gate == 0.0f) return;
+    float d = 0.0f, q = 0.0f;
+    if (mode != 4) {
+        for (int i = threadIdx.x; i < m; i += kBlock) {
+            d += part[2 * i];
+            q += part[2 * i + 1];
+        }
+        d = block_reduce(d, false);
+        q = block_reduce(q, false);
+    }
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        const bool ok = q > 1e-15f;
+        S[0] = ok ? d / q : 0.0f;
+        S[1] = ok ? 1.0f : 0.0f;
+    } else if (mode == 1) {
+        const float nv = S[4];
+        const float cs = nv > 1e-15f ? fabsf(d) / sqrtf(nv) : 0.0f;
+        S[2] = fmaxf(S[2], cs);
+    } else if (mode == 2) {
+        S[4] = d;
+    } else if (mode == 3) {
+        S[5] = d > 1e-15f ? 1.0f / sqrtf(d) : 0.0f;
+    } else {
+        S[3] = S[2] >= 0.7f ? 1.0f : 0.0f;
+    }
+}
+
+// w -= S[0] * c_p when S[1] says so (and the gate is open)
+__global__ __launch_bounds__(kBlock) void k_gs_axpy(float* __restrict__ w,
+                                                    const float* __restrict__ c, int64_t n,
+                                                    const float* __restrict__ S,
+                                                    const float* __restrict__ gate) {
+    if ((gate && *gate == 0.0f) || S[1] == 0.0f) return;
+    const float coeff = S[0];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) w[i] = __builtin_fmaf(-coeff, c[i], w[i]);
+}
+
+// column v = w * S[5], or e_(v mod n) when the column is degenerate
+__global__ __launch_bounds__(kBlock) void k_gs_store(float* __restrict__ c,
+                                                     const float* __restrict__ w, int64_t n,
+                                                     int64_t v, const float* __restrict__ S) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float inv = S[5];
+    c[i] = inv != 0.0f ? w[i] * inv : (i == v % n ? 1.0f : 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gs_copy(float* __restrict__ w,
+                                                    const float* __restrict__ c, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) w[i] = c[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gs_reset(float* __restrict__ S, float v2) {
+    if (threadIdx.x == 0) {
+        S[2] = 0.0f;
+        S[3] = v2;
+    }
+}
+
+// B (K x n) = A (n x K)^T and back, through LDS tiles of 16 x 16
+__global__ __launch_bounds__(kBlock) void k_transpose_rect(const float* __restrict__ A,
+                                                           float* __restrict__ B,
+                                                           int64_t rows, int64_t cols) {
+    __shared__ float tile[16][17];
+    const int64_t bi = (int64_t)blockIdx.y * 16, bj = (int64_t)blockIdx.x * 16;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    if (bi + ty < rows && bj + tx < cols) tile[ty][tx] = A[(bi + ty) * cols + bj + tx];
+    __syncthreads();
+    if (bj + ty < cols && bi + tx < rows) B[(bj + ty) * rows + bi + tx] = tile[tx][ty];
+}
+
 // B = A^T for n x n row-major A (16 x 16 tiles through LDS).
 __global__ __launch_bounds__(kBlock) void k_transpose(const float* __restrict__ A,
                                                       float* __restrict__ B, int64_t n) {
@@ -354,6 +465,59 @@ void launch_identity(float* A, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_identity, dim3(blocks_for(n * n)), dim3(kBlock), 0, s, A, n);
     check_launch("identity launch");
+}
+
+void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
+                           hipStream_t s) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(k_transpose_rect, dim3((unsigned)((cols + 15) / 16),
+                                              (unsigned)((rows + 15) / 16)),
+                       dim3(kBlock), 0, s, A, B, rows, cols);
+    check_launch("transpose launch");
+}
+
+// Modified Gram-Schmidt (diffusion.hpp:370-446) of the K contiguous columns of
+// C (K x n), across the whole GPU: w is an n-float workspace, part holds
+// 2 * kParts partials, S the 6 scalars above.
+void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
+                         hipStream_t s) {
+    if (n <= 0 || K <= 0) return;
+    const unsigned p = parts_for(n), b = blocks_for(n);
+    auto dots = [&](const float* a, const float* c, const float* gate) {
+        hipLaunchKernelGGL(k_gs_dots, dim3(p), dim3(kBlock), 0, s, a, c, n, part, gate);
+    };
+    auto fold = [&](int mode, const float* gate) {
+        hipLaunchKernelGGL(k_gs_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, mode, S, gate);
+    };
+    for (int64_t v = 0; v < K; ++v) {
+        float* cv = C + v * n;
+        hipLaunchKernelGGL(k_gs_reset, dim3(1), dim3(64), 0, s, S, 1.0f);
+        hipLaunchKernelGGL(k_gs_copy, dim3(b), dim3(kBlock), 0, s, w, cv, n);
+        for (int pass = 0; pass < 2 && (pass == 0 || v > 0); ++pass) {
+            // the second pass runs only when the first left some
+            // |<w, c_p>| / |w| >= 0.7 (gate S[3])
+            const float* gate = pass == 0 ? nullptr : S + 3;
+            for (int64_t q = 0; q < v; ++q) {
+                dots(w, C + q * n, gate);
+                fold(0, gate);
+                hipLaunchKernelGGL(k_gs_axpy, dim3(b), dim3(kBlock), 0, s, w, C + q * n, n, S,
+                                   gate);
+            }
+            if (pass == 0 && v > 0) {
+                dots(w, w, nullptr);
+                fold(2, nullptr);
+                for (int64_t q = 0; q < v; ++q) {
+                    dots(w, C + q * n, nullptr);
+                    fold(1, nullptr);
+                }
+                fold(4, nullptr);
+            }
+        }
+        dots(w, w, nullptr);
+        fold(3, nullptr);
+        hipLaunchKernelGGL(k_gs_store, dim3(b), dim3(kBlock), 0, s, cv, w, n, v, S);
+    }
+    check_launch("gram-schmidt launch");
 }
 
 void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s) {

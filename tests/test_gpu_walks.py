@@ -85,6 +85,18 @@ def test_diffusion_map_embedding(gpu):
     assert diffusion.diffusion_map_embedding(P, 5).shape == (n, 5)
 
 
+def test_diffusion_map_embedding_wide(gpu):
+    # from 16384 rows the basis is orthogonalised by grid-wide reductions
+    # (transposed columns); sums fold in another order than the restatement's
+    g, P = op(30_000, 10, seed=9)
+    for K, iters in ((6, 15), (1, 3)):
+        got = diffusion.diffusion_map_embedding(P, K, iters)
+        ref = orc.dmap_f32(g.indptr, g.indices, g.data, K, iters)
+        assert np.abs(got - ref).max() < 2e-4, K
+        G = got.T.astype(np.float64) @ got
+        assert np.abs(G - np.eye(K)).max() < 1e-5
+
+
 @pytest.mark.parametrize("n", [300, 1100])
 def test_diffusion_distance(gpu, n):
     g, P = op(n, 6, seed=n, sym=False)
