@@ -329,12 +329,15 @@ __global__ __launch_bounds__(kBlock) void k_gs_reset(float* __restrict__ S, floa
     }
 }
 
-// B (K x n) = A (n x K)^T and back, through LDS tiles of 16 x 16
+// B (K x n) = A (n x K)^T and back, through LDS tiles of 16 x 16; one
+// workgroup per tile on a 1-D grid (tile rows outer)
 __global__ __launch_bounds__(kBlock) void k_transpose_rect(const float* __restrict__ A,
                                                            float* __restrict__ B,
                                                            int64_t rows, int64_t cols) {
     __shared__ float tile[16][17];
-    const int64_t bi = (int64_t)blockIdx.y * 16, bj = (int64_t)blockIdx.x * 16;
+    const int64_t tiles_c = (cols + 15) / 16;
+    const int64_t bi = ((int64_t)blockIdx.x / tiles_c) * 16;
+    const int64_t bj = ((int64_t)blockIdx.x % tiles_c) * 16;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     if (bi + ty < rows && bj + tx < cols) tile[ty][tx] = A[(bi + ty) * cols + bj + tx];
     __syncthreads();
@@ -470,9 +473,10 @@ void launch_identity(float* A, int64_t n, hipStream_t s) {
 void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
                            hipStream_t s) {
     if (rows <= 0 || cols <= 0) return;
-    hipLaunchKernelGGL(k_transpose_rect, dim3((unsigned)((cols + 15) / 16),
-                                              (unsigned)((rows + 15) / 16)),
-                       dim3(kBlock), 0, s, A, B, rows, cols);
+    const int64_t tiles = ((rows + 15) / 16) * ((cols + 15) / 16);
+    if (tiles > 0x7fffffff) throw std::runtime_error("transpose too large");
+    hipLaunchKernelGGL(k_transpose_rect, dim3((unsigned)tiles), dim3(kBlock), 0, s, A, B,
+                       rows, cols);
     check_launch("transpose launch");
 }
 
