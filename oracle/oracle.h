@@ -160,6 +160,31 @@ int orc_mgs_f32(int64_t n, int64_t K, float* Q);
 int orc_diffusion_distance_f32(int64_t n, const int64_t* ip, const int64_t* ix,
                                const float* v, float* D, int64_t n_steps);
 
+/* ---- other callers of the SpMM (callers.c) ---- */
+/* Dense copy of a sparse cells x genes matrix (impute.hpp:612-640, 683-697). */
+void orc_densify_f32(int64_t n_cells, int64_t n_genes, const int64_t* ip,
+                     const int64_t* ix, const float* v, int is_csr, float* out);
+/* diffusion imputation steps (impute.hpp:599-658) on the dense X (n x G). */
+int orc_impute_diffusion_f32(int64_t n, int64_t G, int64_t n_steps, const int64_t* ip,
+                             const int64_t* ix, const float* v, float* X);
+/* MAGIC steps with row-sum normalisation (impute.hpp:669-741, 355-389). */
+int orc_magic_f32(int64_t n, int64_t G, int64_t t, const int64_t* ip, const int64_t* ix,
+                  const float* v, float* X);
+/* FastRNG of pseudotime.hpp:153-182: seeding (the four 64-bit z and the
+ * state), raw steps from a state, and uniform() draws from a seed. */
+void orc_fastrng_seed(uint64_t seed, uint64_t* z4, uint32_t* state4);
+void orc_fastrng_step(uint32_t* state4, int64_t m, uint32_t* out);
+void orc_fastrng_uniform(uint64_t seed, int64_t m, float* out);
+/* orthonormalize_columns (pseudotime.hpp:418-474) of row-major n x K Q. */
+int orc_pt_orthonormalise_f32(int64_t n, int64_t K, float* Q);
+/* diffusion_pseudotime (pseudotime.hpp:646-712). */
+int orc_pseudotime_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                       int64_t root, int64_t K, int64_t n_iter, float* pt);
+/* algebra::spmm, Y = alpha A X + beta Y (algebra.hpp:546-609); A has m rows. */
+void orc_algebra_spmm_f32(int64_t m, int64_t F, const int64_t* ip, const int64_t* ix,
+                          const float* v, const float* X, float* Y, float alpha,
+                          float beta);
+
 #ifdef __cplusplus
 }
 #endif

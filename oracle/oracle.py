@@ -77,6 +77,16 @@ def load():
         "orc_dmap_f32": (ctypes.c_int, [I64, P, P, P, P, I64, I64]),
         "orc_mgs_f32": (ctypes.c_int, [I64, I64, P]),
         "orc_diffusion_distance_f32": (ctypes.c_int, [I64, P, P, P, P, I64]),
+        "orc_densify_f32": (None, [I64, I64, P, P, P, ctypes.c_int, P]),
+        "orc_impute_diffusion_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, P]),
+        "orc_magic_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, P]),
+        "orc_fastrng_seed": (None, [ctypes.c_uint64, P, P]),
+        "orc_fastrng_step": (None, [P, I64, P]),
+        "orc_fastrng_uniform": (None, [ctypes.c_uint64, I64, P]),
+        "orc_pt_orthonormalise_f32": (ctypes.c_int, [I64, I64, P]),
+        "orc_pseudotime_f32": (ctypes.c_int, [I64, P, P, P, I64, I64, I64, P]),
+        "orc_algebra_spmm_f32": (None, [I64, I64, P, P, P, P, P, ctypes.c_float,
+                                        ctypes.c_float]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -345,3 +355,83 @@ def diffusion_distance_f32(indptr, indices, vals, n_steps):
                                          D.ctypes.data, n_steps):
         raise MemoryError
     return D
+
+
+# ------------------------------------------------ other callers of the SpMM --
+
+def densify_f32(indptr, indices, vals, n_cells, n_genes, is_csr=True):
+    """Dense cells x genes copy of a sparse matrix whose primary slots are
+    cells (CSR) or genes (CSC)."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    out = np.empty((n_cells, n_genes), np.float32)
+    load().orc_densify_f32(n_cells, n_genes, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                           int(bool(is_csr)), out.ctypes.data)
+    return out
+
+
+def impute_diffusion_f32(indptr, indices, vals, X, n_steps):
+    ip, ix, v = _csr(indptr, indices, vals)
+    X = _f32(X).copy()
+    if load().orc_impute_diffusion_f32(X.shape[0], X.shape[1], n_steps, ip.ctypes.data,
+                                       ix.ctypes.data, v.ctypes.data, X.ctypes.data):
+        raise MemoryError
+    return X
+
+
+def magic_f32(indptr, indices, vals, X, t):
+    ip, ix, v = _csr(indptr, indices, vals)
+    X = _f32(X).copy()
+    if load().orc_magic_f32(X.shape[0], X.shape[1], t, ip.ctypes.data, ix.ctypes.data,
+                            v.ctypes.data, X.ctypes.data):
+        raise MemoryError
+    return X
+
+
+def fastrng_seed(seed):
+    """(the four 64-bit seeding values z, the 4-word state)."""
+    z = np.empty(4, np.uint64)
+    st = np.empty(4, np.uint32)
+    load().orc_fastrng_seed(seed, z.ctypes.data, st.ctypes.data)
+    return z, st
+
+
+def fastrng_step(state, m):
+    """m raw outputs from a 4-word state; returns (outputs, new state)."""
+    st = np.ascontiguousarray(state, dtype=np.uint32).copy()
+    out = np.empty(m, np.uint32)
+    load().orc_fastrng_step(st.ctypes.data, m, out.ctypes.data)
+    return out, st
+
+
+def fastrng_uniform(seed, m):
+    out = np.empty(m, np.float32)
+    load().orc_fastrng_uniform(seed, m, out.ctypes.data)
+    return out
+
+
+def pt_orthonormalise_f32(Q):
+    Q = _f32(Q).copy()
+    if load().orc_pt_orthonormalise_f32(Q.shape[0], Q.shape[1], Q.ctypes.data):
+        raise MemoryError
+    return Q
+
+
+def pseudotime_f32(indptr, indices, vals, root, n_dcs, n_iter=100):
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    out = np.empty(n, np.float32)
+    if load().orc_pseudotime_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, root,
+                                 n_dcs, n_iter, out.ctypes.data):
+        raise MemoryError
+    return out
+
+
+def algebra_spmm_f32(indptr, indices, vals, X, Y, alpha, beta):
+    """Y = alpha A X + beta Y for A with len(indptr) - 1 rows; returns a new Y."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    X = _f32(X)
+    Y = _f32(Y).copy()
+    F = Y.shape[1] if Y.ndim == 2 else 1
+    load().orc_algebra_spmm_f32(len(ip) - 1, F, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                X.ctypes.data, Y.ctypes.data, alpha, beta)
+    return Y

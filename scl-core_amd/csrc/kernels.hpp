@@ -196,8 +196,10 @@ void launch_dmap_init(float* Q, int64_t n, int64_t K, hipStream_t s);
 void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s);
 // The same over the whole GPU on a column-major K x n matrix (columns
 // contiguous); part: 2 * 1024 floats, S: 6 floats of device scratch.
+// reorth: second pass and unit-vector fallback (diffusion maps); otherwise
+// one pass and FastRNG(v * rng_mult) directions (pseudotime).
 void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
-                         hipStream_t s);
+                         bool reorth, uint64_t rng_mult, hipStream_t s);
 // B (cols x rows) = A (rows x cols)^T, both row-major.
 void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
                            hipStream_t s);
@@ -205,5 +207,30 @@ void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
 void launch_identity(float* A, int64_t n, hipStream_t s);
 void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s);
 void launch_pairwise_rows(const float* A, float* D, int64_t n, hipStream_t s);
+
+// ---- other callers of the SpMM (callers.hip) ----
+// flag = 1 when some primary slot of the int64 pattern is not strictly
+// increasing; false when there is nothing to check (flag untouched).
+bool launch_find_unsorted(const int64_t* ip, const int64_t* ix, int64_t slots, int* flag,
+                          hipStream_t s);
+// Scatters the stored entries of `slots` primary slots into the zeroed dense
+// n_cells x G matrix (cells-major input when is_csr, genes-major otherwise);
+// `ordered` walks each slot in stored order on one lane.
+void launch_densify(const int64_t* ip, const int64_t* ix, const float* v, int64_t slots,
+                    int64_t n_cells, int64_t G, bool is_csr, bool ordered, float* out,
+                    hipStream_t s);
+// MAGIC weights a_ij / (FP32 row sum); rows summing below 1e-12 get zero
+// weights and are appended to copy[] (count in *n_copy, cleared by the caller).
+void launch_magic_weights(const int32_t* row_ptr, const float* val, int32_t n, float* w,
+                          int32_t* copy, int* n_copy, hipStream_t s);
+void launch_copy_rows(const int32_t* rows, int32_t m, const float* in, float* out, int64_t G,
+                      hipStream_t s);
+// Pseudotime distances to the root row of the n x K components, divided by
+// their maximum when it is positive; part: 1024 floats.
+void launch_pt_distance(const float* Q, int64_t n, int64_t K, int64_t root, float* pt,
+                        float* part, hipStream_t s);
+// Y = alpha A X + beta Y with zero weights skipped (one wave per row).
+void launch_spmm_axpby(const CsrView& A, const float* X, float* Y, int64_t F, float alpha,
+                       float beta, hipStream_t s);
 
 }  // namespace scl_amd

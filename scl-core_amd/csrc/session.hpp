@@ -1,0 +1,99 @@
+// session.hpp -- device buffers and the per-call operator session shared by
+// the entry points that keep whole iterations on the GPU (walks.cpp,
+// callers.cpp).
+#pragma once
+
+#include "internal.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+namespace scl_amd {
+
+// Device buffer of `count` elements of `elem` bytes, freed at scope exit.
+class DevBuf {
+public:
+    explicit DevBuf(size_t count, size_t elem = sizeof(float)) {
+        if (hipMalloc(&p_, std::max<size_t>(count, 1) * elem) != hipSuccess) {
+            (void)hipGetLastError();
+            fail(SCL_ERROR_OUT_OF_MEMORY, "device allocation failed");
+        }
+    }
+    ~DevBuf() { (void)hipFree(p_); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    float* get() const { return static_cast<float*>(p_); }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p_);
+    }
+
+private:
+    void* p_ = nullptr;
+};
+
+// The operator on the current device, its view and the stream, with the
+// device and handle locks held for the whole call.  A square session also
+// checks that n_nodes covers the operator's rows.
+struct Session {
+    int dev;
+    std::unique_lock<std::mutex> call_lk, lk;
+    DeviceCSR* d;
+    CsrView v;
+    hipStream_t s;
+    int64_t n;
+
+    Session(scl_sparse_matrix* P, int64_t n_nodes, bool square = true)
+        : dev(current_device()), call_lk(device_mutex(dev)), lk(P->mu) {
+        if (square && P->rows != P->cols)
+            fail(SCL_ERROR_DIMENSION_MISMATCH, "the transition matrix must be square");
+        n = P->primary();
+        if (square && n_nodes < n)
+            fail(SCL_ERROR_DIMENSION_MISMATCH,
+                 "output buffer smaller than the transition matrix");
+        if (P->wrapped) P->generation++;
+        d = &device_csr(P, dev);
+        v = CsrView{d->n, d->n_sec, d->nnz, d->row_ptr, d->col, d->val, d->sched, &d->merge,
+                    d->grouped};
+        s = library_stream(dev);
+    }
+
+    // The shape tuned or set for F features when it fits X and Y, else the
+    // default.
+    SpmmShape shape(int64_t F, const float* X, const float* Y) const {
+        const auto it = d->shapes.find(F);
+        if (it != d->shapes.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
+        return spmm_default_shape(F, X, Y);
+    }
+    // y = P x
+    void spmv(const float* x, float* y) { launch_spmm_f32(v, x, y, 1, shape(1, x, y), s); }
+    void spmm(const float* X, float* Y, int64_t F) {
+        launch_spmm_f32(v, X, Y, F, shape(F, X, Y), s);
+    }
+    // Y = P' X for the operator's pattern with other values (same order as
+    // the stored ones): no row-grouped form, whose values are a copy.
+    void spmm_values(const float* val, const float* X, float* Y, int64_t F) {
+        CsrView w = v;
+        w.val = val;
+        w.grouped = nullptr;
+        SpmmShape sh = shape(F, X, Y);
+        if (sh.rows_per_group) sh = spmm_default_shape(F, X, Y);
+        launch_spmm_f32(w, X, Y, F, sh, s);
+    }
+    void upload(float* dst, const float* src, int64_t m) {
+        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyHostToDevice, s), "upload");
+    }
+    void download(float* dst, const float* src, int64_t m) {
+        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyDeviceToHost, s),
+                  "download");
+        hip_check(hipStreamSynchronize(s), "download");
+    }
+    float scalar(const float* src) {
+        float h = 0.0f;
+        download(&h, src, 1);
+        return h;
+    }
+};
+
+}  // namespace scl_amd

@@ -23,6 +23,7 @@
 #include "internal.hpp"
 #include "kernels.hpp"
 #include "scl_diffusion.h"
+#include "session.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -43,79 +44,6 @@ constexpr int64_t kDefaultHksSteps = 10;
 constexpr float kDefaultLaziness = 0.5f;
 // rows from which the diffusion-map basis is orthogonalised across the GPU
 constexpr int64_t kWideGramSchmidtRows = 16384;
-
-// Device vector freed at scope exit.
-class DevBuf {
-public:
-    explicit DevBuf(size_t count) {
-        if (hipMalloc(&p_, std::max<size_t>(count, 1) * sizeof(float)) != hipSuccess) {
-            (void)hipGetLastError();
-            fail(SCL_ERROR_OUT_OF_MEMORY, "device allocation failed");
-        }
-    }
-    ~DevBuf() { (void)hipFree(p_); }
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    float* get() const { return static_cast<float*>(p_); }
-
-private:
-    void* p_ = nullptr;
-};
-
-// The operator on the current device, its view and the stream, with the
-// device and handle locks held for the whole call.
-struct Session {
-    int dev;
-    std::unique_lock<std::mutex> call_lk, lk;
-    DeviceCSR* d;
-    CsrView v;
-    hipStream_t s;
-    int64_t n;
-
-    Session(scl_sparse_matrix* P, int64_t n_nodes)
-        : dev(current_device()), call_lk(device_mutex(dev)), lk(P->mu) {
-        if (P->rows != P->cols)
-            fail(SCL_ERROR_DIMENSION_MISMATCH, "the transition matrix must be square");
-        n = P->primary();
-        if (n_nodes < n)
-            fail(SCL_ERROR_DIMENSION_MISMATCH,
-                 "output buffer smaller than the transition matrix");
-        if (P->wrapped) P->generation++;
-        d = &device_csr(P, dev);
-        v = CsrView{d->n, d->n_sec, d->nnz, d->row_ptr, d->col, d->val, d->sched, &d->merge,
-                    d->grouped};
-        s = library_stream(dev);
-    }
-
-    // y = P x (the shape tuned or set for one feature, else the default)
-    void spmv(const float* x, float* y) {
-        SpmmShape sh;
-        const auto it = d->shapes.find(1);
-        if (it != d->shapes.end() && spmm_shape_valid(it->second, 1, x, y)) sh = it->second;
-        else sh = spmm_default_shape(1, x, y);
-        launch_spmm_f32(v, x, y, 1, sh, s);
-    }
-    void spmm(const float* X, float* Y, int64_t F) {
-        SpmmShape sh;
-        const auto it = d->shapes.find(F);
-        if (it != d->shapes.end() && spmm_shape_valid(it->second, F, X, Y)) sh = it->second;
-        else sh = spmm_default_shape(F, X, Y);
-        launch_spmm_f32(v, X, Y, F, sh, s);
-    }
-    void upload(float* dst, const float* src, int64_t m) {
-        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyHostToDevice, s), "upload");
-    }
-    void download(float* dst, const float* src, int64_t m) {
-        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyDeviceToHost, s),
-                  "download");
-        hip_check(hipStreamSynchronize(s), "download");
-    }
-    float scalar(const float* src) {
-        float h = 0.0f;
-        download(&h, src, 1);
-        return h;
-    }
-};
 
 // compute_dpt (diffusion.hpp:670-768) on device buffers; pt is device.
 void dpt_device(Session& S, int64_t root, float* pt, int64_t max_iter, float tol) {
@@ -354,7 +282,7 @@ SCL_EXPORT scl_error_t scl_diffusion_diffusion_map_embedding(scl_sparse_t transi
                     return;
                 }
                 launch_transpose_rect(M, C.get(), n, K, S.s);
-                launch_gram_schmidt(C.get(), w.get(), n, K, part.get(), sc.get(), S.s);
+                launch_gram_schmidt(C.get(), w.get(), n, K, part.get(), sc.get(), true, 0, S.s);
                 launch_transpose_rect(C.get(), M, K, n, S.s);
             };
             launch_dmap_init(Q.get(), n, K, S.s);

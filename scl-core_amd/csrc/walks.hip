@@ -306,14 +306,60 @@ __global__ __launch_bounds__(kBlock) void k_gs_axpy(float* __restrict__ w,
     if (i < n) w[i] = __builtin_fmaf(-coeff, c[i], w[i]);
 }
 
-// column v = w * S[5], or e_(v mod n) when the column is degenerate
+// column v = w * S[5]; a degenerate column (S[5] == 0) becomes e_(v mod n)
+// when `unit`, else it is left to k_gs_random
 __global__ __launch_bounds__(kBlock) void k_gs_store(float* __restrict__ c,
                                                      const float* __restrict__ w, int64_t n,
-                                                     int64_t v, const float* __restrict__ S) {
+                                                     int64_t v, const float* __restrict__ S,
+                                                     int unit) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float inv = S[5];
-    c[i] = inv != 0.0f ? w[i] * inv : (i == v % n ? 1.0f : 0.0f);
+    if (inv != 0.0f) c[i] = w[i] * inv;
+    else if (unit) c[i] = i == v % n ? 1.0f : 0.0f;
+}
+
+// FastRNG of pseudotime.hpp:153-182: four chained rounds of the first half of
+// a SplitMix64 output seed the state; each step is the bit-mirrored form of
+// Marsaglia's xorshift128; uniform() = next() * 2^-32.
+struct FastRng {
+    uint32_t s[4];
+    __device__ explicit FastRng(uint64_t seed) {
+        uint64_t z = seed;
+        for (int i = 0; i < 4; ++i) {
+            z += 0x9e3779b97f4a7c15ULL;
+            z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+            s[i] = (uint32_t)(z >> 32);
+        }
+    }
+    __device__ float uniform() {
+        uint32_t t = s[3];
+        const uint32_t x = s[0];
+        s[3] = s[2];
+        s[2] = s[1];
+        s[1] = x;
+        t ^= t >> 11;
+        t ^= t << 8;
+        s[0] = t ^ x ^ (x << 19);
+        return (float)s[0] * 2.3283064365386963e-10f;
+    }
+};
+
+// Degenerate column v (S[5] == 0) of the pseudotime basis: the normalised
+// vector uniform() - 0.5 of FastRng(v * mult), drawn and summed in index
+// order by one thread (pseudotime.hpp:460-471).  w: n floats.
+__global__ void k_gs_random(float* __restrict__ c, float* __restrict__ w, int64_t n,
+                            int64_t v, uint64_t mult, const float* __restrict__ S) {
+    if (S[5] != 0.0f || threadIdx.x != 0 || blockIdx.x != 0) return;
+    FastRng r((uint64_t)v * mult);
+    float nrm = 0.0f;
+    for (int64_t i = 0; i < n; ++i) {
+        const float x = r.uniform() - 0.5f;
+        w[i] = x;
+        nrm = __builtin_fmaf(x, x, nrm);
+    }
+    const float inv = 1.0f / sqrtf(nrm);
+    for (int64_t i = 0; i < n; ++i) c[i] = w[i] * inv;
 }
 
 __global__ __launch_bounds__(kBlock) void k_gs_copy(float* __restrict__ w,
@@ -480,11 +526,14 @@ void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
     check_launch("transpose launch");
 }
 
-// Modified Gram-Schmidt (diffusion.hpp:370-446) of the K contiguous columns of
-// C (K x n), across the whole GPU: w is an n-float workspace, part holds
-// 2 * kParts partials, S the 6 scalars above.
+// Modified Gram-Schmidt of the K contiguous columns of C (K x n), across the
+// whole GPU: w is an n-float workspace, part holds 2 * kParts partials, S the
+// 6 scalars above.  reorth: the diffusion-map form (diffusion.hpp:370-446:
+// a second pass when some |<w, c_p>| / |w| >= 0.7, degenerate columns become
+// unit vectors); otherwise the pseudotime form (pseudotime.hpp:418-474: one
+// pass, degenerate columns drawn from FastRng(v * rng_mult)).
 void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
-                         hipStream_t s) {
+                         bool reorth, uint64_t rng_mult, hipStream_t s) {
     if (n <= 0 || K <= 0) return;
     const unsigned p = parts_for(n), b = blocks_for(n);
     auto dots = [&](const float* a, const float* c, const float* gate) {
@@ -497,7 +546,7 @@ void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, 
         float* cv = C + v * n;
         hipLaunchKernelGGL(k_gs_reset, dim3(1), dim3(64), 0, s, S, 1.0f);
         hipLaunchKernelGGL(k_gs_copy, dim3(b), dim3(kBlock), 0, s, w, cv, n);
-        for (int pass = 0; pass < 2 && (pass == 0 || v > 0); ++pass) {
+        for (int pass = 0; pass < (reorth ? 2 : 1) && (pass == 0 || v > 0); ++pass) {
             // the second pass runs only when the first left some
             // |<w, c_p>| / |w| >= 0.7 (gate S[3])
             const float* gate = pass == 0 ? nullptr : S + 3;
@@ -507,7 +556,7 @@ void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, 
                 hipLaunchKernelGGL(k_gs_axpy, dim3(b), dim3(kBlock), 0, s, w, C + q * n, n, S,
                                    gate);
             }
-            if (pass == 0 && v > 0) {
+            if (reorth && pass == 0 && v > 0) {
                 dots(w, w, nullptr);
                 fold(2, nullptr);
                 for (int64_t q = 0; q < v; ++q) {
@@ -519,7 +568,10 @@ void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, 
         }
         dots(w, w, nullptr);
         fold(3, nullptr);
-        hipLaunchKernelGGL(k_gs_store, dim3(b), dim3(kBlock), 0, s, cv, w, n, v, S);
+        hipLaunchKernelGGL(k_gs_store, dim3(b), dim3(kBlock), 0, s, cv, w, n, v, S,
+                           rng_mult == 0 ? 1 : 0);
+        if (rng_mult != 0)
+            hipLaunchKernelGGL(k_gs_random, dim3(1), dim3(64), 0, s, cv, w, n, v, rng_mult, S);
     }
     check_launch("gram-schmidt launch");
 }

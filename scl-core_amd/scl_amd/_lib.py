@@ -85,6 +85,11 @@ SIGNATURES = {
     "scl_diffusion_heat_kernel_signature": (c_err, [P, P, c_size, c_real, c_index]),
     "scl_diffusion_diffusion_distance": (c_err, [P, P, c_size, c_index]),
     "scl_diffusion_lazy_random_walk": (c_err, [P, P, c_size, c_index, c_real]),
+    # scl_impute.h, scl_pseudotime.h, scl_algebra.h
+    "scl_impute_diffusion": (c_err, [P, P, c_index, c_index, c_index, P]),
+    "scl_impute_magic": (c_err, [P, P, c_index, c_index, c_index, P]),
+    "scl_pseudotime_diffusion": (c_err, [P, c_index, P, c_index, c_index]),
+    "scl_algebra_spmm": (c_err, [P, P, c_index, P, c_real, c_real]),
     # scl_diffusion_gpu.h
     "scl_gpu_device_count": (c_err, [ctypes.POINTER(ctypes.c_int)]),
     "scl_gpu_sparse_invalidate": (c_err, [P]),

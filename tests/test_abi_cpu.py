@@ -140,6 +140,41 @@ def test_diffusion_argument_checks_come_first():
     assert np.array_equal(X, np.ones((3, 2), np.float32))
 
 
+def test_caller_argument_checks_come_first():
+    """The imputation, pseudotime and algebra entry points reject null and
+    non-positive arguments with the reference's codes (impute.cpp:101-104,
+    136-140; pseudotime.cpp:87-95; algebra.cpp:172-176) before any device
+    work, and the imputations check the operator's shape before it."""
+    lib = _lib.load()
+    ip, ix, d = small_csr()
+    P = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    X = np.ones((3, 2), np.float32)
+    for f in (lib.scl_impute_diffusion, lib.scl_impute_magic):
+        assert f(None, P.handle, 3, 2, 1, X.ctypes.data) == 4
+        assert f(P.handle, None, 3, 2, 1, X.ctypes.data) == 4
+        assert f(P.handle, P.handle, 3, 2, 1, None) == 4
+        assert f(P.handle, P.handle, 0, 2, 1, X.ctypes.data) == 10
+        assert f(P.handle, P.handle, 3, 0, 1, X.ctypes.data) == 10
+        assert f(P.handle, P.handle, 3, 2, 0, X.ctypes.data) == 10
+        assert f(P.handle, P.handle, 4, 2, 1, X.ctypes.data) == _lib.SCL_ERROR_DIMENSION_MISMATCH
+    rect = CsrMatrix.from_arrays([0, 1, 2], [0, 2], [1, 1], (2, 3))
+    assert lib.scl_impute_diffusion(P.handle, rect.handle, 2, 2, 1,
+                                    X.ctypes.data) == _lib.SCL_ERROR_DIMENSION_MISMATCH
+    pt = np.zeros(3, np.float32)
+    f = lib.scl_pseudotime_diffusion
+    assert f(None, 0, pt.ctypes.data, 3, 2) == 4
+    assert f(P.handle, 0, None, 3, 2) == 4
+    assert f(P.handle, 3, pt.ctypes.data, 3, 2) == 10
+    assert f(P.handle, -1, pt.ctypes.data, 3, 2) == 10
+    assert f(P.handle, 0, pt.ctypes.data, 3, -2) == 10
+    f = lib.scl_algebra_spmm
+    assert f(None, X.ctypes.data, 2, X.ctypes.data, 1.0, 0.0) == 4
+    assert f(P.handle, None, 2, X.ctypes.data, 1.0, 0.0) == 4
+    assert f(P.handle, X.ctypes.data, 2, None, 1.0, 0.0) == 4
+    assert f(P.handle, X.ctypes.data, 0, X.ctypes.data, 1.0, 0.0) == 10
+    assert np.array_equal(X, np.ones((3, 2), np.float32)) and not np.any(pt)
+
+
 @pytest.mark.skipif(scl_amd.device_count() > 0, reason="host has a HIP device")
 def test_no_device_reports_feature_unavailable():
     ip, ix, d = small_csr()
