@@ -33,6 +33,9 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
   wk  the vector-side entry points (lazy walk, heat kernel, RWR, DPT,
       diffusion map, diffusion distance) through the C ABI vs the CPU
       restatement
+  cl  the other SpMM callers (diffusion imputation, MAGIC, diffusion-
+      component pseudotime, Y = alpha A X + beta Y) through the C ABI vs the
+      CPU restatement
 """
 from __future__ import annotations
 
@@ -313,6 +316,74 @@ def leg_wk(steps: int, with_cpu: bool = True) -> dict:
     return out
 
 
+def leg_cl(steps: int, with_cpu: bool = True) -> dict:
+    """The other SpMM callers (imputation, MAGIC, diffusion-component
+    pseudotime, Y = alpha A X + beta Y) through the C ABI on knn30_200k with
+    a 200000 x 500 expression matrix at 10% density; with the CPU baseline
+    on, the CPU restatement of each is timed beside it on this host."""
+    import numpy as np
+    from scl_amd import CsrMatrix, callers, synth
+    orc = None
+    if with_cpu:
+        from oracle import oracle as orc
+    n, G = 200_000, 500
+    raw = synth.knn_graph(n, 30, seed=7)
+    g = raw.normalized()
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    A = CsrMatrix.from_arrays(raw.indptr, raw.indices, raw.data, (n, n))
+    P.prepare_device()
+    A.prepare_device()
+    rng = np.random.Generator(np.random.PCG64(8))
+    rows, cols = [], []
+    for r0 in range(0, n, 20_000):
+        r, c = np.nonzero(rng.random((min(20_000, n - r0), G), dtype=np.float32) < 0.1)
+        rows.append(r + r0)
+        cols.append(c)
+    rows, cols = np.concatenate(rows), np.concatenate(cols).astype(np.int64)
+    ip = np.zeros(n + 1, np.int64)
+    np.add.at(ip, rows + 1, 1)
+    ip = np.cumsum(ip)
+    ev = rng.gamma(1.0, 1.0, len(cols)).astype(np.float32)
+    M = CsrMatrix.from_arrays(ip, cols, ev, (n, G))
+    F = 64
+    Xa = synth.normal(n, F, seed=9)
+    Ya = synth.normal(n, F, seed=10)
+    out = {"workload": f"knn30_200k, expression {n} x {G} ({len(cols)} stored), "
+                       "host arrays in/out"}
+    if with_cpu:
+        out["cpu_threads"] = orc.num_threads()
+    cases = {
+        "impute_diffusion_t3": (
+            lambda: callers.impute_diffusion(M, P, n, G, 3),
+            lambda: orc.impute_diffusion_f32(g.indptr, g.indices, g.data,
+                                             orc.densify_f32(ip, cols, ev, n, G), 3)),
+        "impute_magic_t3": (
+            lambda: callers.impute_magic(M, A, n, G, 3),
+            lambda: orc.magic_f32(raw.indptr, raw.indices, raw.data,
+                                  orc.densify_f32(ip, cols, ev, n, G), 3)),
+        "pseudotime_k10": (
+            lambda: callers.pseudotime_diffusion(A, 0, 10),
+            lambda: orc.pseudotime_f32(raw.indptr, raw.indices, raw.data, 0, 10, 100)),
+        "algebra_spmm_f64": (
+            lambda: callers.algebra_spmm(P, Xa, Ya.copy(), 0.5, 2.0),
+            lambda: orc.algebra_spmm_f32(g.indptr, g.indices, g.data, Xa, Ya, 0.5, 2.0)),
+    }
+    for name, (gpu_fn, cpu_fn) in cases.items():
+        gpu_fn()
+        reps = max(1, steps // 5)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gpu_fn()
+        g_ms = (time.perf_counter() - t0) * 1e3 / reps
+        out[name] = {"gpu_ms": round(g_ms, 2)}
+        if with_cpu:
+            t0 = time.perf_counter()
+            cpu_fn()
+            c_ms = (time.perf_counter() - t0) * 1e3
+            out[name].update({"cpu_ms": round(c_ms, 1), "speedup": round(c_ms / g_ms, 1)})
+    return out
+
+
 def leg_c5(steps: int) -> dict:
     """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
     import numpy as np
@@ -395,7 +466,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk")
+    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk,cl")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="vec,group,unroll,flags for the headline SpMM instead of the "
@@ -489,12 +560,12 @@ def main() -> int:
     del Xd, Yd, Wd, A
     torch.cuda.empty_cache()
     for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5), ("mp", leg_mp),
-                     ("wk", leg_wk)):
+                     ("wk", leg_wk), ("cl", leg_cl)):
         if name not in args.legs.split(","):
             continue
         log(f"leg {name}")
         try:
-            if name == "wk":
+            if name in ("wk", "cl"):
                 legs[name] = fn(max(3, args.steps // 2), not args.no_cpu_baseline)
             else:
                 legs[name] = fn(max(3, args.steps // 2))
