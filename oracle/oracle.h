@@ -185,6 +185,27 @@ void orc_algebra_spmm_f32(int64_t m, int64_t F, const int64_t* ip, const int64_t
                           const float* v, const float* X, float* Y, float alpha,
                           float beta);
 
+/* ---- construction of the operator (construct.c) ---- */
+/* compute_kernel_matrix (kernel.hpp:530-590): kernel of each stored distance;
+ * type is the C ABI's scl_kernel_type_t value. */
+void orc_kernel_matrix_f32(int64_t nnz, const float* d, float bandwidth, int type, float* out);
+/* kernel_row_sums (kernel.hpp:596-645). */
+void orc_kernel_row_sums_f32(int64_t n, const int64_t* ip, const float* d, float bandwidth,
+                             int type, float* sums);
+/* vector_sum of transition.hpp:165-179 (8 strided partials, then the tail). */
+float orc_vector_sum8_f32(const float* v, int64_t n);
+/* row_normalize_to_stochastic (transition.hpp:493-525). */
+void orc_row_normalize_f32(int64_t n, const int64_t* ip, const float* v, float* out);
+/* symmetrize_transition (transition.hpp:530-565), dense n x n output. */
+void orc_symmetrize_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                        float* S);
+/* transition_matrix_from_velocity (transition.hpp:449-487), dense n x n. */
+void orc_from_velocity_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                           float* S);
+/* stationary_distribution (transition.hpp:571-623); returns iterations run. */
+int64_t orc_stationary_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                           int64_t max_iter, float tol, float* pi);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,13 @@ def load():
         "orc_pseudotime_f32": (ctypes.c_int, [I64, P, P, P, I64, I64, I64, P]),
         "orc_algebra_spmm_f32": (None, [I64, I64, P, P, P, P, P, ctypes.c_float,
                                         ctypes.c_float]),
+        "orc_kernel_matrix_f32": (None, [I64, P, ctypes.c_float, ctypes.c_int, P]),
+        "orc_kernel_row_sums_f32": (None, [I64, P, P, ctypes.c_float, ctypes.c_int, P]),
+        "orc_vector_sum8_f32": (ctypes.c_float, [P, I64]),
+        "orc_row_normalize_f32": (None, [I64, P, P, P]),
+        "orc_symmetrize_f32": (None, [I64, P, P, P, P]),
+        "orc_from_velocity_f32": (None, [I64, P, P, P, P]),
+        "orc_stationary_f32": (I64, [I64, P, P, P, I64, ctypes.c_float, P]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -435,3 +442,61 @@ def algebra_spmm_f32(indptr, indices, vals, X, Y, alpha, beta):
     load().orc_algebra_spmm_f32(len(ip) - 1, F, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
                                 X.ctypes.data, Y.ctypes.data, alpha, beta)
     return Y
+
+
+# ------------------------------------------------ construction of the operator --
+
+def kernel_matrix_f32(d, bandwidth, kind):
+    d = _f32(d)
+    out = np.empty_like(d)
+    load().orc_kernel_matrix_f32(len(d), d.ctypes.data, bandwidth, kind, out.ctypes.data)
+    return out
+
+
+def kernel_row_sums_f32(indptr, d, bandwidth, kind):
+    ip, d = _i64(indptr), _f32(d)
+    out = np.empty(len(ip) - 1, np.float32)
+    load().orc_kernel_row_sums_f32(len(ip) - 1, ip.ctypes.data, d.ctypes.data, bandwidth, kind,
+                                   out.ctypes.data)
+    return out
+
+
+def vector_sum8_f32(v):
+    v = _f32(v)
+    return float(np.float32(load().orc_vector_sum8_f32(v.ctypes.data, len(v))))
+
+
+def row_normalize_f32(indptr, vals):
+    ip, v = _i64(indptr), _f32(vals)
+    out = np.empty_like(v)
+    load().orc_row_normalize_f32(len(ip) - 1, ip.ctypes.data, v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def symmetrize_f32(indptr, indices, vals):
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    S = np.empty((n, n), np.float32)
+    load().orc_symmetrize_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, S.ctypes.data)
+    return S
+
+
+def from_velocity_f32(indptr, indices, vals):
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    S = np.empty((n, n), np.float32)
+    load().orc_from_velocity_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data,
+                                 S.ctypes.data)
+    return S
+
+
+def stationary_f32(indptr, indices, vals, max_iter, tol):
+    """Returns (pi, iterations run)."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    pi = np.empty(n, np.float32)
+    it = load().orc_stationary_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, max_iter,
+                                   tol, pi.ctypes.data)
+    if it < 0:
+        raise MemoryError
+    return pi, it

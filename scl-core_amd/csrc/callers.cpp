@@ -32,54 +32,6 @@ void check_dense(int64_t rows, int64_t cols) {
         fail(SCL_ERROR_INVALID_ARGUMENT, "dense operand too large");
 }
 
-// The first `slots` primary slots of an expression handle on the device, as
-// stored (int64 pattern, float values), and whether every slot is strictly
-// increasing.  Copied under the handle's lock, before any other lock.
-struct Expression {
-    int64_t slots;
-    DevBuf ip, ix, v;
-    bool ordered = false;
-
-    Expression(scl_sparse_matrix* X, int64_t slots_, hipStream_t s)
-        : slots(slots_), ip(slots_ + 1, 8), ix(count(X, slots_), 8), v(count(X, slots_)) {
-        std::lock_guard<std::mutex> lk(X->mu);
-        const int64_t nnz = X->indptr[slots] - X->indptr[0];
-        std::vector<int64_t> rebased((size_t)slots + 1);
-        for (int64_t i = 0; i <= slots; ++i) rebased[i] = X->indptr[i] - X->indptr[0];
-        hip_check(hipMemcpyAsync(ip.as<int64_t>(), rebased.data(), (size_t)(slots + 1) * 8,
-                                 hipMemcpyHostToDevice, s),
-                  "upload pattern");
-        if (nnz > 0) {
-            hip_check(hipMemcpyAsync(ix.as<int64_t>(), X->indices + X->indptr[0],
-                                     (size_t)nnz * 8, hipMemcpyHostToDevice, s),
-                      "upload pattern");
-            hip_check(hipMemcpyAsync(v.get(), X->data + X->indptr[0], (size_t)nnz * 4,
-                                     hipMemcpyHostToDevice, s),
-                      "upload values");
-        }
-        // unordered slots (a repeated or descending index) are walked in
-        // stored order by one lane, so a repeated index keeps its last value
-        DevBuf flag(1, sizeof(int));
-        hip_check(hipMemsetAsync(flag.get(), 0, sizeof(int), s), "clear flag");
-        int h = 0;
-        if (launch_find_unsorted(ip.as<int64_t>(), ix.as<int64_t>(), slots, flag.as<int>(), s))
-            hip_check(hipMemcpyAsync(&h, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s),
-                      "download flag");
-        hip_check(hipStreamSynchronize(s), "upload expression");
-        ordered = h != 0;
-    }
-
-    static int64_t count(scl_sparse_matrix* X, int64_t slots) {
-        std::lock_guard<std::mutex> lk(X->mu);
-        return X->indptr[slots] - X->indptr[0];
-    }
-
-    void densify(float* out, int64_t n_cells, int64_t G, bool is_csr, hipStream_t s) const {
-        launch_densify(ip.as<int64_t>(), ix.as<int64_t>(), v.get(), slots, n_cells, G, is_csr,
-                       ordered, out, s);
-    }
-};
-
 // Operator checks shared by the imputations: square, n_cells rows.
 void check_cells_operator(const scl_sparse_matrix* P, int64_t n_cells) {
     if (P->rows != P->cols || P->rows != n_cells)
@@ -137,7 +89,7 @@ SCL_EXPORT scl_error_t scl_impute_diffusion(scl_sparse_t matrix,
             fail(SCL_ERROR_DIMENSION_MISMATCH,
                  "the expression matrix has fewer cells / genes than requested");
         const int dev = current_device();
-        const Expression X(matrix, slots, library_stream(dev));
+        const HostPattern X(matrix, slots, library_stream(dev));
         Session S(transition_matrix, n_cells);
         const int64_t total = n_cells * n_genes;
         DevBuf a(total), b(total);
@@ -173,8 +125,8 @@ SCL_EXPORT scl_error_t scl_impute_magic(scl_sparse_t matrix, scl_sparse_t affini
             fail(SCL_ERROR_DIMENSION_MISMATCH,
                  "the expression matrix has fewer cells than requested");
         const int dev = current_device();
-        std::unique_ptr<Expression> X;
-        if (is_csr) X = std::make_unique<Expression>(matrix, n_cells, library_stream(dev));
+        std::unique_ptr<HostPattern> X;
+        if (is_csr) X = std::make_unique<HostPattern>(matrix, n_cells, library_stream(dev));
         Session S(affinity_matrix, n_cells);
         const int64_t total = n_cells * n_genes;
         DevBuf a(total), b(total), w(S.d->nnz), copy(n_cells, sizeof(int32_t)),

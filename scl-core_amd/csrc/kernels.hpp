@@ -189,6 +189,8 @@ void launch_dpt_finish(const float* vis, float* hit, float* pt, int64_t n, float
                        float* max_time, hipStream_t s);
 // x = value * e_at (x = 0 when at is outside [0, n)).
 void launch_set_unit(float* x, int64_t n, int64_t at, float value, hipStream_t s);
+// x[0, n) = value
+void launch_fill(float* x, int64_t n, float value, hipStream_t s);
 void launch_add(float* acc, const float* x, int64_t n, hipStream_t s);
 void launch_scale(float* x, int64_t n, float a, hipStream_t s);
 // Diffusion-map start vectors and modified Gram-Schmidt (one workgroup).
@@ -232,5 +234,34 @@ void launch_pt_distance(const float* Q, int64_t n, int64_t K, int64_t root, floa
 // Y = alpha A X + beta Y with zero weights skipped (one wave per row).
 void launch_spmm_axpby(const CsrView& A, const float* X, float* Y, int64_t F, float alpha,
                        float beta, hipStream_t s);
+
+// ---- construction of the operator (construct.hip) ----
+// KernelParams of kernel.hpp:108-122 for one bandwidth and kernel kind (the
+// C ABI's scl_kernel_type_t value).
+struct KernelWeights {
+    float inv_h, inv_h2, inv_2h2, h2;
+    int kind;
+};
+KernelWeights kernel_weights(float bandwidth, int kind);
+// kernel value of every stored distance
+void launch_kernel_values(const float* d, int64_t nnz, const KernelWeights& p, float* out,
+                          hipStream_t s);
+// per-row sums of the kernel values in stored order (int64 row pointer)
+void launch_kernel_row_sums(const int64_t* ip, const float* d, int64_t n,
+                            const KernelWeights& p, float* sums, hipStream_t s);
+// row_normalize_to_stochastic of n rows (int64 row pointer)
+void launch_row_normalize(const int64_t* ip, const float* v, int64_t n, float* out,
+                          hipStream_t s);
+// dense (T + T^T)/2 into the zeroed n x n S, then row normalisation; serial:
+// some row is not strictly increasing (one thread, stored order)
+void launch_symmetrize(const int64_t* ip, const int64_t* ix, const float* v, int64_t n,
+                       bool serial, float* S, hipStream_t s);
+// dense velocity transition into the zeroed n x n S
+void launch_from_velocity(const int64_t* ip, const int64_t* ix, const float* v, int64_t n,
+                          float* S, hipStream_t s);
+// after y = T^T pi: y /= vector_sum(y) (when above 1e-15) and diff[0] =
+// max |y - pi|; scale: 1 float, part: 1024 floats
+void launch_stationary_update(float* y, const float* pi, int64_t n, float* scale, float* part,
+                              float* diff, hipStream_t s);
 
 }  // namespace scl_amd

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <vector>
 
 namespace scl_amd {
 
@@ -93,6 +94,55 @@ struct Session {
         float h = 0.0f;
         download(&h, src, 1);
         return h;
+    }
+};
+
+// The first `slots` primary slots of a handle on the device, as stored (int64
+// pattern rebased to start at 0, float values).  `serial` is set when some
+// slot is not strictly increasing (a repeated or descending index): such
+// slots are walked in stored order by one lane, so that a repeated index
+// keeps its last value.  Copied under the handle's lock; take it before any
+// other lock.
+struct HostPattern {
+    int64_t slots;
+    DevBuf ip, ix, v;
+    bool serial = false;
+
+    HostPattern(scl_sparse_matrix* X, int64_t slots_, hipStream_t s)
+        : slots(slots_), ip(slots_ + 1, 8), ix(count(X, slots_), 8), v(count(X, slots_)) {
+        std::lock_guard<std::mutex> lk(X->mu);
+        const int64_t nnz = X->indptr[slots] - X->indptr[0];
+        std::vector<int64_t> rebased((size_t)slots + 1);
+        for (int64_t i = 0; i <= slots; ++i) rebased[i] = X->indptr[i] - X->indptr[0];
+        hip_check(hipMemcpyAsync(ip.as<int64_t>(), rebased.data(), (size_t)(slots + 1) * 8,
+                                 hipMemcpyHostToDevice, s),
+                  "upload pattern");
+        if (nnz > 0) {
+            hip_check(hipMemcpyAsync(ix.as<int64_t>(), X->indices + X->indptr[0],
+                                     (size_t)nnz * 8, hipMemcpyHostToDevice, s),
+                      "upload pattern");
+            hip_check(hipMemcpyAsync(v.get(), X->data + X->indptr[0], (size_t)nnz * 4,
+                                     hipMemcpyHostToDevice, s),
+                      "upload values");
+        }
+        DevBuf flag(1, sizeof(int));
+        hip_check(hipMemsetAsync(flag.get(), 0, sizeof(int), s), "clear flag");
+        int h = 0;
+        if (launch_find_unsorted(ip.as<int64_t>(), ix.as<int64_t>(), slots, flag.as<int>(), s))
+            hip_check(hipMemcpyAsync(&h, flag.get(), sizeof(int), hipMemcpyDeviceToHost, s),
+                      "download flag");
+        hip_check(hipStreamSynchronize(s), "upload pattern");
+        serial = h != 0;
+    }
+
+    static int64_t count(scl_sparse_matrix* X, int64_t slots) {
+        std::lock_guard<std::mutex> lk(X->mu);
+        return X->indptr[slots] - X->indptr[0];
+    }
+
+    void densify(float* out, int64_t n_cells, int64_t G, bool is_csr, hipStream_t s) const {
+        launch_densify(ip.as<int64_t>(), ix.as<int64_t>(), v.get(), slots, n_cells, G, is_csr,
+                       serial, out, s);
     }
 };
 

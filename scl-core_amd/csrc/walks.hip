@@ -7,6 +7,8 @@
 // explicit fmaf, matching the CPU restatement operation for operation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -138,6 +140,13 @@ __global__ __launch_bounds__(kBlock) void k_set_unit(float* __restrict__ x, int6
                                                      int64_t at, float value) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i < n) x[i] = (i == at) ? value : 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(float* __restrict__ x, int64_t n,
+                                                 float value) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kBlock)
+        x[i] = value;
 }
 
 __global__ __launch_bounds__(kBlock) void k_add(float* __restrict__ acc,
@@ -483,6 +492,13 @@ void launch_dpt_finish(const float* vis, float* hit, float* pt, int64_t n, float
 void launch_set_unit(float* x, int64_t n, int64_t at, float value, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_set_unit, dim3(blocks_for(n)), dim3(kBlock), 0, s, x, n, at, value);
+    check_launch("fill launch");
+}
+
+void launch_fill(float* x, int64_t n, float value, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t b = std::min<int64_t>((n + kBlock - 1) / kBlock, 1 << 20);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)b), dim3(kBlock), 0, s, x, n, value);
     check_launch("fill launch");
 }
 

@@ -90,6 +90,13 @@ SIGNATURES = {
     "scl_impute_magic": (c_err, [P, P, c_index, c_index, c_index, P]),
     "scl_pseudotime_diffusion": (c_err, [P, c_index, P, c_index, c_index]),
     "scl_algebra_spmm": (c_err, [P, P, c_index, P, c_real, c_real]),
+    # scl_kernel.h, scl_transition.h
+    "scl_kernel_compute_matrix": (c_err, [P, P, c_real, ctypes.c_int]),
+    "scl_kernel_row_sums": (c_err, [P, P, c_size, c_real, ctypes.c_int]),
+    "scl_transition_matrix_from_velocity": (c_err, [P, c_index, P]),
+    "scl_transition_row_normalize_to_stochastic": (c_err, [P, c_index, P, c_size]),
+    "scl_transition_symmetrize": (c_err, [P, c_index, P]),
+    "scl_transition_stationary_distribution": (c_err, [P, c_index, P, c_index, c_real]),
     # scl_diffusion_gpu.h
     "scl_gpu_device_count": (c_err, [ctypes.POINTER(ctypes.c_int)]),
     "scl_gpu_sparse_invalidate": (c_err, [P]),

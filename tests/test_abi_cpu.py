@@ -175,6 +175,36 @@ def test_caller_argument_checks_come_first():
     assert np.array_equal(X, np.ones((3, 2), np.float32)) and not np.any(pt)
 
 
+def test_construct_argument_checks_come_first():
+    """The kernel and transition entry points reject null arguments
+    (kernel.cpp:151-152, 176-179; transition.cpp:34, 63, 94, 124) and, before
+    any device work, dimensions the reference would read or write past."""
+    lib = _lib.load()
+    ip, ix, d = small_csr()
+    P = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    out = np.zeros(9, np.float32)
+    assert lib.scl_kernel_compute_matrix(None, out.ctypes.data, 1.0, 0) == 4
+    assert lib.scl_kernel_compute_matrix(P.handle, None, 1.0, 0) == 4
+    assert lib.scl_kernel_row_sums(P.handle, None, 3, 1.0, 0) == 4
+    assert lib.scl_kernel_row_sums(P.handle, out.ctypes.data, 0, 1.0, 0) == 10
+    assert lib.scl_kernel_row_sums(P.handle, out.ctypes.data, 4, 1.0, 0) == 11
+    for f in (lib.scl_transition_symmetrize, lib.scl_transition_matrix_from_velocity):
+        assert f(None, 3, out.ctypes.data) == 4
+        assert f(P.handle, 3, None) == 4
+        assert f(P.handle, 2, out.ctypes.data) == 11
+        assert f(P.handle, -1, out.ctypes.data) == 10
+    f = lib.scl_transition_row_normalize_to_stochastic
+    assert f(None, 3, out.ctypes.data, 9) == 4
+    assert f(P.handle, 4, out.ctypes.data, 9) == 11
+    assert f(P.handle, 3, out.ctypes.data, 1) == 11
+    f = lib.scl_transition_stationary_distribution
+    assert f(P.handle, 3, None, 10, 1e-6) == 4
+    assert f(P.handle, 4, out.ctypes.data, 10, 1e-6) == 11
+    rect = CsrMatrix.from_arrays([0, 1, 2], [0, 2], [1, 1], (2, 3))
+    assert f(rect.handle, 2, out.ctypes.data, 10, 1e-6) == 11
+    assert not np.any(out)
+
+
 @pytest.mark.skipif(scl_amd.device_count() > 0, reason="host has a HIP device")
 def test_no_device_reports_feature_unavailable():
     ip, ix, d = small_csr()

@@ -1,0 +1,101 @@
+"""GPU parity of the operator-construction entry points (scl_kernel_*,
+scl_transition_*) against the CPU restatement (oracle/construct.c).  The
+transition functions are bit-identical, the stationary distribution
+included (same gather order, the eight-partial vector_sum on the device);
+kernel weights are exact for the algebraic kernels and within a few ulp
+where exp or cos is evaluated (device libm vs host libm)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from scl_amd import CscMatrix, CsrMatrix, construct, synth
+from scl_amd._lib import SCLError
+
+pytestmark = pytest.mark.gpu
+
+
+def distances(n, k, seed):
+    g = synth.uniform_knn(n, k, seed=seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = rng.gamma(2.0, 0.6, len(g.data)).astype(np.float32)
+    return g.indptr, g.indices, d
+
+
+@pytest.mark.parametrize("kind", list(construct.KERNELS))
+def test_kernel_weights(gpu, kind):
+    ip, ix, d = distances(20_000, 12, seed=31)
+    D = CsrMatrix.from_arrays(ip, ix, d, (20_000, 20_000))
+    k = construct.KERNELS[kind]
+    for h in (0.8, 2.5):
+        got = construct.kernel_matrix(D, h, kind)
+        ref = orc.kernel_matrix_f32(d, h, k)
+        sums = construct.kernel_row_sums(D, h, kind)
+        rsum = orc.kernel_row_sums_f32(ip, d, h, k)
+        if kind in ("epanechnikov", "cauchy", "uniform", "triangular"):
+            assert np.array_equal(got, ref) and np.array_equal(sums, rsum), (kind, h)
+        else:
+            # exp / cos: device and host libm may differ in the last bits
+            assert np.abs(got - ref).max() <= 4e-7 * max(1.0, np.abs(ref).max()), (kind, h)
+            assert np.allclose(sums, rsum, rtol=2e-6, atol=1e-7), (kind, h)
+
+
+def test_row_normalize(gpu):
+    ip, ix, d = distances(30_000, 10, seed=32)
+    d[ip[5]:ip[6]] = 0.0                     # a zero row
+    M = CsrMatrix.from_arrays(ip, ix, d, (30_000, 30_000))
+    assert np.array_equal(construct.row_normalize(M), orc.row_normalize_f32(ip, d))
+    part = construct.row_normalize(M, 1000)
+    assert np.array_equal(part, orc.row_normalize_f32(ip[:1001], d[:ip[1000]]))
+
+
+def test_symmetrize_and_velocity(gpu):
+    n = 3000
+    g = synth.uniform_knn(n, 9, seed=33)
+    T = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    assert np.array_equal(construct.symmetrize(T),
+                          orc.symmetrize_f32(g.indptr, g.indices, g.data))
+    # rows with repeated and descending indices and self-loops: stored order
+    ip = np.array([0, 4, 6, 9, 10], np.int64)
+    ix = np.array([2, 2, 0, 1, 3, 0, 2, 2, 1, 1], np.int64)
+    v = np.array([0.3, 0.2, 0.5, 1e-3, 1, 2, 0.7, 0.1, 0.2, 4], np.float32)
+    U = CsrMatrix.from_arrays(ip, ix, v, (4, 4))
+    assert np.array_equal(construct.symmetrize(U), orc.symmetrize_f32(ip, ix, v))
+    # velocity graph with signed weights
+    rng = np.random.Generator(np.random.PCG64(34))
+    sv = rng.standard_normal(len(g.data)).astype(np.float32)
+    sv[g.indptr[11]:g.indptr[12]] = -1.0
+    V = CsrMatrix.from_arrays(g.indptr, g.indices, sv, (n, n))
+    got = construct.matrix_from_velocity(V)
+    assert np.array_equal(got, orc.from_velocity_f32(g.indptr, g.indices, sv))
+    assert got[11, 11] == 1.0
+    assert np.array_equal(construct.matrix_from_velocity(U), orc.from_velocity_f32(ip, ix, v))
+
+
+def test_stationary_distribution(gpu):
+    for g, tol in ((synth.knn_graph(50_000, 10, seed=35).normalized(), 1e-9),
+                   (synth.uniform_knn(40_000, 8, seed=36).normalized(), 1e-6),
+                   (synth.uniform_knn(2000, 5, seed=37).normalized(), 1e-4)):
+        T = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
+        for max_iter in (0, 1, 7, 1000):
+            got = construct.stationary_distribution(T, max_iter, tol)
+            ref, _ = orc.stationary_f32(g.indptr, g.indices, g.data, max_iter, tol)
+            assert np.array_equal(got, ref), (g.n, tol, max_iter)
+
+
+def test_construct_argument_errors(gpu):
+    g = synth.uniform_knn(100, 4, seed=38)
+    T = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (100, 100))
+    from scl_amd import _lib
+    lib = _lib.load()
+    out = np.zeros(100 * 100, np.float32)
+    assert lib.scl_transition_symmetrize(T.handle, 99, out.ctypes.data) == 11
+    assert lib.scl_transition_stationary_distribution(T.handle, 101, out.ctypes.data, 5,
+                                                      1e-6) == 11
+    assert lib.scl_transition_row_normalize_to_stochastic(T.handle, 100, out.ctypes.data,
+                                                          10) == 11
+    assert lib.scl_kernel_row_sums(T.handle, out.ctypes.data, 99, 1.0, 0) == 11
+    with pytest.raises(SCLError) as e:
+        construct.row_normalize(T, 101)
+    assert e.value.code == 11
