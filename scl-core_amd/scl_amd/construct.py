@@ -50,7 +50,8 @@ def row_normalize(matrix: SparseMatrixBase, n: int = -1) -> np.ndarray:
     """scl_transition_row_normalize_to_stochastic over the first n rows."""
     n = matrix.shape[0] if n < 0 else n
     ip, _, _ = matrix.export()
-    out = np.empty(int(ip[n] - ip[0]), np.float32)
+    # rows past the handle's are the library's error to report
+    out = np.empty(int(ip[min(n, len(ip) - 1)] - ip[0]), np.float32)
     _lib.check(_lib.load().scl_transition_row_normalize_to_stochastic(
         matrix.handle, n, out.ctypes.data, len(out)))
     return out
