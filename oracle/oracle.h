@@ -206,6 +206,15 @@ void orc_from_velocity_f32(int64_t n, const int64_t* ip, const int64_t* ix, cons
 int64_t orc_stationary_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
                            int64_t max_iter, float tol, float* pi);
 
+/* ---- k-nearest neighbours (neighbors.c) ---- */
+/* vectorize::sum_squared, 8-lane float model (vectorize.hpp:782-822). */
+float orc_sum_squared_f32(const float* v, int64_t n);
+/* compute_norms (neighbors.hpp:303-317). */
+void orc_norms_f32(int64_t n, const int64_t* ip, const float* v, float* norms);
+/* knn (neighbors.hpp:319-476): out_idx / out_dist are n x k. */
+int orc_knn_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                const float* norms, int64_t k, int64_t* out_idx, float* out_dist);
+
 #ifdef __cplusplus
 }
 #endif

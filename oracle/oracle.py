@@ -94,6 +94,9 @@ def load():
         "orc_symmetrize_f32": (None, [I64, P, P, P, P]),
         "orc_from_velocity_f32": (None, [I64, P, P, P, P]),
         "orc_stationary_f32": (I64, [I64, P, P, P, I64, ctypes.c_float, P]),
+        "orc_sum_squared_f32": (ctypes.c_float, [P, I64]),
+        "orc_norms_f32": (None, [I64, P, P, P]),
+        "orc_knn_f32": (ctypes.c_int, [I64, P, P, P, P, I64, P, P]),
     }
     for k, (res, args) in sig.items():
         fn = getattr(lib, k)
@@ -500,3 +503,30 @@ def stationary_f32(indptr, indices, vals, max_iter, tol):
     if it < 0:
         raise MemoryError
     return pi, it
+
+
+# ------------------------------------------------------ k-nearest neighbours --
+
+def sum_squared_f32(v):
+    v = _f32(v)
+    return float(np.float32(load().orc_sum_squared_f32(v.ctypes.data, len(v))))
+
+
+def norms_f32(indptr, vals):
+    ip, v = _i64(indptr), _f32(vals)
+    out = np.empty(len(ip) - 1, np.float32)
+    load().orc_norms_f32(len(ip) - 1, ip.ctypes.data, v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def knn_f32(indptr, indices, vals, k, norms=None):
+    """(indices n x k int64, distances n x k float32), as the reference."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    nr = norms_f32(ip, v) if norms is None else _f32(norms)
+    idx = np.empty((n, k), np.int64)
+    dist = np.empty((n, k), np.float32)
+    if load().orc_knn_f32(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, nr.ctypes.data, k,
+                          idx.ctypes.data, dist.ctypes.data):
+        raise MemoryError
+    return idx, dist

@@ -264,4 +264,16 @@ void launch_from_velocity(const int64_t* ip, const int64_t* ix, const float* v, 
 void launch_stationary_update(float* y, const float* pi, int64_t n, float* scale, float* part,
                               float* diff, hipStream_t s);
 
+// ---- k-nearest neighbours (neighbors.hip) ----
+// vectorize::sum_squared of each row's stored values (int64 row pointer)
+void launch_sum_squared(const int64_t* ip, const float* v, int64_t n, float* norms,
+                        hipStream_t s);
+// padded shape of the dense sample matrix the scan reads
+int64_t knn_rows_padded(int64_t n);
+int64_t knn_features_padded(int64_t f);
+// k nearest of every sample of the zero-padded dense X (n_pad x f_pad);
+// ld / lj: n x k list scratch; out_*: n x k
+void launch_knn(const float* X, const float* norms, int64_t n, int64_t f_pad, int64_t k,
+                float* ld, int32_t* lj, float* out_dist, int64_t* out_idx, hipStream_t s);
+
 }  // namespace scl_amd

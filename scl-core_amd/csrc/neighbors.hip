@@ -1,0 +1,244 @@
+// neighbors.hip -- gfx950 kernels of the exact k-nearest-neighbour search
+// (scl/kernel/neighbors.hpp:300-476): row norms and the all-pairs distance
+// scan with a running top-k per query.
+//
+// The reference's sparse dot products add the products of matching indices
+// in increasing index order, one FMA each.  Over dense rows the same chain
+// in feature order adds only exact zeros besides them, so the tiled product
+// below, which keeps one sequential FMA chain per (query, candidate) over
+// the features, yields the reference's dot products bit for bit.  The
+// distance is fma(-2, dot, n_i + n_j), clamped at 0, as the reference's
+// contracted "n_i + n_j - 2 * dot".  No MFMA: its reduction order is not a
+// sequential chain.
+// Contraction is off: every fused multiply-add is an explicit fmaf.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace scl_amd {
+
+using namespace dev;
+
+namespace {
+
+// vectorize::sum_squared for 8-lane float vectors (vectorize.hpp:782-822)
+// over one row's stored values, one wave per row: lane t < 32 keeps the
+// accumulator of position t in every block of 32 (accumulator t / 8, lane
+// t % 8); lanes 0..7 fold (a0 + a1) + (a2 + a3), run the remaining whole
+// vectors, and the eight lanes are summed ((l0 + l4) + (l2 + l6)) +
+// ((l1 + l5) + (l3 + l7)) before the tail.
+__global__ __launch_bounds__(kBlock) void k_sum_squared(const int64_t* __restrict__ ip,
+                                                        const float* __restrict__ v, int64_t n,
+                                                        float* __restrict__ norms) {
+    const int64_t row = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    if (row >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t beg = ip[row], len = ip[row + 1] - beg;
+    const float* x = v + beg;
+    float acc = 0.0f;
+    int64_t i = 0;
+    for (; i + 32 <= len; i += 32) {
+        const float e = lane < 32 ? x[i + lane] : 0.0f;
+        if (lane < 32) acc = __builtin_fmaf(e, e, acc);
+    }
+    const float a1 = __shfl(acc, (lane & 7) + 8, 64);
+    const float a2 = __shfl(acc, (lane & 7) + 16, 64);
+    const float a3 = __shfl(acc, (lane & 7) + 24, 64);
+    float a = (acc + a1) + (a2 + a3);
+    for (; i + 8 <= len; i += 8) {
+        const float e = lane < 8 ? x[i + lane] : 0.0f;
+        a = __builtin_fmaf(e, e, a);
+    }
+    float l[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) l[t] = __shfl(a, t, 64);
+    float r = ((l[0] + l[4]) + (l[2] + l[6])) + ((l[1] + l[5]) + (l[3] + l[7]));
+    for (; i < len; ++i) r = __builtin_fmaf(x[i], x[i], r);
+    if (lane == 0) norms[row] = r;
+}
+
+constexpr int kQ = 64;   // queries per workgroup
+constexpr int kC = 64;   // candidates per tile
+constexpr int kK = 16;   // features per LDS stage
+constexpr int kPad = 4;  // LDS row padding (floats)
+
+// (d, j) before (d', j'): smaller distance, then smaller index
+__device__ __forceinline__ bool before(float d, int j, float d2, int j2) {
+    return d < d2 || (d == d2 && j < j2);
+}
+
+// One workgroup owns kQ query rows of the dense, zero-padded n_pad x f_pad
+// matrix X and scans every candidate tile.  Thread (ty, tx) computes the
+// 4 x 4 dot products of queries 4ty..4ty+3 with candidates 4tx..4tx+3, each
+// a sequential FMA chain over the features.  Candidates that beat a query's
+// current k-th entry (in (distance, index) order) go to an LDS buffer; after
+// each tile one thread per query sorts its buffer and merges it into the
+// query's sorted list in ld / lj (k entries per query, global memory).  At
+// the end the lists are written out as sqrt(distance) and index, padded
+// with inf / -1.
+__global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
+                                                const float* __restrict__ norms, int64_t n,
+                                                int64_t f_pad, int64_t k,
+                                                float* __restrict__ ld, int32_t* __restrict__ lj,
+                                                float* __restrict__ out_dist,
+                                                int64_t* __restrict__ out_idx) {
+    __shared__ float As[kK][kQ + kPad];
+    __shared__ float Bs[kK][kC + kPad];
+    __shared__ float bd[kQ][kC];
+    __shared__ int bj[kQ][kC];
+    __shared__ int cnt[kQ], len[kQ], thr_j[kQ];
+    __shared__ float thr_d[kQ], qn[kQ];
+
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int64_t q0 = (int64_t)blockIdx.x * kQ;
+    if (tid < kQ) {
+        cnt[tid] = 0;
+        len[tid] = 0;
+        thr_d[tid] = __builtin_inff();
+        thr_j[tid] = 0x7fffffff;
+        qn[tid] = q0 + tid < n ? norms[q0 + tid] : 0.0f;
+    }
+    // loader mapping: thread tid moves 4 consecutive features of one row
+    const int lr = tid >> 2, lk = (tid & 3) * 4;
+    __syncthreads();
+
+    for (int64_t c0 = 0; c0 < n; c0 += kC) {
+        float acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[r][c] = 0.0f;
+        for (int64_t k0 = 0; k0 < f_pad; k0 += kK) {
+            const f4 a = *reinterpret_cast<const f4*>(X + (q0 + lr) * f_pad + k0 + lk);
+            const f4 b = *reinterpret_cast<const f4*>(X + (c0 + lr) * f_pad + k0 + lk);
+            As[lk + 0][lr] = a.x;
+            As[lk + 1][lr] = a.y;
+            As[lk + 2][lr] = a.z;
+            As[lk + 3][lr] = a.w;
+            Bs[lk + 0][lr] = b.x;
+            Bs[lk + 1][lr] = b.y;
+            Bs[lk + 2][lr] = b.z;
+            Bs[lk + 3][lr] = b.w;
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < kK; ++kk) {
+                const f4 av = *reinterpret_cast<const f4*>(&As[kk][ty * 4]);
+                const f4 bv = *reinterpret_cast<const f4*>(&Bs[kk][tx * 4]);
+                const float ar[4] = {av.x, av.y, av.z, av.w};
+                const float bc[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_fmaf(ar[r], bc[c], acc[r][c]);
+            }
+            __syncthreads();
+        }
+        // distances of this tile; candidates ahead of the k-th go to the buffer
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ql = ty * 4 + r;
+            const int64_t q = q0 + ql;
+            if (q >= n) continue;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int64_t j = c0 + tx * 4 + c;
+                if (j >= n || j == q) continue;
+                float d2 = __builtin_fmaf(-2.0f, acc[r][c], qn[ql] + norms[j]);
+                if (d2 < 0.0f) d2 = 0.0f;
+                if (before(d2, (int)j, thr_d[ql], thr_j[ql])) {
+                    const int slot = atomicAdd(&cnt[ql], 1);
+                    bd[ql][slot] = d2;
+                    bj[ql][slot] = (int)j;
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < kQ && cnt[tid] > 0) {
+            const int ql = tid;
+            const int m = cnt[ql];
+            // insertion sort of the buffer
+            for (int a = 1; a < m; ++a) {
+                const float kd = bd[ql][a];
+                const int kj = bj[ql][a];
+                int b = a - 1;
+                while (b >= 0 && before(kd, kj, bd[ql][b], bj[ql][b])) {
+                    bd[ql][b + 1] = bd[ql][b];
+                    bj[ql][b + 1] = bj[ql][b];
+                    --b;
+                }
+                bd[ql][b + 1] = kd;
+                bj[ql][b + 1] = kj;
+            }
+            // merge from the back into the list, keeping the first k
+            float* Ld = ld + (q0 + ql) * k;
+            int32_t* Lj = lj + (q0 + ql) * k;
+            const int64_t L = len[ql];
+            const int64_t total = L + m, keep = total < k ? total : k;
+            int64_t i = L - 1;
+            int b = m - 1;
+            for (int64_t t = 0; t < total - keep; ++t) {
+                if (b < 0 || (i >= 0 && before(bd[ql][b], bj[ql][b], Ld[i], Lj[i]))) --i;
+                else --b;
+            }
+            for (int64_t pos = keep - 1; pos >= 0; --pos) {
+                if (b < 0 || (i >= 0 && before(bd[ql][b], bj[ql][b], Ld[i], Lj[i]))) {
+                    Ld[pos] = Ld[i];
+                    Lj[pos] = Lj[i];
+                    --i;
+                } else {
+                    Ld[pos] = bd[ql][b];
+                    Lj[pos] = bj[ql][b];
+                    --b;
+                }
+            }
+            len[ql] = (int)keep;
+            if (keep == k) {
+                thr_d[ql] = Ld[k - 1];
+                thr_j[ql] = Lj[k - 1];
+            }
+            cnt[ql] = 0;
+        }
+        __syncthreads();
+    }
+    if (tid < kQ && q0 + tid < n) {
+        const int64_t q = q0 + tid;
+        for (int64_t m = 0; m < k; ++m) {
+            if (m < len[tid]) {
+                out_dist[q * k + m] = sqrtf(ld[q * k + m]);
+                out_idx[q * k + m] = lj[q * k + m];
+            } else {
+                out_dist[q * k + m] = __builtin_inff();
+                out_idx[q * k + m] = -1;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+void launch_sum_squared(const int64_t* ip, const float* v, int64_t n, float* norms,
+                        hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_sum_squared, dim3((unsigned)((n * 64 + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, ip, v, n, norms);
+    check_launch("norms launch");
+}
+
+int64_t knn_rows_padded(int64_t n) { return (n + kQ - 1) / kQ * kQ; }
+int64_t knn_features_padded(int64_t f) { return (f + kK - 1) / kK * kK; }
+
+void launch_knn(const float* X, const float* norms, int64_t n, int64_t f_pad, int64_t k,
+                float* ld, int32_t* lj, float* out_dist, int64_t* out_idx, hipStream_t s) {
+    if (n <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_knn, dim3((unsigned)((n + kQ - 1) / kQ)), dim3(kBlock), 0, s, X, norms,
+                       n, f_pad, k, ld, lj, out_dist, out_idx);
+    check_launch("knn launch");
+}
+
+}  // namespace scl_amd

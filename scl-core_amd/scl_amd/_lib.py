@@ -90,6 +90,10 @@ SIGNATURES = {
     "scl_impute_magic": (c_err, [P, P, c_index, c_index, c_index, P]),
     "scl_pseudotime_diffusion": (c_err, [P, c_index, P, c_index, c_index]),
     "scl_algebra_spmm": (c_err, [P, P, c_index, P, c_real, c_real]),
+    # scl_neighbors.h
+    "scl_neighbors_compute_norms": (c_err, [P, P]),
+    "scl_knn": (c_err, [P, P, c_size, c_size, P, P]),
+    "scl_knn_simple": (c_err, [P, c_size, c_size, P, P]),
     # scl_kernel.h, scl_transition.h
     "scl_kernel_compute_matrix": (c_err, [P, P, c_real, ctypes.c_int]),
     "scl_kernel_row_sums": (c_err, [P, P, c_size, c_real, ctypes.c_int]),

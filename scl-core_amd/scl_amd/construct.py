@@ -1,8 +1,9 @@
 """Host-array wrappers of the operator-construction entry points:
-scl_kernel.h (kernel weights of kNN distances) and scl_transition.h
-(row normalisation, symmetrisation, velocity transition, stationary
-distribution), called exactly as a ctypes binding of the reference's
-``scl/binding/c_api/kernel.h`` and ``transition.h`` would."""
+scl_neighbors.h (exact k-nearest neighbours), scl_kernel.h (kernel weights
+of kNN distances) and scl_transition.h (row normalisation, symmetrisation,
+velocity transition, stationary distribution), called exactly as a ctypes
+binding of the reference's ``scl/binding/c_api/neighbors.h``, ``kernel.h``
+and ``transition.h`` would."""
 from __future__ import annotations
 
 import numpy as np
@@ -73,3 +74,26 @@ def stationary_distribution(matrix: SparseMatrixBase, max_iter: int = 1000,
     _lib.check(_lib.load().scl_transition_stationary_distribution(
         matrix.handle, n, out.ctypes.data, max_iter, tol))
     return out
+
+
+def neighbor_norms(matrix: SparseMatrixBase) -> np.ndarray:
+    """scl_neighbors_compute_norms: squared norm of every sample."""
+    out = np.zeros(matrix.shape[0], np.float32)
+    _lib.check(_lib.load().scl_neighbors_compute_norms(matrix.handle, out.ctypes.data))
+    return out
+
+
+def knn(matrix: SparseMatrixBase, k: int, norms_sq: np.ndarray = None):
+    """scl_knn (with norms) or scl_knn_simple: (indices n x k int64,
+    distances n x k float32) of the samples (primary slots)."""
+    n = matrix.shape[0] if matrix._is_csr else matrix.shape[1]
+    idx = np.empty((n, k), np.int64)
+    dist = np.empty((n, k), np.float32)
+    lib = _lib.load()
+    if norms_sq is None:
+        _lib.check(lib.scl_knn_simple(matrix.handle, n, k, idx.ctypes.data, dist.ctypes.data))
+    else:
+        nr = np.ascontiguousarray(norms_sq, dtype=np.float32)
+        _lib.check(lib.scl_knn(matrix.handle, nr.ctypes.data, n, k, idx.ctypes.data,
+                               dist.ctypes.data))
+    return idx, dist

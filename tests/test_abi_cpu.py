@@ -205,6 +205,28 @@ def test_construct_argument_checks_come_first():
     assert not np.any(out)
 
 
+def test_neighbors_argument_checks_come_first():
+    """scl_knn / scl_knn_simple / scl_neighbors_compute_norms reject null and
+    non-positive arguments (neighbors.cpp:25-26, 52-58, 88-90) before any
+    device work."""
+    lib = _lib.load()
+    ip, ix, d = small_csr()
+    P = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    i = np.zeros(9, np.int64)
+    x = np.zeros(9, np.float32)
+    assert lib.scl_neighbors_compute_norms(None, x.ctypes.data) == 4
+    assert lib.scl_neighbors_compute_norms(P.handle, None) == 4
+    assert lib.scl_knn(None, x.ctypes.data, 3, 2, i.ctypes.data, x.ctypes.data) == 4
+    assert lib.scl_knn(P.handle, None, 3, 2, i.ctypes.data, x.ctypes.data) == 4
+    assert lib.scl_knn(P.handle, x.ctypes.data, 3, 2, None, x.ctypes.data) == 4
+    assert lib.scl_knn(P.handle, x.ctypes.data, 3, 2, i.ctypes.data, None) == 4
+    assert lib.scl_knn(P.handle, x.ctypes.data, 0, 2, i.ctypes.data, x.ctypes.data) == 10
+    assert lib.scl_knn(P.handle, x.ctypes.data, 3, 0, i.ctypes.data, x.ctypes.data) == 10
+    assert lib.scl_knn_simple(None, 3, 2, i.ctypes.data, x.ctypes.data) == 4
+    assert lib.scl_knn_simple(P.handle, 3, 0, i.ctypes.data, x.ctypes.data) == 10
+    assert not np.any(i) and not np.any(x)
+
+
 @pytest.mark.skipif(scl_amd.device_count() > 0, reason="host has a HIP device")
 def test_no_device_reports_feature_unavailable():
     ip, ix, d = small_csr()

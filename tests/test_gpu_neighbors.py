@@ -1,0 +1,85 @@
+"""GPU parity of the exact k-nearest-neighbour search (scl_knn,
+scl_knn_simple, scl_neighbors_compute_norms) against the CPU restatement of
+the reference's heap search (oracle/neighbors.c).  Norms and distances are
+bit-identical; the GPU lists are sorted by (distance, index), so the
+restatement's lists are compared after the same sort (the reference's heap
+orders equal distances its own way)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from scl_amd import CscMatrix, CsrMatrix, construct
+from scl_amd._lib import SCLError
+
+pytestmark = pytest.mark.gpu
+
+
+def samples(n, f, density, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ip, ix = [0], []
+    for _ in range(n):
+        cols = np.flatnonzero(rng.random(f) < density)
+        ix.extend(cols)
+        ip.append(len(ix))
+    v = rng.gamma(1.0, 1.0, len(ix)).astype(np.float32)
+    return np.array(ip, np.int64), np.array(ix, np.int64), v
+
+
+def by_distance_then_index(idx, dist):
+    order = np.lexsort((idx, dist), axis=1) if idx.ndim == 1 else None
+    out_i, out_d = idx.copy(), dist.copy()
+    for r in range(idx.shape[0]):
+        o = np.lexsort((idx[r], dist[r]))
+        out_i[r], out_d[r] = idx[r, o], dist[r, o]
+    return out_i, out_d
+
+
+@pytest.mark.parametrize("n,f,k", [(3000, 150, 15), (1000, 37, 1), (777, 64, 70),
+                                   (130, 20, 200)])
+def test_knn_matches_restatement(gpu, n, f, k):
+    ip, ix, v = samples(n, f, 0.15, seed=n + f)
+    M = CsrMatrix.from_arrays(ip, ix, v, (n, f))
+    norms = construct.neighbor_norms(M)
+    assert np.array_equal(norms, orc.norms_f32(ip, v))
+    ri, rd = orc.knn_f32(ip, ix, v, k)
+    ri, rd = by_distance_then_index(ri, rd)
+    for gi, gd in (construct.knn(M, k), construct.knn(M, k, norms)):
+        assert np.array_equal(gd, rd), (n, f, k)
+        assert np.array_equal(gi, ri), (n, f, k)
+    if k >= n:
+        assert np.all(gi[:, n - 1:] == -1) and np.all(np.isinf(gd[:, n - 1:]))
+
+
+def test_knn_samples_of_a_csc_handle_and_ties(gpu):
+    n, f = 500, 40
+    ip, ix, v = samples(n, f, 0.2, seed=5)
+    C = CscMatrix.from_arrays(ip, ix, v, (f, n))     # samples are the columns
+    gi, gd = construct.knn(C, 12)
+    ri, rd = by_distance_then_index(*orc.knn_f32(ip, ix, v, 12))
+    assert np.array_equal(gd, rd) and np.array_equal(gi, ri)
+    # four identical samples: zero distances among them, nearest first
+    ip = np.array([0, 2, 4, 6, 8, 11], np.int64)
+    ix = np.array([0, 2, 0, 2, 0, 2, 0, 2, 0, 1, 2], np.int64)
+    v = np.array([1, 2, 1, 2, 1, 2, 1, 2, 9, 9, 9], np.float32)
+    gi, gd = construct.knn(CsrMatrix.from_arrays(ip, ix, v, (5, 3)), 3)
+    assert list(gi[0]) == [1, 2, 3] and np.all(gd[:4] == 0)
+    assert list(gi[4]) == [0, 1, 2]
+    assert np.allclose(gd[4], np.sqrt(64 + 81 + 49), rtol=1e-6)
+
+
+def test_knn_argument_errors(gpu):
+    ip, ix, v = samples(100, 10, 0.3, seed=6)
+    M = CsrMatrix.from_arrays(ip, ix, v, (100, 10))
+    from scl_amd import _lib
+    lib = _lib.load()
+    out_i = np.zeros(100 * 5, np.int64)
+    out_d = np.zeros(100 * 5, np.float32)
+    nr = np.ones(100, np.float32)
+    assert lib.scl_knn(M.handle, nr.ctypes.data, 99, 5, out_i.ctypes.data,
+                       out_d.ctypes.data) == 11
+    assert lib.scl_knn_simple(M.handle, 100, 0, out_i.ctypes.data, out_d.ctypes.data) == 10
+    with pytest.raises(SCLError) as e:
+        construct.knn(M, 0)
+    assert e.value.code == 10
