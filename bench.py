@@ -36,6 +36,8 @@ Further legs (in "legs"; --legs selects, "c2" alone skips them):
   cl  the other SpMM callers (diffusion imputation, MAGIC, diffusion-
       component pseudotime, Y = alpha A X + beta Y) through the C ABI vs the
       CPU restatement
+  kn  building the operator: exact kNN, kernel weights, row normalisation,
+      stationary distribution, through the C ABI vs the CPU restatement
 """
 from __future__ import annotations
 
@@ -384,6 +386,73 @@ def leg_cl(steps: int, with_cpu: bool = True) -> dict:
     return out
 
 
+def leg_kn(steps: int, with_cpu: bool = True) -> dict:
+    """Building the operator through the C ABI: exact kNN of 20000 sparse
+    samples x 500 features (10% stored) with k = 15 (beside the CPU
+    restatement), the same scan at 60000 samples (GPU only, with its
+    all-pairs FMA rate), then kernel weights, row normalisation and the
+    stationary distribution on knn30_200k."""
+    import numpy as np
+    from scl_amd import CsrMatrix, construct, synth
+    orc = None
+    if with_cpu:
+        from oracle import oracle as orc
+    out = {"workload": "exact kNN (k=15) of sparse samples; construction steps on knn30_200k"}
+    if with_cpu:
+        out["cpu_threads"] = orc.num_threads()
+
+    def samples(n, f, seed):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        mask = rng.random((n, f), dtype=np.float32) < 0.1
+        r, c = np.nonzero(mask)
+        ip = np.zeros(n + 1, np.int64)
+        np.add.at(ip, r + 1, 1)
+        return np.cumsum(ip), c.astype(np.int64), rng.gamma(1.0, 1.0, len(c)).astype(np.float32)
+
+    def timed(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    for n, f, cpu in ((20_000, 500, with_cpu), (60_000, 500, False)):
+        ip, ix, v = samples(n, f, 11 + n)
+        M = CsrMatrix.from_arrays(ip, ix, v, (n, f))
+        g_ms = timed(lambda: construct.knn(M, 15), 1)
+        f_pad = (f + 15) // 16 * 16
+        # every pair's dot product over the padded features, 2 flops per FMA
+        rec = {"gpu_ms": round(g_ms, 1),
+               "tflops_fp32": round(2.0 * n * n * f_pad / (g_ms * 1e-3) / 1e12, 1)}
+        if cpu:
+            t0 = time.perf_counter()
+            orc.knn_f32(ip, ix, v, 15)
+            c_ms = (time.perf_counter() - t0) * 1e3
+            rec.update({"cpu_ms": round(c_ms, 1), "speedup": round(c_ms / g_ms, 1)})
+        out[f"knn_{n}x{f}_k15"] = rec
+    raw = synth.knn_graph(200_000, 30, seed=7)
+    D = CsrMatrix.from_arrays(raw.indptr, raw.indices, raw.data, (raw.n, raw.n))
+    g = raw.normalized()
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
+    cases = {
+        "kernel_gaussian": (lambda: construct.kernel_matrix(D, 0.7, "gaussian"),
+                            lambda: orc.kernel_matrix_f32(raw.data, 0.7, 0)),
+        "row_normalize": (lambda: construct.row_normalize(D),
+                          lambda: orc.row_normalize_f32(raw.indptr, raw.data)),
+        "stationary_1e-6": (lambda: construct.stationary_distribution(P, 1000, 1e-6),
+                            lambda: orc.stationary_f32(g.indptr, g.indices, g.data, 1000, 1e-6)),
+    }
+    for name, (gpu_fn, cpu_fn) in cases.items():
+        rec = {"gpu_ms": round(timed(gpu_fn, max(1, steps // 5)), 2)}
+        if with_cpu:
+            t0 = time.perf_counter()
+            cpu_fn()
+            c_ms = (time.perf_counter() - t0) * 1e3
+            rec.update({"cpu_ms": round(c_ms, 1), "speedup": round(c_ms / rec["gpu_ms"], 1)})
+        out[name] = rec
+    return out
+
+
 def leg_c5(steps: int) -> dict:
     """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
     import numpy as np
@@ -466,7 +535,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk,cl")
+    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk,cl,kn")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="vec,group,unroll,flags for the headline SpMM instead of the "
@@ -561,12 +630,12 @@ def main() -> int:
     del Xd, Yd, Wd, A
     torch.cuda.empty_cache()
     for name, fn in (("c3", leg_c3), ("c4", leg_c4), ("c5", leg_c5), ("mp", leg_mp),
-                     ("wk", leg_wk), ("cl", leg_cl)):
+                     ("wk", leg_wk), ("cl", leg_cl), ("kn", leg_kn)):
         if name not in args.legs.split(","):
             continue
         log(f"leg {name}")
         try:
-            if name in ("wk", "cl"):
+            if name in ("wk", "cl", "kn"):
                 legs[name] = fn(max(3, args.steps // 2), not args.no_cpu_baseline)
             else:
                 legs[name] = fn(max(3, args.steps // 2))

@@ -28,7 +28,6 @@ def samples(n, f, density, seed):
 
 
 def by_distance_then_index(idx, dist):
-    order = np.lexsort((idx, dist), axis=1) if idx.ndim == 1 else None
     out_i, out_d = idx.copy(), dist.copy()
     for r in range(idx.shape[0]):
         o = np.lexsort((idx[r], dist[r]))
