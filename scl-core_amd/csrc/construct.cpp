@@ -94,7 +94,7 @@ SCL_EXPORT scl_error_t scl_kernel_row_sums(scl_sparse_t distances, scl_real_t* s
             fail(SCL_ERROR_DIMENSION_MISMATCH, "KernelRowSums: output buffer too small");
         const int dev = current_device();
         hipStream_t s = library_stream(dev);
-        const HostPattern d(distances, p, s);
+        const HostRows d(distances, p, s);
         DevBuf out(p);
         launch_kernel_row_sums(d.ip.as<int64_t>(), d.v.get(), p,
                                kernel_weights(bandwidth, (int)kernel_type), out.get(), s);
@@ -143,7 +143,7 @@ SCL_EXPORT scl_error_t scl_transition_row_normalize_to_stochastic(scl_sparse_t i
             fail(SCL_ERROR_DIMENSION_MISMATCH, "output_values smaller than the rows' entries");
         const int dev = current_device();
         hipStream_t s = library_stream(dev);
-        const HostPattern m(input, n, s);
+        const HostRows m(input, n, s);
         DevBuf out(need);
         launch_row_normalize(m.ip.as<int64_t>(), m.v.get(), n, out.get(), s);
         download(output_values, out.get(), need, s);

@@ -78,7 +78,7 @@ SCL_EXPORT scl_error_t scl_neighbors_compute_norms(scl_sparse_t matrix, scl_real
         if (p > matrix->rows) fail(SCL_ERROR_DIMENSION_MISMATCH, "Norms size mismatch");
         const int dev = current_device();
         hipStream_t s = library_stream(dev);
-        const HostPattern P(matrix, p, s);
+        const HostRows P(matrix, p, s);
         DevBuf out(p);
         launch_sum_squared(P.ip.as<int64_t>(), P.v.get(), p, out.get(), s);
         download_bytes(norms_sq, out.get(), (size_t)p * 4, s);

@@ -146,4 +146,28 @@ struct HostPattern {
     }
 };
 
+// The row pointer (rebased to 0) and stored values of the first `slots`
+// primary slots on the device, without the column indices: for the
+// per-row operations that never read them.
+struct HostRows {
+    int64_t slots;
+    DevBuf ip, v;
+
+    HostRows(scl_sparse_matrix* X, int64_t slots_, hipStream_t s)
+        : slots(slots_), ip(slots_ + 1, 8), v(HostPattern::count(X, slots_)) {
+        std::lock_guard<std::mutex> lk(X->mu);
+        const int64_t nnz = X->indptr[slots] - X->indptr[0];
+        std::vector<int64_t> rebased((size_t)slots + 1);
+        for (int64_t i = 0; i <= slots; ++i) rebased[i] = X->indptr[i] - X->indptr[0];
+        hip_check(hipMemcpyAsync(ip.as<int64_t>(), rebased.data(), (size_t)(slots + 1) * 8,
+                                 hipMemcpyHostToDevice, s),
+                  "upload row pointer");
+        if (nnz > 0)
+            hip_check(hipMemcpyAsync(v.get(), X->data + X->indptr[0], (size_t)nnz * 4,
+                                     hipMemcpyHostToDevice, s),
+                      "upload values");
+        hip_check(hipStreamSynchronize(s), "upload rows");
+    }
+};
+
 }  // namespace scl_amd
