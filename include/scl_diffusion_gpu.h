@@ -113,6 +113,14 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           increasing column indices in every row and < 2^24 columns;
  *           setting it builds the grouped form on the host (one pass over
  *           the pattern), which follows later value changes
+ *           SCL_GPU_SHAPE_STAGED_GROUPS (vec 4, group 64, transpose 0, no
+ *           occupancy cap): a workgroup computes 64 consecutive rows of the
+ *           visiting order for 256 features; the union of their columns is
+ *           staged through LDS in rounds of 64 X rows, each gathered once
+ *           for the whole group, and every row reads its neighbours there in
+ *           sorted order (bit-identical).  Needs strictly increasing column
+ *           indices in every row; setting it builds the staged pattern on
+ *           the host (the values stay the operator's own)
  * Any row-split SpMM shape gives bit-identical results; only the speed
  * differs.  The
  * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
@@ -124,6 +132,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_OCCUPANCY(c) (((c) & 0xF) << 8)
 #define SCL_GPU_SHAPE_MERGE_PATH(m) (((m) & 0xF) << 12)
 #define SCL_GPU_SHAPE_ROW_GROUPS(r) ((((r) / 4) & 0x3) << 16)
+#define SCL_GPU_SHAPE_STAGED_GROUPS (3 << 16)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

@@ -144,6 +144,11 @@ void DeviceCSR::release_grouped() {
         dev_free(g.grow);
         g = GroupedCsr{};
     }
+    dev_free(staged.gptr);
+    dev_free(staged.ucol);
+    dev_free(staged.upos);
+    dev_free(staged.grow);
+    staged = StagedCsr{};
     // shapes that read the grouped form go with it
     for (auto* m : {&shapes, &t_shapes})
         for (auto it = m->begin(); it != m->end();)
@@ -234,12 +239,14 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->adj_shapes.clear();
         }
     }
-    if (slot->grouped[0].R || slot->grouped[1].R) {
-        // same pattern: re-gather the grouped values; else drop the forms
+    if (slot->grouped[0].R || slot->grouped[1].R || slot->staged.built) {
+        // same pattern: re-gather the grouped values (the staged form holds
+        // no values); else drop the forms
         const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
         bool keep = same_shape;
         for (const GroupedCsr& g : slot->grouped)
             if (g.R && g.fingerprint != fp) keep = false;
+        if (slot->staged.built && slot->staged.fingerprint != fp) keep = false;
         if (keep) {
             for (GroupedCsr& g : slot->grouped)
                 if (g.R) launch_gather_f32(g.gperm, slot->val, g.gval, nnz, nullptr);
@@ -300,6 +307,74 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
     d.t_generation = d.generation;
 }
 
+
+bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d) {
+    StagedCsr& g = d.staged;
+    const int64_t p = h->primary(), q = h->secondary(), nnz = h->nnz;
+    const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
+    if (g.built && g.fingerprint == fp) return true;
+    for (int64_t i = 0; i < p; ++i)
+        for (int64_t k = h->indptr[i] + 1; k < h->indptr[i + 1]; ++k)
+            if (h->indices[k] <= h->indices[k - 1]) return false;
+    if (d.pins > 0)
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "a diffusion plan holds this operator: its staged form cannot be rebuilt");
+    // groups follow the device's visiting order (the locality schedule)
+    std::vector<int32_t> order((size_t)p);
+    if (d.sched) {
+        hip_check(hipMemcpy(order.data(), d.sched, (size_t)p * 4, hipMemcpyDeviceToHost),
+                  "download schedule");
+    } else {
+        for (int64_t i = 0; i < p; ++i) order[i] = (int32_t)i;
+    }
+    const int64_t R = kStagedRows, n_groups = (p + R - 1) / R;
+    std::vector<int32_t> gptr((size_t)n_groups + 1, 0), grow((size_t)n_groups * R, -1);
+    std::vector<int32_t> ucol, upos((size_t)std::max<int64_t>(nnz, 1), 0);
+    std::vector<int32_t> place((size_t)std::max<int64_t>(q, 1), -1);
+    std::vector<int32_t> cols;
+    ucol.reserve((size_t)nnz / 2 + 16);
+    for (int64_t gi = 0; gi < n_groups; ++gi) {
+        cols.clear();
+        for (int64_t r = 0; r < R && gi * R + r < p; ++r) {
+            const int32_t row = order[gi * R + r];
+            grow[gi * R + r] = row;
+            for (int64_t k = h->indptr[row]; k < h->indptr[row + 1]; ++k)
+                cols.push_back((int32_t)h->indices[k]);
+        }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        for (size_t u = 0; u < cols.size(); ++u) place[cols[u]] = (int32_t)u;
+        for (int64_t r = 0; r < R && gi * R + r < p; ++r) {
+            const int32_t row = order[gi * R + r];
+            for (int64_t k = h->indptr[row]; k < h->indptr[row + 1]; ++k)
+                upos[k] = place[h->indices[k]];
+        }
+        for (int32_t c : cols) place[c] = -1;
+        ucol.insert(ucol.end(), cols.begin(), cols.end());
+        if (ucol.size() > (size_t)INT32_MAX) return false;
+        gptr[gi + 1] = (int32_t)ucol.size();
+    }
+    dev_free(g.gptr);
+    dev_free(g.ucol);
+    dev_free(g.upos);
+    dev_free(g.grow);
+    g = StagedCsr{};
+    g.gptr = dev_alloc<int32_t>(gptr.size(), "staged gptr");
+    g.grow = dev_alloc<int32_t>(grow.size(), "staged rows");
+    g.ucol = dev_alloc<int32_t>(std::max<size_t>(ucol.size(), 1), "staged columns");
+    g.upos = dev_alloc<int32_t>(upos.size(), "staged places");
+    hip_check(hipMemcpy(g.gptr, gptr.data(), gptr.size() * 4, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(g.grow, grow.data(), grow.size() * 4, hipMemcpyHostToDevice), "upload");
+    if (!ucol.empty())
+        hip_check(hipMemcpy(g.ucol, ucol.data(), ucol.size() * 4, hipMemcpyHostToDevice),
+                  "upload");
+    hip_check(hipMemcpy(g.upos, upos.data(), upos.size() * 4, hipMemcpyHostToDevice), "upload");
+    g.n_groups = n_groups;
+    g.n_union = (int64_t)ucol.size();
+    g.fingerprint = fp;
+    g.built = true;
+    return true;
+}
 
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s) {
     for (GroupedCsr& g : d.grouped)

@@ -21,7 +21,7 @@ constexpr int64_t kInt32Max = std::numeric_limits<int32_t>::max();
 
 CsrView view_of(const DeviceCSR& d) {
     return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge,
-                   d.grouped};
+                   d.grouped, &d.staged};
 }
 
 CsrView transposed_view_of(const DeviceCSR& d) {
@@ -358,9 +358,10 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
     sh.merge = (flags >> kShapeMergeShift) & kShapeMergeMask;
     const int gcode = (flags >> kShapeGroupShift) & kShapeGroupMask;
-    SCL_API_CHECK(gcode <= 2 && (gcode == 0 || transpose == 0), SCL_ERROR_INVALID_ARGUMENT,
-                  "grouped form: code 1 (4 rows) or 2 (8 rows), forward direction only");
-    sh.rows_per_group = gcode ? 4 * gcode : 0;
+    SCL_API_CHECK(gcode == 0 || transpose == 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "grouped form: code 1 (4 rows), 2 (8 rows) or 3 (LDS-staged groups), "
+                  "forward direction only");
+    sh.rows_per_group = gcode == 3 ? kStagedRows : gcode ? 4 * gcode : 0;
     SCL_API_CHECK(sh.merge <= kMergeMax && (sh.merge == 0 || transpose != 2),
                   SCL_ERROR_INVALID_ARGUMENT,
                   "merge-path code must be 0..8, and 0 for the fused adjoint");
@@ -387,10 +388,15 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
         if (sh.merge > 0)
             ensure_merge_scratch(d, merge_waves(transpose ? d.n_sec : d.n, d.nnz, sh.merge),
                                  n_features);
-        if (sh.rows_per_group > 0 && !ensure_grouped(transition, d, sh.rows_per_group))
+        if (sh.rows_per_group == kStagedRows) {
+            if (!ensure_staged(transition, d))
+                fail(SCL_ERROR_INVALID_ARGUMENT,
+                     "staged form needs strictly increasing column indices in every row");
+        } else if (sh.rows_per_group > 0 && !ensure_grouped(transition, d, sh.rows_per_group)) {
             fail(SCL_ERROR_INVALID_ARGUMENT,
                  "grouped form needs strictly increasing column indices in every row "
                  "and fewer than 2^24 columns");
+        }
         (transpose == 2 ? d.adj_shapes : transpose ? d.t_shapes : d.shapes)[n_features] = sh;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
@@ -438,7 +444,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
                  (sh.scalar_rows ? kShapeScalarRows : 0) |
                  (sh.wg_per_cu << kShapeOccupancyShift) |
                  (sh.merge << kShapeMergeShift) |
-                 ((sh.rows_per_group / 4) << kShapeGroupShift);
+                 ((sh.rows_per_group == kStagedRows ? 3 : sh.rows_per_group / 4)
+                  << kShapeGroupShift);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }
@@ -479,6 +486,18 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         if (transpose == 0 && kGroupedMinRows <= d.n) {
             // row groups share gathers of common neighbours: try 4 and 8
             // rows per wave when the pattern allows the grouped form
+            // LDS-staged groups of kStagedRows rows
+            if (ensure_staged(transition, d))
+                for (int nt = 0; nt < 2; ++nt)
+                    for (int xr = 0; xr < 2; ++xr) {
+                        SpmmShape c;
+                        c.vec = 4;
+                        c.group = 64;
+                        c.rows_per_group = kStagedRows;
+                        c.nontemporal = nt != 0;
+                        c.xcd_rows = xr != 0;
+                        if (spmm_shape_valid(c, n_features, X_dev, Y_dev)) cands.push_back(c);
+                    }
             for (int R : {4, 8}) {
                 if (!ensure_grouped(transition, d, R)) break;
                 for (int vec : {4, 2})

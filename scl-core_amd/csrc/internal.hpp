@@ -105,6 +105,8 @@ struct DeviceCSR {
     // row-grouped forms of the forward operator, R = 4 and R = 8 (built when
     // a grouped shape is set or tuned; values refreshed with the operator's)
     GroupedCsr grouped[2];
+    // LDS-staged groups of kStagedRows rows (pattern only: no values)
+    StagedCsr staged;
 
     ~DeviceCSR();
     void release_grouped();
@@ -153,6 +155,10 @@ void ensure_merge_scratch(DeviceCSR& d, int64_t waves, int64_t F);
 // stale; false when the pattern cannot be grouped (a row with unsorted or
 // repeated column indices, or >= 2^24 columns).
 bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R);
+// Builds d.staged for the handle's current pattern when missing or stale;
+// false when the pattern cannot be staged (a row with unsorted or repeated
+// column indices).
+bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d);
 // Re-gathers the grouped forms' values from d.val (after the device values
 // changed in place), stream-ordered on s.
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s);

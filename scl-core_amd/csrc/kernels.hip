@@ -491,8 +491,12 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     // merge path: lanes over features, one piece of the path per wave
     if (s.merge < 0 || s.merge > kMergeMax || (s.merge > 0 && s.group != 64)) return false;
     // grouped form: lanes over features, 4 or 8 rows per wave, not with merge
-    if (s.rows_per_group != 0 &&
+    if (s.rows_per_group != 0 && s.rows_per_group != kStagedRows &&
         (!(s.rows_per_group == 4 || s.rows_per_group == 8) || s.group != 64 || s.merge))
+        return false;
+    // staged form: 64 lanes x 4 features, its own LDS (no occupancy cap)
+    if (s.rows_per_group == kStagedRows &&
+        (s.vec != 4 || s.group != 64 || s.merge || s.wg_per_cu || s.scalar_rows))
         return false;
     return s.unroll == 4 || s.unroll == 8 || s.unroll == 16;
 }
@@ -619,6 +623,118 @@ void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F
                        (int32_t)rb_per_xcd);
 }
 
+// ---------------------------------------------------- LDS-staged groups --
+
+constexpr int kStagedCols = 64;                       // union columns per round
+constexpr int kStagedRowsPerWave = kStagedRows / (kBlock / 64);
+
+// One workgroup: group g of kStagedRows rows (16 per wave) and one slab of
+// 256 features (4 per lane).  The group's union columns are staged through
+// LDS in rounds of kStagedCols (1 KiB of X each, one coalesced load per
+// column); every row then walks its stored entries whose union place falls
+// in the round, in stored order, reading X from LDS.  A row's (union place,
+// value) pairs sit in two VGPRs, lane l holding entry base + l, and are
+// read with readlane; a row longer than 64 entries reloads the window.
+// Rows with ascending columns visit their entries in stored order, so each
+// output element is the same FMA chain as the row-split kernel.
+template <bool NT>
+__global__ __launch_bounds__(kBlock) void k_spmm_staged(
+    const int32_t* __restrict__ row_ptr, const float* __restrict__ val,
+    const int32_t* __restrict__ upos, const int32_t* __restrict__ gptr,
+    const int32_t* __restrict__ ucol, const int32_t* __restrict__ grow,
+    const float* __restrict__ X, float* __restrict__ Y, int32_t n_groups, int32_t F,
+    int32_t rb_per_xcd) {
+    constexpr int RW = kStagedRowsPerWave;
+    __shared__ f4 stage[kStagedCols][64];
+    int slab, g;
+    if (!block_coords(n_groups, rb_per_xcd, slab, g)) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int f = slab * 256 + lane * 4;
+    const bool active = f < F;
+    // lanes past the last feature stage and read column 0 and store nothing
+    const float* Xf = X + (active ? f : 0);
+    int row[RW], k[RW], e[RW], wb[RW], wpos[RW];
+    float wval[RW];
+    float acc[RW][4];
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+        const int r =
+            __builtin_amdgcn_readfirstlane(grow[(int64_t)g * kStagedRows + wave * RW + i]);
+        row[i] = r;
+        k[i] = r >= 0 ? __builtin_amdgcn_readfirstlane(row_ptr[r]) : 0;
+        e[i] = r >= 0 ? __builtin_amdgcn_readfirstlane(row_ptr[r + 1]) : 0;
+        wb[i] = k[i];
+        const int idx = k[i] + lane;
+        wpos[i] = idx < e[i] ? upos[idx] : 0x7fffffff;
+        wval[i] = idx < e[i] ? val[idx] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[i][c] = 0.0f;
+    }
+    const int ub = __builtin_amdgcn_readfirstlane(gptr[g]);
+    const int un = __builtin_amdgcn_readfirstlane(gptr[g + 1]) - ub;
+    for (int cb = 0; cb < un; cb += kStagedCols) {
+        const int cn = min(kStagedCols, un - cb);
+        // this wave stages columns [16 wave, 16 wave + 16) of the round
+        const int c0 = wave * 16;
+        const int mine = (lane < 16 && c0 + lane < cn) ? ucol[ub + cb + c0 + lane] : 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            if (c0 + t < cn) {
+                const int j = __builtin_amdgcn_readlane(mine, t);
+                stage[c0 + t][lane] = *reinterpret_cast<const f4*>(Xf + (size_t)j * F);
+            }
+        }
+        __syncthreads();
+        const int lim = cb + cn;
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            while (k[i] < e[i]) {
+                int o = k[i] - wb[i];
+                if (o == 64) {
+                    wb[i] = k[i];
+                    const int idx = k[i] + lane;
+                    wpos[i] = idx < e[i] ? upos[idx] : 0x7fffffff;
+                    wval[i] = idx < e[i] ? val[idx] : 0.0f;
+                    o = 0;
+                }
+                const int p = __builtin_amdgcn_readlane(wpos[i], o);
+                if (p >= lim) break;
+                const float v =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), o));
+                fma_vec(v, stage[p - cb][lane], acc[i]);
+                ++k[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i)
+            if (row[i] >= 0) store_vec<4, NT>(Y + (size_t)row[i] * F + f, acc[i]);
+    }
+}
+
+void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
+                        const SpmmShape& shape, hipStream_t s) {
+    const StagedCsr* G = P.staged;
+    if (!G || !G->built || G->n_groups * kStagedRows < P.n)
+        throw std::runtime_error("staged form of the operator missing for this launch");
+    const int64_t n_slabs = (F + 255) / 256;
+    const int64_t rb_per_xcd = shape.xcd_rows ? (G->n_groups + kNumXcd - 1) / kNumXcd : 0;
+    const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : G->n_groups * n_slabs;
+    if (nb > INT32_MAX) throw std::runtime_error("staged spmm grid too large");
+    if (shape.nontemporal)
+        hipLaunchKernelGGL(k_spmm_staged<true>, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                           P.row_ptr, P.val, G->upos, G->gptr, G->ucol, G->grow, X, Y,
+                           (int32_t)G->n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+    else
+        hipLaunchKernelGGL(k_spmm_staged<false>, dim3((unsigned)nb), dim3(kBlock), 0, s,
+                           P.row_ptr, P.val, G->upos, G->gptr, G->ucol, G->grow, X, Y,
+                           (int32_t)G->n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+    check_launch("staged spmm launch");
+}
+
 void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
                          const SpmmShape& shape, hipStream_t s) {
     const int R = shape.rows_per_group;
@@ -656,6 +772,10 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     if (P.n <= 0 || F <= 0) return;
     if (shape.merge > 0) {
         launch_spmm_merge(P, X, Y, F, shape, s);
+        return;
+    }
+    if (shape.rows_per_group == kStagedRows) {
+        launch_spmm_staged(P, X, Y, F, shape, s);
         return;
     }
     if (shape.rows_per_group > 0) {

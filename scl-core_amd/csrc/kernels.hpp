@@ -38,6 +38,24 @@ struct GroupedCsr {
     bool schedule = false;        // built over the locality order
 };
 
+// LDS-staged form of the forward operator: groups of kStagedRows
+// consecutive rows (in the visiting order) with the sorted union of their
+// columns.  A workgroup stages the union's X rows through LDS in rounds and
+// every row of the group reads its neighbours there; each row keeps its own
+// stored-order FMA chain, reading its values from the operator (any values
+// with the same pattern work).  Needs strictly increasing columns per row.
+constexpr int kStagedRows = 64;
+struct StagedCsr {
+    bool built = false;
+    int64_t n_groups = 0;
+    int64_t n_union = 0;
+    int32_t* gptr = nullptr;   // n_groups + 1: first union column of each group
+    int32_t* ucol = nullptr;   // n_union: each group's distinct columns, ascending
+    int32_t* upos = nullptr;   // nnz: every stored entry's place in its group's union
+    int32_t* grow = nullptr;   // n_groups * kStagedRows: the rows, -1 past the end
+    uint64_t fingerprint = 0;  // pattern it was built for
+};
+
 struct CsrView {
     int32_t n;            // rows of the applied operator
     int32_t n_sec;        // columns
@@ -48,6 +66,7 @@ struct CsrView {
     const int32_t* sched;  // row visiting order or null
     const MergeScratch* merge = nullptr;  // for merge-path shapes
     const GroupedCsr* grouped = nullptr;  // [0]: R = 4, [1]: R = 8
+    const StagedCsr* staged = nullptr;    // LDS-staged groups of kStagedRows rows
 };
 
 // Launch shape of the row-split SpMM (see kernels.hip).
@@ -63,7 +82,8 @@ struct SpmmShape {
                         // reserving LDS the kernel does not use
     int merge = 0;      // 0: row-split; m > 0: merge path, 256 << (m - 1)
                         // path items (row ends + nonzeros) per wave
-    int rows_per_group = 0;  // 0: one row per lane group; 4 or 8: grouped form
+    int rows_per_group = 0;  // 0: one row per lane group; 4 or 8: grouped form;
+                             // kStagedRows: LDS-staged form
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
@@ -80,7 +100,8 @@ int occupancy_lds_bytes(int wg_per_cu);
 constexpr int kShapeMergeShift = 12;
 constexpr int kShapeMergeMask = 0xF;
 constexpr int kMergeMax = 8;
-// bits 16..17: grouped form (0 = none, 1 = 4 rows, 2 = 8 rows per group)
+// bits 16..17: grouped form (0 = none, 1 = 4 rows, 2 = 8 rows per group,
+// 3 = LDS-staged groups of kStagedRows rows)
 constexpr int kShapeGroupShift = 16;
 constexpr int kShapeGroupMask = 0x3;
 // Waves a merge-path launch with code m uses for n rows and nnz nonzeros.

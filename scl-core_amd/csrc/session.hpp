@@ -56,7 +56,7 @@ struct Session {
         if (P->wrapped) P->generation++;
         d = &device_csr(P, dev);
         v = CsrView{d->n, d->n_sec, d->nnz, d->row_ptr, d->col, d->val, d->sched, &d->merge,
-                    d->grouped};
+                    d->grouped, &d->staged};
         s = library_stream(dev);
     }
 
@@ -73,13 +73,16 @@ struct Session {
         launch_spmm_f32(v, X, Y, F, shape(F, X, Y), s);
     }
     // Y = P' X for the operator's pattern with other values (same order as
-    // the stored ones): no row-grouped form, whose values are a copy.
+    // the stored ones).
     void spmm_values(const float* val, const float* X, float* Y, int64_t F) {
         CsrView w = v;
         w.val = val;
         w.grouped = nullptr;
         SpmmShape sh = shape(F, X, Y);
-        if (sh.rows_per_group) sh = spmm_default_shape(F, X, Y);
+        // the staged form reads the view's values; the grouped forms hold
+        // copies of the operator's
+        if (sh.rows_per_group && sh.rows_per_group != kStagedRows)
+            sh = spmm_default_shape(F, X, Y);
         launch_spmm_f32(w, X, Y, F, sh, s);
     }
     void upload(float* dst, const float* src, int64_t m) {

@@ -293,3 +293,59 @@ def test_row_grouped_needs_sorted_rows(gpu):
     torch_ops.set_shape(B, 8, 4, 64, 8, rows_per_group=4)
     with pytest.raises(Exception):   # forward direction only
         torch_ops.set_shape(B, 8, 4, 64, 8, rows_per_group=4, transpose=True)
+
+
+@pytest.mark.parametrize("F", [4, 260, 2000])
+def test_staged_groups_are_bit_identical(gpu, F):
+    """LDS-staged groups of 64 rows: the union of a group's columns is
+    staged through LDS and each row reads its neighbours there in sorted
+    order, so the result is the restatement's bit for bit -- with and
+    without the locality schedule, for rows longer than one 64-entry window,
+    through t steps, and after the values change in place."""
+    import torch
+    from scl_amd import torch_ops
+    cases = [(20_000, 12), (3_001, 12), (1_500, 150)]   # schedule; none; long rows
+    for n, k in cases:
+        g = synth.knn_graph(n, k, seed=F + n) if k < 100 else synth.uniform_knn(n, k, seed=F)
+        A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+        ip, ix, p = A.export()
+        X = synth.normal(n, F, seed=F)
+        Xd = torch.from_numpy(X).cuda()
+        ref = orc.spmm_step_f32(ip, ix, p, X)
+        for nt, xr in ((False, False), (True, True)):
+            torch_ops.set_shape(A, F, 4, 64, 8, rows_per_group=64, nontemporal=nt,
+                                xcd_rows=xr)
+            assert torch_ops.get_shape(A, F)["rows_per_group"] == 64
+            Y = torch_ops.spmm(A, Xd).cpu().numpy()
+            assert np.array_equal(Y, ref), (n, k, nt, xr)
+        # values change in place: the staged form reads the operator's values
+        diffusion.compute_transition_matrix(A)
+        ip, ix, p = A.export()
+        Y = torch_ops.diffuse(A, Xd, 3).cpu().numpy()
+        assert torch_ops.get_shape(A, F)["rows_per_group"] == 64
+        assert np.array_equal(Y, orc.diffuse_matrix_f32(ip, ix, p, X, 3)), (n, k)
+
+
+def test_staged_groups_with_other_values_and_errors(gpu):
+    """MAGIC runs the operator's pattern with its own weights: under a staged
+    shape the kernel reads those weights, bit-identical to the restatement.
+    The staged form needs sorted rows, vec 4 and the forward direction."""
+    from scl_amd import callers, torch_ops
+    n, G = 20_000, 64
+    g = synth.knn_graph(n, 10, seed=23)
+    A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    torch_ops.set_shape(A, G, 4, 64, 8, rows_per_group=64)
+    X = synth.normal(n, G, seed=24)
+    ip = np.arange(0, n * G + 1, G, dtype=np.int64)
+    ix = np.tile(np.arange(G, dtype=np.int64), n)
+    M = CsrMatrix.from_arrays(ip, ix, X.ravel(), (n, G))
+    got = callers.impute_magic(M, A, n, G, 2)
+    assert np.array_equal(got, orc.magic_f32(g.indptr, g.indices, g.data, X, 2))
+    bad = CsrMatrix.from_arrays(np.array([0, 2, 3], np.int64), np.array([1, 0, 1], np.int64),
+                                np.ones(3, np.float32), (2, 2))
+    with pytest.raises(Exception):
+        torch_ops.set_shape(bad, 8, 4, 64, 8, rows_per_group=64)
+    with pytest.raises(Exception):
+        torch_ops.set_shape(A, 8, 2, 64, 8, rows_per_group=64)
+    with pytest.raises(Exception):
+        torch_ops.set_shape(A, 8, 4, 64, 8, rows_per_group=64, transpose=True)
