@@ -625,19 +625,20 @@ void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F
 
 // ---------------------------------------------------- LDS-staged groups --
 
-constexpr int kStagedCols = 64;                       // union columns per round
 constexpr int kStagedRowsPerWave = kStagedRows / (kBlock / 64);
 
 // One workgroup: group g of kStagedRows rows (16 per wave) and one slab of
 // 256 features (4 per lane).  The group's union columns are staged through
-// LDS in rounds of kStagedCols (1 KiB of X each, one coalesced load per
+// LDS in rounds of C columns (1 KiB of X each, one coalesced load per
 // column); every row then walks its stored entries whose union place falls
 // in the round, in stored order, reading X from LDS.  A row's (union place,
-// value) pairs sit in two VGPRs, lane l holding entry base + l, and are
-// read with readlane; a row longer than 64 entries reloads the window.
-// Rows with ascending columns visit their entries in stored order, so each
-// output element is the same FMA chain as the row-split kernel.
-template <bool NT>
+// value) pairs sit in two VGPRs, lane l holding entry base + l: a ballot
+// counts the round's entries of the row, their LDS reads are issued four at
+// a time, and the pairs are read with readlane; a row longer than 64
+// entries reloads the window.  Rows with ascending columns visit their
+// entries in stored order, so each output element is the same FMA chain as
+// the row-split kernel.
+template <int C, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     const int32_t* __restrict__ row_ptr, const float* __restrict__ val,
     const int32_t* __restrict__ upos, const int32_t* __restrict__ gptr,
@@ -645,7 +646,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     const float* __restrict__ X, float* __restrict__ Y, int32_t n_groups, int32_t F,
     int32_t rb_per_xcd) {
     constexpr int RW = kStagedRowsPerWave;
-    __shared__ f4 stage[kStagedCols][64];
+    constexpr int CW = C / (kBlock / 64);  // columns each wave stages per round
+    __shared__ f4 stage[C][64];
     int slab, g;
     if (!block_coords(n_groups, rb_per_xcd, slab, g)) return;
     const int lane = threadIdx.x & 63;
@@ -654,14 +656,13 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     const bool active = f < F;
     // lanes past the last feature stage and read column 0 and store nothing
     const float* Xf = X + (active ? f : 0);
-    int row[RW], k[RW], e[RW], wb[RW], wpos[RW];
+    const int32_t* rows = grow + (int64_t)g * kStagedRows + wave * RW;
+    int k[RW], e[RW], wb[RW], wpos[RW];
     float wval[RW];
     float acc[RW][4];
 #pragma unroll
     for (int i = 0; i < RW; ++i) {
-        const int r =
-            __builtin_amdgcn_readfirstlane(grow[(int64_t)g * kStagedRows + wave * RW + i]);
-        row[i] = r;
+        const int r = __builtin_amdgcn_readfirstlane(rows[i]);
         k[i] = r >= 0 ? __builtin_amdgcn_readfirstlane(row_ptr[r]) : 0;
         e[i] = r >= 0 ? __builtin_amdgcn_readfirstlane(row_ptr[r + 1]) : 0;
         wb[i] = k[i];
@@ -673,16 +674,20 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     }
     const int ub = __builtin_amdgcn_readfirstlane(gptr[g]);
     const int un = __builtin_amdgcn_readfirstlane(gptr[g + 1]) - ub;
-    for (int cb = 0; cb < un; cb += kStagedCols) {
-        const int cn = min(kStagedCols, un - cb);
-        // this wave stages columns [16 wave, 16 wave + 16) of the round
-        const int c0 = wave * 16;
-        const int mine = (lane < 16 && c0 + lane < cn) ? ucol[ub + cb + c0 + lane] : 0;
+    for (int cb = 0; cb < un; cb += C) {
+        const int cn = min(C, un - cb);
+        // this wave stages columns [CW wave, CW wave + CW) of the round
+        const int c0 = wave * CW;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            if (c0 + t < cn) {
-                const int j = __builtin_amdgcn_readlane(mine, t);
-                stage[c0 + t][lane] = *reinterpret_cast<const f4*>(Xf + (size_t)j * F);
+        for (int t0 = 0; t0 < CW; t0 += 64) {
+            const int mine =
+                (t0 + lane < CW && c0 + t0 + lane < cn) ? ucol[ub + cb + c0 + t0 + lane] : 0;
+#pragma unroll
+            for (int t = 0; t < (CW < 64 ? CW : 64); ++t) {
+                if (c0 + t0 + t < cn) {
+                    const int j = __builtin_amdgcn_readlane(mine, t);
+                    stage[c0 + t0 + t][lane] = *reinterpret_cast<const f4*>(Xf + (size_t)j * F);
+                }
             }
         }
         __syncthreads();
@@ -698,23 +703,53 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
                     wval[i] = idx < e[i] ? val[idx] : 0.0f;
                     o = 0;
                 }
-                const int p = __builtin_amdgcn_readlane(wpos[i], o);
-                if (p >= lim) break;
-                const float v =
-                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), o));
-                fma_vec(v, stage[p - cb][lane], acc[i]);
-                ++k[i];
+                // entries o.. of the window that fall in this round (the
+                // window is ascending, so they are consecutive)
+                const int m = __popcll(__ballot(lane >= o && wpos[i] < lim));
+                for (int t = 0; t < m; t += 4) {
+                    f4 x[4];
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int oo = o + min(t + u, m - 1);
+                        const int p = __builtin_amdgcn_readlane(wpos[i], oo);
+                        v[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), oo));
+                        x[u] = stage[p - cb][lane];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (t + u < m) fma_vec(v[u], x[u], acc[i]);
+                }
+                k[i] += m;
+                if (o + m < 64) break;  // the row's next entry is in a later round
             }
         }
         __syncthreads();
     }
     if (active) {
 #pragma unroll
-        for (int i = 0; i < RW; ++i)
-            if (row[i] >= 0) store_vec<4, NT>(Y + (size_t)row[i] * F + f, acc[i]);
+        for (int i = 0; i < RW; ++i) {
+            const int r = __builtin_amdgcn_readfirstlane(rows[i]);
+            if (r >= 0) store_vec<4, NT>(Y + (size_t)r * F + f, acc[i]);
+        }
     }
 }
 
+template <int C>
+void launch_staged_c(const CsrView& P, const StagedCsr& G, const float* X, float* Y,
+                     int64_t F, bool nt, int64_t nb, int64_t rb_per_xcd, hipStream_t s) {
+    if (nt)
+        hipLaunchKernelGGL((k_spmm_staged<C, true>), dim3((unsigned)nb), dim3(kBlock), 0, s,
+                           P.row_ptr, P.val, G.upos, G.gptr, G.ucol, G.grow, X, Y,
+                           (int32_t)G.n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+    else
+        hipLaunchKernelGGL((k_spmm_staged<C, false>), dim3((unsigned)nb), dim3(kBlock), 0, s,
+                           P.row_ptr, P.val, G.upos, G.gptr, G.ucol, G.grow, X, Y,
+                           (int32_t)G.n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+}
+
+// unroll selects the round width: 8 -> 64 union columns (64 KiB of LDS, two
+// workgroups per CU), 16 -> 128 columns (128 KiB, one workgroup per CU)
 void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
                         const SpmmShape& shape, hipStream_t s) {
     const StagedCsr* G = P.staged;
@@ -724,14 +759,10 @@ void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
     const int64_t rb_per_xcd = shape.xcd_rows ? (G->n_groups + kNumXcd - 1) / kNumXcd : 0;
     const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : G->n_groups * n_slabs;
     if (nb > INT32_MAX) throw std::runtime_error("staged spmm grid too large");
-    if (shape.nontemporal)
-        hipLaunchKernelGGL(k_spmm_staged<true>, dim3((unsigned)nb), dim3(kBlock), 0, s,
-                           P.row_ptr, P.val, G->upos, G->gptr, G->ucol, G->grow, X, Y,
-                           (int32_t)G->n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+    if (shape.unroll >= 16)
+        launch_staged_c<128>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
     else
-        hipLaunchKernelGGL(k_spmm_staged<false>, dim3((unsigned)nb), dim3(kBlock), 0, s,
-                           P.row_ptr, P.val, G->upos, G->gptr, G->ucol, G->grow, X, Y,
-                           (int32_t)G->n_groups, (int32_t)F, (int32_t)rb_per_xcd);
+        launch_staged_c<64>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
     check_launch("staged spmm launch");
 }
 
