@@ -676,18 +676,21 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     const int un = __builtin_amdgcn_readfirstlane(gptr[g + 1]) - ub;
     for (int cb = 0; cb < un; cb += C) {
         const int cn = min(C, un - cb);
-        // this wave stages columns [CW wave, CW wave + CW) of the round
+        // this wave stages columns [CW wave, CW wave + CW) of the round; past
+        // the round's last column it loads that column again (slots never
+        // read), so the loads issue back to back without branches
         const int c0 = wave * CW;
+        if (c0 < cn) {
 #pragma unroll
-        for (int t0 = 0; t0 < CW; t0 += 64) {
-            const int mine =
-                (t0 + lane < CW && c0 + t0 + lane < cn) ? ucol[ub + cb + c0 + t0 + lane] : 0;
+            for (int t0 = 0; t0 < CW; t0 += 64) {
+                const int mine = ucol[ub + cb + min(c0 + t0 + (lane % CW), cn - 1)];
+                f4 xs[CW < 64 ? CW : 64];
 #pragma unroll
-            for (int t = 0; t < (CW < 64 ? CW : 64); ++t) {
-                if (c0 + t0 + t < cn) {
-                    const int j = __builtin_amdgcn_readlane(mine, t);
-                    stage[c0 + t0 + t][lane] = *reinterpret_cast<const f4*>(Xf + (size_t)j * F);
-                }
+                for (int t = 0; t < (CW < 64 ? CW : 64); ++t)
+                    xs[t] = *reinterpret_cast<const f4*>(
+                        Xf + (size_t)__builtin_amdgcn_readlane(mine, t) * F);
+#pragma unroll
+                for (int t = 0; t < (CW < 64 ? CW : 64); ++t) stage[c0 + t0 + t][lane] = xs[t];
             }
         }
         __syncthreads();
