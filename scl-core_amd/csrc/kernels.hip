@@ -625,28 +625,29 @@ void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F
 
 // ---------------------------------------------------- LDS-staged groups --
 
-constexpr int kStagedRowsPerWave = kStagedRows / (kBlock / 64);
-
-// One workgroup: group g of kStagedRows rows (16 per wave) and one slab of
-// 256 features (4 per lane).  The group's union columns are staged through
-// LDS in rounds of C columns (1 KiB of X each, one coalesced load per
-// column); every row then walks its stored entries whose union place falls
-// in the round, in stored order, reading X from LDS.  A row's (union place,
-// value) pairs sit in two VGPRs, lane l holding entry base + l: a ballot
-// counts the round's entries of the row, their LDS reads are issued four at
-// a time, and the pairs are read with readlane; a row longer than 64
-// entries reloads the window.  Rows with ascending columns visit their
-// entries in stored order, so each output element is the same FMA chain as
-// the row-split kernel.
-template <int C, bool NT>
-__global__ __launch_bounds__(kBlock) void k_spmm_staged(
+// One workgroup of W waves: group g of kStagedRows rows (kStagedRows / W per
+// wave) and one slab of 256 features (4 per lane).  The group's union
+// columns are staged through LDS in rounds of C columns (1 KiB of X each,
+// one coalesced load per column, issued back to back); every row then takes
+// its stored entries whose union place falls in the round, in stored order,
+// reading X from LDS.  A row's (union place, value) pairs sit in two VGPRs,
+// lane l holding entry base + l: a ballot counts the round's entries of each
+// row, and the rows of a wave advance in lockstep, entry t of every row at
+// once, with a select keeping rows that have no entry t unchanged (their
+// LDS reads all in flight together); a row longer than one 64-entry window
+// finishes its round entry by entry after a reload.  Rows with ascending
+// columns visit their entries in stored order, so each output element is
+// the same FMA chain as the row-split kernel.
+template <int C, int W, bool NT>
+__global__ __launch_bounds__(64 * W) void k_spmm_staged(
     const int32_t* __restrict__ row_ptr, const float* __restrict__ val,
     const int32_t* __restrict__ upos, const int32_t* __restrict__ gptr,
     const int32_t* __restrict__ ucol, const int32_t* __restrict__ grow,
     const float* __restrict__ X, float* __restrict__ Y, int32_t n_groups, int32_t F,
     int32_t rb_per_xcd) {
-    constexpr int RW = kStagedRowsPerWave;
-    constexpr int CW = C / (kBlock / 64);  // columns each wave stages per round
+    constexpr int RW = kStagedRows / W;   // rows per wave
+    constexpr int CW = C / W;             // columns each wave stages per round
+    static_assert(CW <= 64 && RW * W == kStagedRows, "staged layout");
     __shared__ f4 stage[C][64];
     int slab, g;
     if (!block_coords(n_groups, rb_per_xcd, slab, g)) return;
@@ -681,50 +682,58 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
         // read), so the loads issue back to back without branches
         const int c0 = wave * CW;
         if (c0 < cn) {
+            const int mine = ucol[ub + cb + min(c0 + (lane % CW), cn - 1)];
+            f4 xs[CW];
 #pragma unroll
-            for (int t0 = 0; t0 < CW; t0 += 64) {
-                const int mine = ucol[ub + cb + min(c0 + t0 + (lane % CW), cn - 1)];
-                f4 xs[CW < 64 ? CW : 64];
+            for (int t = 0; t < CW; ++t)
+                xs[t] = *reinterpret_cast<const f4*>(
+                    Xf + (size_t)__builtin_amdgcn_readlane(mine, t) * F);
 #pragma unroll
-                for (int t = 0; t < (CW < 64 ? CW : 64); ++t)
-                    xs[t] = *reinterpret_cast<const f4*>(
-                        Xf + (size_t)__builtin_amdgcn_readlane(mine, t) * F);
-#pragma unroll
-                for (int t = 0; t < (CW < 64 ? CW : 64); ++t) stage[c0 + t0 + t][lane] = xs[t];
-            }
+            for (int t = 0; t < CW; ++t) stage[c0 + t][lane] = xs[t];
         }
         __syncthreads();
         const int lim = cb + cn;
+        // entries of each row in this round within its current window
+        int m[RW], o[RW];
+        int tmax = 0;
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            while (k[i] < e[i]) {
-                int o = k[i] - wb[i];
-                if (o == 64) {
-                    wb[i] = k[i];
-                    const int idx = k[i] + lane;
-                    wpos[i] = idx < e[i] ? upos[idx] : 0x7fffffff;
-                    wval[i] = idx < e[i] ? val[idx] : 0.0f;
-                    o = 0;
-                }
-                // entries o.. of the window that fall in this round (the
-                // window is ascending, so they are consecutive)
-                const int m = __popcll(__ballot(lane >= o && wpos[i] < lim));
-                for (int t = 0; t < m; t += 4) {
-                    f4 x[4];
-                    float v[4];
+            o[i] = k[i] - wb[i];
+            m[i] = o[i] < 64 ? (int)__popcll(__ballot(lane >= o[i] && wpos[i] < lim)) : 0;
+            tmax = max(tmax, m[i]);
+        }
+        for (int t = 0; t < tmax; ++t) {
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int oo = o + min(t + u, m - 1);
-                        const int p = __builtin_amdgcn_readlane(wpos[i], oo);
-                        v[u] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), oo));
-                        x[u] = stage[p - cb][lane];
-                    }
+            for (int i = 0; i < RW; ++i) {
+                const bool on = t < m[i];
+                const int oo = on ? o[i] + t : 0;
+                const int p = __builtin_amdgcn_readlane(wpos[i], oo);
+                const float v =
+                    __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), oo));
+                const f4 x = stage[on ? p - cb : 0][lane];
+                float nx[4] = {acc[i][0], acc[i][1], acc[i][2], acc[i][3]};
+                fma_vec(v, x, nx);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (t + u < m) fma_vec(v[u], x[u], acc[i]);
+                for (int c = 0; c < 4; ++c) acc[i][c] = on ? nx[c] : acc[i][c];
+            }
+        }
+        // advance; a row whose window ran out keeps going entry by entry
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            k[i] += m[i];
+            while (k[i] < e[i] && k[i] - wb[i] == 64) {
+                wb[i] = k[i];
+                const int idx = k[i] + lane;
+                wpos[i] = idx < e[i] ? upos[idx] : 0x7fffffff;
+                wval[i] = idx < e[i] ? val[idx] : 0.0f;
+                const int mm = (int)__popcll(__ballot(wpos[i] < lim));
+                for (int t = 0; t < mm; ++t) {
+                    const int p = __builtin_amdgcn_readlane(wpos[i], t);
+                    const float v =
+                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wval[i]), t));
+                    fma_vec(v, stage[p - cb][lane], acc[i]);
                 }
-                k[i] += m;
-                if (o + m < 64) break;  // the row's next entry is in a later round
+                k[i] += mm;
             }
         }
         __syncthreads();
@@ -738,21 +747,22 @@ __global__ __launch_bounds__(kBlock) void k_spmm_staged(
     }
 }
 
-template <int C>
+template <int C, int W>
 void launch_staged_c(const CsrView& P, const StagedCsr& G, const float* X, float* Y,
                      int64_t F, bool nt, int64_t nb, int64_t rb_per_xcd, hipStream_t s) {
     if (nt)
-        hipLaunchKernelGGL((k_spmm_staged<C, true>), dim3((unsigned)nb), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((k_spmm_staged<C, W, true>), dim3((unsigned)nb), dim3(64 * W), 0, s,
                            P.row_ptr, P.val, G.upos, G.gptr, G.ucol, G.grow, X, Y,
                            (int32_t)G.n_groups, (int32_t)F, (int32_t)rb_per_xcd);
     else
-        hipLaunchKernelGGL((k_spmm_staged<C, false>), dim3((unsigned)nb), dim3(kBlock), 0, s,
-                           P.row_ptr, P.val, G.upos, G.gptr, G.ucol, G.grow, X, Y,
+        hipLaunchKernelGGL((k_spmm_staged<C, W, false>), dim3((unsigned)nb), dim3(64 * W), 0,
+                           s, P.row_ptr, P.val, G.upos, G.gptr, G.ucol, G.grow, X, Y,
                            (int32_t)G.n_groups, (int32_t)F, (int32_t)rb_per_xcd);
 }
 
-// unroll selects the round width: 8 -> 64 union columns (64 KiB of LDS, two
-// workgroups per CU), 16 -> 128 columns (128 KiB, one workgroup per CU)
+// unroll selects the layout: 4 -> 8 waves, rounds of 64 union columns
+// (64 KiB of LDS, two workgroups per CU); 8 -> 4 waves, 64 columns;
+// 16 -> 8 waves, 128 columns (128 KiB, one workgroup per CU)
 void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
                         const SpmmShape& shape, hipStream_t s) {
     const StagedCsr* G = P.staged;
@@ -763,9 +773,11 @@ void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
     const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : G->n_groups * n_slabs;
     if (nb > INT32_MAX) throw std::runtime_error("staged spmm grid too large");
     if (shape.unroll >= 16)
-        launch_staged_c<128>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
+        launch_staged_c<128, 8>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
+    else if (shape.unroll >= 8)
+        launch_staged_c<64, 4>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
     else
-        launch_staged_c<64>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
+        launch_staged_c<64, 8>(P, *G, X, Y, F, shape.nontemporal, nb, rb_per_xcd, s);
     check_launch("staged spmm launch");
 }
 

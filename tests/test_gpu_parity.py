@@ -312,7 +312,7 @@ def test_staged_groups_are_bit_identical(gpu, F):
         X = synth.normal(n, F, seed=F)
         Xd = torch.from_numpy(X).cuda()
         ref = orc.spmm_step_f32(ip, ix, p, X)
-        for nt, xr, u in ((False, False, 8), (True, True, 8), (True, False, 16)):
+        for nt, xr, u in ((False, False, 4), (True, True, 8), (True, False, 16)):
             torch_ops.set_shape(A, F, 4, 64, u, rows_per_group=64, nontemporal=nt,
                                 xcd_rows=xr)
             assert torch_ops.get_shape(A, F)["rows_per_group"] == 64
