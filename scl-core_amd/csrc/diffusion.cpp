@@ -486,18 +486,18 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         if (transpose == 0 && kGroupedMinRows <= d.n) {
             // row groups share gathers of common neighbours: try 4 and 8
             // rows per wave when the pattern allows the grouped form
-            // LDS-staged groups of kStagedRows rows
+            // LDS-staged groups of kStagedRows rows (8 waves, rounds of 64
+            // union columns: the fastest staged layout)
             if (ensure_staged(transition, d))
-                for (int nt = 0; nt < 2; ++nt)
-                    for (int xr = 0; xr < 2; ++xr) {
-                        SpmmShape c;
-                        c.vec = 4;
-                        c.group = 64;
-                        c.rows_per_group = kStagedRows;
-                        c.nontemporal = nt != 0;
-                        c.xcd_rows = xr != 0;
-                        if (spmm_shape_valid(c, n_features, X_dev, Y_dev)) cands.push_back(c);
-                    }
+                for (int nt = 0; nt < 2; ++nt) {
+                    SpmmShape c;
+                    c.vec = 4;
+                    c.group = 64;
+                    c.unroll = 4;
+                    c.rows_per_group = kStagedRows;
+                    c.nontemporal = nt != 0;
+                    if (spmm_shape_valid(c, n_features, X_dev, Y_dev)) cands.push_back(c);
+                }
             for (int R : {4, 8}) {
                 if (!ensure_grouped(transition, d, R)) break;
                 for (int vec : {4, 2})
