@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <array>
+#include <cstring>
 #include <limits>
+#include <thread>
 #include <vector>
 
 namespace scl_amd {
@@ -17,7 +19,48 @@ struct DeviceState {
     std::mutex call_mu;   // serialises host-pointer entry points
     std::mutex ws_mu;
     std::vector<std::pair<void*, size_t>> ws;
+    // pinned bounce buffers of the host copies (two, kBounceBytes each) and
+    // the events that say when the copy engine is done with each
+    std::mutex bounce_mu;
+    void* bounce[2] = {nullptr, nullptr};
+    hipEvent_t bounce_done[2] = {nullptr, nullptr};
 };
+
+// Host copies at least this large go through the pinned bounce buffers.
+constexpr size_t kBounceMin = 4u << 20;
+constexpr size_t kBounceBytes = 64u << 20;
+constexpr int kCopyThreads = 8;
+
+// dst = src on up to kCopyThreads host threads (chunks of 1 MiB or more)
+void parallel_memcpy(void* dst, const void* src, size_t bytes) {
+    const size_t per = std::max<size_t>(1u << 20, (bytes + kCopyThreads - 1) / kCopyThreads);
+    const int n = (int)std::min<size_t>(kCopyThreads, (bytes + per - 1) / per);
+    if (n <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)n - 1);
+    for (int i = 1; i < n; ++i) {
+        const size_t off = per * (size_t)i, len = std::min(per, bytes - off);
+        th.emplace_back([=] {
+            std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, len);
+        });
+    }
+    std::memcpy(dst, src, std::min(per, bytes));
+    for (auto& t : th) t.join();
+}
+
+void ensure_bounce(DeviceState& st) {
+    for (int b = 0; b < 2; ++b) {
+        if (!st.bounce[b])
+            hip_check(hipHostMalloc(&st.bounce[b], kBounceBytes, hipHostMallocDefault),
+                      "pinned staging buffer");
+        if (!st.bounce_done[b])
+            hip_check(hipEventCreateWithFlags(&st.bounce_done[b], hipEventDisableTiming),
+                      "staging event");
+    }
+}
 
 std::mutex g_mu;
 // Never destroyed: device memory and streams are reclaimed by the runtime at
@@ -70,6 +113,65 @@ hipStream_t library_stream(int device) {
 }
 
 std::mutex& device_mutex(int device) { return state(device).call_mu; }
+
+void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s, int device) {
+    if (bytes == 0) return;
+    if (bytes < kBounceMin) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "upload");
+        return;
+    }
+    DeviceState& st = state(device);
+    std::lock_guard<std::mutex> lk(st.bounce_mu);
+    ensure_bounce(st);
+    // chunk i: wait until buffer i % 2 is free, fill it on the host, queue
+    // its copy; the copy engine moves one buffer while the host fills the
+    // other
+    for (size_t off = 0, i = 0; off < bytes; off += kBounceBytes, ++i) {
+        const int b = (int)(i & 1);
+        const size_t len = std::min(kBounceBytes, bytes - off);
+        hip_check(hipEventSynchronize(st.bounce_done[b]), "staging wait");
+        parallel_memcpy(st.bounce[b], static_cast<const char*>(src) + off, len);
+        hip_check(hipMemcpyAsync(static_cast<char*>(dst) + off, st.bounce[b], len,
+                                 hipMemcpyHostToDevice, s),
+                  "upload");
+        hip_check(hipEventRecord(st.bounce_done[b], s), "staging event");
+    }
+    // the buffers stay busy until their events: later users wait on them
+}
+
+void download_host(void* dst, const void* src, size_t bytes, hipStream_t s, int device) {
+    if (bytes == 0) return;
+    if (bytes < kBounceMin) {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "download");
+        hip_check(hipStreamSynchronize(s), "download");
+        return;
+    }
+    DeviceState& st = state(device);
+    std::lock_guard<std::mutex> lk(st.bounce_mu);
+    ensure_bounce(st);
+    // chunk i is copied into buffer i % 2 by the copy engine while the host
+    // empties chunk i - 1 from the other buffer
+    size_t prev_off = 0, prev_len = 0;
+    int prev_b = -1;
+    for (size_t off = 0, i = 0; off < bytes; off += kBounceBytes, ++i) {
+        const int b = (int)(i & 1);
+        const size_t len = std::min(kBounceBytes, bytes - off);
+        hip_check(hipEventSynchronize(st.bounce_done[b]), "staging wait");
+        hip_check(hipMemcpyAsync(st.bounce[b], static_cast<const char*>(src) + off, len,
+                                 hipMemcpyDeviceToHost, s),
+                  "download");
+        hip_check(hipEventRecord(st.bounce_done[b], s), "staging event");
+        if (prev_b >= 0) {
+            hip_check(hipEventSynchronize(st.bounce_done[prev_b]), "download");
+            parallel_memcpy(static_cast<char*>(dst) + prev_off, st.bounce[prev_b], prev_len);
+        }
+        prev_off = off;
+        prev_len = len;
+        prev_b = b;
+    }
+    hip_check(hipEventSynchronize(st.bounce_done[prev_b]), "download");
+    parallel_memcpy(static_cast<char*>(dst) + prev_off, st.bounce[prev_b], prev_len);
+}
 
 void* device_workspace(int device, int slot, size_t bytes) {
     DeviceState& st = state(device);

@@ -83,10 +83,10 @@ void diffuse_host(scl_sparse_matrix* P, float* X, int64_t n, int64_t F,
     const size_t bytes = (size_t)n * (size_t)F * sizeof(float);
     float* buf[2] = {static_cast<float*>(device_workspace(dev, 0, bytes)),
                      static_cast<float*>(device_workspace(dev, 1, bytes))};
-    hip_check(hipMemcpyAsync(buf[0], X, bytes, hipMemcpyHostToDevice, s), "upload X");
+    upload_host(buf[0], X, bytes, s, dev);
     const SpmmShape shape = choose_shape(d, false, F, buf[0], buf[1]);
     const int fin = pingpong(view_of(d), shape, buf, 0, F, t, s);
-    hip_check(hipMemcpyAsync(X, buf[fin], bytes, hipMemcpyDeviceToHost, s), "download X");
+    download_host(X, buf[fin], bytes, s, dev);
     hip_check(hipStreamSynchronize(s), "diffusion");
 }
 

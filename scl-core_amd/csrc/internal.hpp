@@ -147,6 +147,13 @@ hipStream_t library_stream(int device);
 // Serialises the host-pointer entry points on one device (they share the
 // device's staging buffers).
 std::mutex& device_mutex(int device);
+// Copies between pageable caller memory and device memory, stream-ordered
+// on s: large copies go through the device's pinned bounce buffers (host
+// side on several threads, overlapped with the copy engine).  upload_host
+// returns once the caller's buffer has been read; download_host returns
+// once the data is in the caller's buffer.
+void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s, int device);
+void download_host(void* dst, const void* src, size_t bytes, hipStream_t s, int device);
 // Grow-only device buffer `slot` of a device, at least `bytes` long.
 void* device_workspace(int device, int slot, size_t bytes);
 // Grows d.merge to hold `waves` waves of F features.

@@ -86,11 +86,10 @@ struct Session {
         launch_spmm_f32(w, X, Y, F, sh, s);
     }
     void upload(float* dst, const float* src, int64_t m) {
-        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyHostToDevice, s), "upload");
+        upload_host(dst, src, (size_t)m * 4, s, dev);
     }
     void download(float* dst, const float* src, int64_t m) {
-        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyDeviceToHost, s),
-                  "download");
+        download_host(dst, src, (size_t)m * 4, s, dev);
         hip_check(hipStreamSynchronize(s), "download");
     }
     float scalar(const float* src) {
