@@ -74,7 +74,7 @@ static float kp_apply(float dsq, const kparams* p) {
 /* compute_kernel_matrix (kernel.hpp:530-590): kernel value of every stored
  * distance d, in stored order, of apply_kernel(d * d).  The reference's
  * Gaussian batch evaluates most entries with Highway's vector Exp, which
- * may differ from expf in the last bit. */
+ * may differ from expf in the last bit: parity unpinned in those bits. */
 void orc_kernel_matrix_f32(int64_t nnz, const float* d, float bandwidth, int type, float* out) {
     const kparams p = kp_make(bandwidth, type);
     for (int64_t k = 0; k < nnz; ++k) out[k] = kp_apply(d[k] * d[k], &p);
