@@ -17,8 +17,12 @@
  * Reference citations are relative to the scl-core source tree.
  *
  * Parity status of each function is recorded next to it; every function is
- * pinned by a test in tests/test_oracle.py (closed forms, reference known
- * answers, invariants, brute force).
+ * pinned by tests in tests/test_oracle*.py (closed forms, reference known
+ * answers, published generator definitions, invariants, brute force).  Two
+ * orders taken from Google Highway, which the reference fetches and which is
+ * not in its tree, are parity unpinned: the vector Exp of the Gaussian
+ * kernel batch (construct.c) and the SumOfLanes tree of sum_squared
+ * (neighbors.c); both affect rounding only.
  */
 #ifndef SCL_ORACLE_H
 #define SCL_ORACLE_H
