@@ -28,9 +28,7 @@ struct HostValues {
     DevBuf v;
     HostValues(scl_sparse_matrix* h, hipStream_t s) : nnz(count(h)), v(nnz) {
         std::lock_guard<std::mutex> lk(h->mu);
-        if (nnz > 0)
-            hip_check(hipMemcpyAsync(v.get(), h->data, (size_t)nnz * 4, hipMemcpyHostToDevice, s),
-                      "upload values");
+        if (nnz > 0) upload_host(v.get(), h->data, (size_t)nnz * 4, s, current_device());
     }
     static int64_t count(scl_sparse_matrix* h) {
         std::lock_guard<std::mutex> lk(h->mu);
@@ -39,8 +37,7 @@ struct HostValues {
 };
 
 void download(float* dst, const float* src, int64_t m, hipStream_t s) {
-    if (m > 0)
-        hip_check(hipMemcpyAsync(dst, src, (size_t)m * 4, hipMemcpyDeviceToHost, s), "download");
+    if (m > 0) download_host(dst, src, (size_t)m * 4, s, current_device());
     hip_check(hipStreamSynchronize(s), "download");
 }
 

@@ -109,10 +109,7 @@ void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
     launch_row_sums_f64(d.row_ptr, d.val, n, deg, s);
     launch_normalize_rows(d.row_ptr, d.col, d.val, n, deg, symmetric, s);
     refresh_grouped_values(d, s);
-    if (A->nnz > 0)
-        hip_check(hipMemcpyAsync(A->data, d.val, (size_t)A->nnz * 4,
-                                 hipMemcpyDeviceToHost, s),
-                  "download values");
+    if (A->nnz > 0) download_host(A->data, d.val, (size_t)A->nnz * 4, s, dev);
     if (degrees_out)
         hip_check(hipMemcpyAsync(degrees_out, deg, sizeof(double) * (size_t)n,
                                  hipMemcpyDeviceToHost, s),

@@ -79,9 +79,7 @@ SCL_EXPORT scl_error_t scl_gpu_sparse_set_values_device(scl_sparse_t matrix,
                       "copy values");
             if (keep_t) launch_gather_f32(d.t_src, d.val, d.t_val, matrix->nnz, s);
             refresh_grouped_values(d, s);
-            hip_check(hipMemcpyAsync(matrix->data, d.val, (size_t)matrix->nnz * 4,
-                                     hipMemcpyDeviceToHost, s),
-                      "download values");
+            download_host(matrix->data, d.val, (size_t)matrix->nnz * 4, s, dev);
             hip_check(hipStreamSynchronize(s), "set values");
         }
         matrix->generation++;

@@ -20,8 +20,7 @@ using namespace scl_amd;
 namespace {
 
 void download_bytes(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (bytes > 0)
-        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "download");
+    if (bytes > 0) download_host(dst, src, bytes, s, current_device());
     hip_check(hipStreamSynchronize(s), "download");
 }
 
@@ -46,20 +45,14 @@ void knn_impl(scl_sparse_matrix* m, const float* host_norms, int64_t n_samples, 
     launch_densify(P.ip.as<int64_t>(), P.ix.as<int64_t>(), P.v.get(), N, n_pad, f_pad, true,
                    P.serial, X.get(), s);
     if (host_norms) {
-        if (N > 0)
-            hip_check(hipMemcpyAsync(norms.get(), host_norms, (size_t)N * 4,
-                                     hipMemcpyHostToDevice, s),
-                      "upload norms");
+        if (N > 0) upload_host(norms.get(), host_norms, (size_t)N * 4, s, dev);
     } else {
         launch_sum_squared(P.ip.as<int64_t>(), P.v.get(), N, norms.get(), s);
     }
     DevBuf ld(N * k), lj(N * k, sizeof(int32_t)), od(N * k), oi(N * k, sizeof(int64_t));
     launch_knn(X.get(), norms.get(), N, f_pad, k, ld.get(), lj.as<int32_t>(), od.get(),
                oi.as<int64_t>(), s);
-    if (N * k > 0)
-        hip_check(hipMemcpyAsync(out_dist, od.get(), (size_t)(N * k) * 4, hipMemcpyDeviceToHost,
-                                 s),
-                  "download distances");
+    download_bytes(out_dist, od.get(), (size_t)(N * k) * 4, s);
     download_bytes(out_idx, oi.get(), (size_t)(N * k) * 8, s);
 }
 

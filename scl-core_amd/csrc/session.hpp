@@ -120,12 +120,9 @@ struct HostPattern {
                                  hipMemcpyHostToDevice, s),
                   "upload pattern");
         if (nnz > 0) {
-            hip_check(hipMemcpyAsync(ix.as<int64_t>(), X->indices + X->indptr[0],
-                                     (size_t)nnz * 8, hipMemcpyHostToDevice, s),
-                      "upload pattern");
-            hip_check(hipMemcpyAsync(v.get(), X->data + X->indptr[0], (size_t)nnz * 4,
-                                     hipMemcpyHostToDevice, s),
-                      "upload values");
+            const int dev = current_device();
+            upload_host(ix.as<int64_t>(), X->indices + X->indptr[0], (size_t)nnz * 8, s, dev);
+            upload_host(v.get(), X->data + X->indptr[0], (size_t)nnz * 4, s, dev);
         }
         DevBuf flag(1, sizeof(int));
         hip_check(hipMemsetAsync(flag.get(), 0, sizeof(int), s), "clear flag");
@@ -165,9 +162,7 @@ struct HostRows {
                                  hipMemcpyHostToDevice, s),
                   "upload row pointer");
         if (nnz > 0)
-            hip_check(hipMemcpyAsync(v.get(), X->data + X->indptr[0], (size_t)nnz * 4,
-                                     hipMemcpyHostToDevice, s),
-                      "upload values");
+            upload_host(v.get(), X->data + X->indptr[0], (size_t)nnz * 4, s, current_device());
         hip_check(hipStreamSynchronize(s), "upload rows");
     }
 };
