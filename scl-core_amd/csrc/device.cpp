@@ -26,8 +26,10 @@ struct DeviceState {
     hipEvent_t bounce_done[2] = {nullptr, nullptr};
 };
 
-// Host copies at least this large go through the pinned bounce buffers.
-constexpr size_t kBounceMin = 4u << 20;
+// Host copies at least this large go through the pinned bounce buffers
+// (smaller ones are one chunk, where the host copy and the transfer cannot
+// overlap, and the runtime's own pageable path is as fast).
+constexpr size_t kBounceMin = 64u << 20;
 constexpr size_t kBounceBytes = 64u << 20;
 constexpr int kCopyThreads = 8;
 
