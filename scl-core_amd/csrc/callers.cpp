@@ -177,7 +177,8 @@ SCL_EXPORT scl_error_t scl_pseudotime_diffusion(scl_sparse_t adjacency,
             std::vector<float> start((size_t)(N * K));
             FastRng rng(kPseudotimeSeed);
             for (auto& x : start) x = rng.uniform() - 0.5f;
-            DevBuf Q(N * K), Qn(N * K), C(N * K), w(N), gpart(2 * 1024), sc(8);
+            DevBuf Q(N * K), Qn(N * K), C(N * K), w(N), gpart(gram_schmidt_part_floats(N, K)),
+                sc(8);
             S.upload(Q.get(), start.data(), N * K);
             // Gram-Schmidt over contiguous columns (C is K x N), across the GPU
             auto orthonormalise = [&](float* M) {

@@ -218,11 +218,13 @@ void launch_scale(float* x, int64_t n, float a, hipStream_t s);
 void launch_dmap_init(float* Q, int64_t n, int64_t K, hipStream_t s);
 void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s);
 // The same over the whole GPU on a column-major K x n matrix (columns
-// contiguous); part: 2 * 1024 floats, S: 6 floats of device scratch.
-// reorth: second pass and unit-vector fallback (diffusion maps); otherwise
-// one pass and FastRNG(v * rng_mult) directions (pseudotime).
+// contiguous); part: gram_schmidt_part_floats(n, K) floats, S: 8 floats of
+// device scratch.  reorth: second pass and unit-vector fallback (diffusion
+// maps); otherwise one pass and FastRNG(v * rng_mult) directions
+// (pseudotime).
 void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
                          bool reorth, uint64_t rng_mult, hipStream_t s);
+int64_t gram_schmidt_part_floats(int64_t n, int64_t K);
 // B (cols x rows) = A (rows x cols)^T, both row-major.
 void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
                            hipStream_t s);

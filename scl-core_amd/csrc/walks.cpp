@@ -275,7 +275,8 @@ SCL_EXPORT scl_error_t scl_diffusion_diffusion_map_embedding(scl_sparse_t transi
             // rows the columns are transposed to contiguous vectors and every
             // dot product spreads over the GPU
             const bool wide = n >= kWideGramSchmidtRows;
-            DevBuf C(wide ? n * K : 0), part(wide ? 2 * 1024 : 0), sc(wide ? 8 : 0);
+            DevBuf C(wide ? n * K : 0), part(wide ? gram_schmidt_part_floats(n, K) : 0),
+                sc(wide ? 8 : 0);
             auto orthogonalise = [&](float* M) {
                 if (!wide) {
                     launch_mgs(M, w.get(), n, K, S.s);

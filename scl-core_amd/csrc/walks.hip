@@ -240,92 +240,112 @@ __global__ __launch_bounds__(kBlock) void k_identity(float* __restrict__ A, int6
 }
 
 // --- multi-workgroup Gram-Schmidt over a column-major K x n matrix C ------
-// (columns contiguous).  Every global sum is two launches: per-block partials
-// then one fold, so a pass over the columns needs no host synchronisation.
-// The fold kernels keep the scalars the next kernels read in S:
-//   S[0] coeff of the current projection      S[1] 1 if it applies
-//   S[2] running max |<w, c_p>| / |w|        S[3] 1 if a second pass runs
-//   S[4] |w|^2 of the current column          S[5] 1 / |w| (0: degenerate)
+// (columns contiguous).  Each step launch ends with one (dot, norm) partial
+// pair per workgroup; the next launch folds those partials again in every
+// workgroup, in the same order, so every workgroup derives the same
+// coefficient and kernel boundaries are the only synchronisation (no
+// cross-workgroup fences, which on this GPU write back an XCD's whole L2).
+// Partials alternate between two buffers of gridDim.x pairs.  S holds
+//   S[2] max |<w, c_p>| / |w| of the re-orthogonalisation test
+//   S[3] 1 if the second pass runs
 
-// partial sums of <w, c_p> and <c_p, c_p> (a = w, b = c_p), or of <w, w>
-__global__ __launch_bounds__(kBlock) void k_gs_dots(const float* __restrict__ a,
-                                                    const float* __restrict__ b, int64_t n,
-                                                    float* __restrict__ part,
+// (sum of the first, sum of the second entries) of m partial pairs, the
+// same in every workgroup
+__device__ __forceinline__ void gs_fold(const float* __restrict__ part, int m, float& d,
+                                        float& q) {
+    d = 0.0f;
+    q = 0.0f;
+    for (int i = threadIdx.x; i < m; i += kBlock) {
+        d += part[2 * i];
+        q += part[2 * i + 1];
+    }
+    d = block_reduce(d, false);
+    q = block_reduce(q, false);
+}
+
+enum : int { kPreNone = 0, kPreCopy = 1, kPreAxpy = 2 };
+
+// One step of the pass.  First the update of w that the previous step
+// decided: pre = kPreCopy copies src (the column) into w; kPreAxpy applies
+// w -= (d / q) * src, (d, q) folded from prev, when q > 1e-15 and pre_gate
+// is open.  Then the partial sums of <w, b> and <b, b> (b = w when b is
+// null) into cur.  A closed `gate` skips the whole step (it is only passed
+// where it also closes pre_gate).  w, src and b are distinct columns, or b
+// is null; prev and cur are distinct.
+__global__ __launch_bounds__(kBlock) void k_gs_step(float* w, const float* src, int pre,
+                                                    const float* pre_gate, const float* b,
+                                                    int64_t n, const float* __restrict__ prev,
+                                                    float* __restrict__ cur,
                                                     const float* __restrict__ gate) {
     if (gate && *gate == 0.0f) return;
+    bool upd = pre == kPreCopy;
+    float coeff = 0.0f;
+    if (pre == kPreAxpy && !(pre_gate && *pre_gate == 0.0f)) {
+        float d, q;
+        gs_fold(prev, (int)gridDim.x, d, q);
+        upd = q > 1e-15f;
+        coeff = upd ? d / q : 0.0f;
+    }
     float d = 0.0f, q = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock) {
-        const float bi = b[i];
-        d = __builtin_fmaf(a[i], bi, d);
+        float wi;
+        if (pre == kPreCopy) {
+            wi = src[i];
+            w[i] = wi;
+        } else {
+            wi = w[i];
+            if (upd) {
+                wi = __builtin_fmaf(-coeff, src[i], wi);
+                w[i] = wi;
+            }
+        }
+        const float bi = b ? b[i] : wi;
+        d = __builtin_fmaf(wi, bi, d);
         q = __builtin_fmaf(bi, bi, q);
     }
     d = block_reduce(d, false);
     q = block_reduce(q, false);
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = d;
-        part[2 * blockIdx.x + 1] = q;
+        cur[2 * blockIdx.x] = d;
+        cur[2 * blockIdx.x + 1] = q;
     }
 }
 
-// mode 0: projection coefficient of c_p (S[0], S[1])
-// mode 1: cos = |dot| / sqrt(S[4]) folded into the running max S[2]
-// mode 2: |w|^2 into S[4] (dot of w with itself: the first partial)
-// mode 3: normalisation factor S[5] from |w|^2
-// mode 4: S[3] = (S[2] >= 0.7): whether the second pass runs
-__global__ __launch_bounds__(kBlock) void k_gs_fold(const float* __restrict__ part, int m,
-                                                    int mode, float* __restrict__ S,
-                                                    const float* __restrict__ gate) {
-    if (gate && *gate == 0.0f) return;
-    float d = 0.0f, q = 0.0f;
-    if (mode != 4) {
-        for (int i = threadIdx.x; i < m; i += kBlock) {
-            d += part[2 * i];
-            q += part[2 * i + 1];
-        }
+// Partials of <w, c_p> for p < v: part[p * gridDim.x + workgroup].
+__global__ __launch_bounds__(kBlock) void k_gs_check(const float* __restrict__ w,
+                                                     const float* __restrict__ C, int64_t n,
+                                                     int64_t v, float* __restrict__ part) {
+    for (int64_t p = 0; p < v; ++p) {
+        const float* c = C + p * n;
+        float d = 0.0f;
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kBlock)
+            d = __builtin_fmaf(w[i], c[i], d);
         d = block_reduce(d, false);
-        q = block_reduce(q, false);
-    }
-    if (threadIdx.x != 0) return;
-    if (mode == 0) {
-        const bool ok = q > 1e-15f;
-        S[0] = ok ? d / q : 0.0f;
-        S[1] = ok ? 1.0f : 0.0f;
-    } else if (mode == 1) {
-        const float nv = S[4];
-        const float cs = nv > 1e-15f ? fabsf(d) / sqrtf(nv) : 0.0f;
-        S[2] = fmaxf(S[2], cs);
-    } else if (mode == 2) {
-        S[4] = d;
-    } else if (mode == 3) {
-        S[5] = d > 1e-15f ? 1.0f / sqrtf(d) : 0.0f;
-    } else {
-        S[3] = S[2] >= 0.7f ? 1.0f : 0.0f;
+        if (threadIdx.x == 0) part[p * gridDim.x + blockIdx.x] = d;
     }
 }
 
-// w -= S[0] * c_p when S[1] says so (and the gate is open)
-__global__ __launch_bounds__(kBlock) void k_gs_axpy(float* __restrict__ w,
-                                                    const float* __restrict__ c, int64_t n,
-                                                    const float* __restrict__ S,
-                                                    const float* __restrict__ gate) {
-    if ((gate && *gate == 0.0f) || S[1] == 0.0f) return;
-    const float coeff = S[0];
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) w[i] = __builtin_fmaf(-coeff, c[i], w[i]);
-}
-
-// column v = w * S[5]; a degenerate column (S[5] == 0) becomes e_(v mod n)
-// when `unit`, else it is left to k_gs_random
-__global__ __launch_bounds__(kBlock) void k_gs_store(float* __restrict__ c,
-                                                     const float* __restrict__ w, int64_t n,
-                                                     int64_t v, const float* __restrict__ S,
-                                                     int unit) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float inv = S[5];
-    if (inv != 0.0f) c[i] = w[i] * inv;
-    else if (unit) c[i] = i == v % n ? 1.0f : 0.0f;
+// The re-orthogonalisation test (one workgroup): |w|^2 folded from the m
+// partial pairs in fin, then S[2] = max over p < v of |<w, c_p>| / |w|
+// (0 when |w|^2 <= 1e-15), folded from check, and S[3] = (S[2] >= 0.7).
+__global__ __launch_bounds__(kBlock) void k_gs_test(const float* __restrict__ fin,
+                                                    const float* __restrict__ check, int m,
+                                                    int64_t v, float* __restrict__ S) {
+    float nv, unused;
+    gs_fold(fin, m, nv, unused);
+    float mx = 0.0f;
+    for (int64_t p = 0; p < v; ++p) {
+        float d = 0.0f;
+        for (int i = threadIdx.x; i < m; i += kBlock) d += check[p * m + i];
+        d = block_reduce(d, false);
+        mx = fmaxf(mx, nv > 1e-15f ? fabsf(d) / sqrtf(nv) : 0.0f);
+    }
+    if (threadIdx.x == 0) {
+        S[2] = mx;
+        S[3] = mx >= 0.7f ? 1.0f : 0.0f;
+    }
 }
 
 // FastRNG of pseudotime.hpp:153-182: four chained rounds of the first half of
@@ -354,34 +374,35 @@ struct FastRng {
     }
 };
 
-// Degenerate column v (S[5] == 0) of the pseudotime basis: the normalised
-// vector uniform() - 0.5 of FastRng(v * mult), drawn and summed in index
-// order by one thread (pseudotime.hpp:460-471).  w: n floats.
-__global__ void k_gs_random(float* __restrict__ c, float* __restrict__ w, int64_t n,
-                            int64_t v, uint64_t mult, const float* __restrict__ S) {
-    if (S[5] != 0.0f || threadIdx.x != 0 || blockIdx.x != 0) return;
-    FastRng r((uint64_t)v * mult);
-    float nrm = 0.0f;
-    for (int64_t i = 0; i < n; ++i) {
-        const float x = r.uniform() - 0.5f;
-        w[i] = x;
-        nrm = __builtin_fmaf(x, x, nrm);
+// Column v = w / |w|, |w|^2 folded from the m partial pairs in fin.  A
+// degenerate column (|w|^2 <= 1e-15) becomes e_(v mod n) when rng_mult is 0
+// (diffusion maps), else the normalised vector uniform() - 0.5 of
+// FastRng(v * rng_mult), drawn and summed in index order by one thread
+// (pseudotime.hpp:460-471), with w as its scratch.
+__global__ __launch_bounds__(kBlock) void k_gs_store(float* __restrict__ c,
+                                                     float* __restrict__ w, int64_t n,
+                                                     int64_t v, const float* __restrict__ fin,
+                                                     int m, uint64_t rng_mult) {
+    float nv, unused;
+    gs_fold(fin, m, nv, unused);
+    const float inv = nv > 1e-15f ? 1.0f / sqrtf(nv) : 0.0f;
+    if (inv == 0.0f && rng_mult != 0) {
+        if (blockIdx.x != 0 || threadIdx.x != 0) return;
+        FastRng r((uint64_t)v * rng_mult);
+        float nrm = 0.0f;
+        for (int64_t i = 0; i < n; ++i) {
+            const float x = r.uniform() - 0.5f;
+            w[i] = x;
+            nrm = __builtin_fmaf(x, x, nrm);
+        }
+        const float s = 1.0f / sqrtf(nrm);
+        for (int64_t i = 0; i < n; ++i) c[i] = w[i] * s;
+        return;
     }
-    const float inv = 1.0f / sqrtf(nrm);
-    for (int64_t i = 0; i < n; ++i) c[i] = w[i] * inv;
-}
-
-__global__ __launch_bounds__(kBlock) void k_gs_copy(float* __restrict__ w,
-                                                    const float* __restrict__ c, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) w[i] = c[i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_gs_reset(float* __restrict__ S, float v2) {
-    if (threadIdx.x == 0) {
-        S[2] = 0.0f;
-        S[3] = v2;
-    }
+    if (i >= n) return;
+    if (inv != 0.0f) c[i] = w[i] * inv;
+    else c[i] = i == v % n ? 1.0f : 0.0f;
 }
 
 // B (K x n) = A (n x K)^T and back, through LDS tiles of 16 x 16; one
@@ -543,53 +564,56 @@ void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
 }
 
 // Modified Gram-Schmidt of the K contiguous columns of C (K x n), across the
-// whole GPU: w is an n-float workspace, part holds 2 * kParts partials, S the
-// 6 scalars above.  reorth: the diffusion-map form (diffusion.hpp:370-446:
-// a second pass when some |<w, c_p>| / |w| >= 0.7, degenerate columns become
-// unit vectors); otherwise the pseudotime form (pseudotime.hpp:418-474: one
-// pass, degenerate columns drawn from FastRng(v * rng_mult)).
+// whole GPU: w is an n-float workspace, part holds
+// gram_schmidt_part_floats(n, K) partials, S 8 floats (the scalars above).
+// reorth: the diffusion-map form (diffusion.hpp:370-446: a second pass when
+// some |<w, c_p>| / |w| >= 0.7, degenerate columns become unit vectors);
+// otherwise the pseudotime form (pseudotime.hpp:418-474: one pass,
+// degenerate columns drawn from FastRng(v * rng_mult)).  Column v costs
+// v + 2 launches in one pass, 2v + 5 with the test and the second pass.
 void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
                          bool reorth, uint64_t rng_mult, hipStream_t s) {
     if (n <= 0 || K <= 0) return;
     const unsigned p = parts_for(n), b = blocks_for(n);
-    auto dots = [&](const float* a, const float* c, const float* gate) {
-        hipLaunchKernelGGL(k_gs_dots, dim3(p), dim3(kBlock), 0, s, a, c, n, part, gate);
-    };
-    auto fold = [&](int mode, const float* gate) {
-        hipLaunchKernelGGL(k_gs_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, mode, S, gate);
+    float* pairs[2] = {part, part + 2 * (int64_t)p};
+    float* check = part + 4 * (int64_t)p;
+    int cur = 0;  // the buffer the next step writes
+    auto step = [&](int pre, const float* src, const float* pre_gate, const float* bv,
+                    const float* gate) {
+        hipLaunchKernelGGL(k_gs_step, dim3(p), dim3(kBlock), 0, s, w, src, pre, pre_gate, bv, n,
+                           pairs[cur ^ 1], pairs[cur], gate);
+        cur ^= 1;
     };
     for (int64_t v = 0; v < K; ++v) {
         float* cv = C + v * n;
-        hipLaunchKernelGGL(k_gs_reset, dim3(1), dim3(64), 0, s, S, 1.0f);
-        hipLaunchKernelGGL(k_gs_copy, dim3(b), dim3(kBlock), 0, s, w, cv, n);
         for (int pass = 0; pass < (reorth ? 2 : 1) && (pass == 0 || v > 0); ++pass) {
             // the second pass runs only when the first left some
-            // |<w, c_p>| / |w| >= 0.7 (gate S[3])
+            // |<w, c_p>| / |w| >= 0.7 (gate S[3]); while it is closed no step
+            // of the pass reads or writes partials
             const float* gate = pass == 0 ? nullptr : S + 3;
+            // step q applies the coefficient of c_(q-1) and sums the partials
+            // of c_q's; the first step of the column copies it into w
             for (int64_t q = 0; q < v; ++q) {
-                dots(w, C + q * n, gate);
-                fold(0, gate);
-                hipLaunchKernelGGL(k_gs_axpy, dim3(b), dim3(kBlock), 0, s, w, C + q * n, n, S,
-                                   gate);
+                if (q == 0) step(pass == 0 ? kPreCopy : kPreNone, cv, gate, C, gate);
+                else step(kPreAxpy, C + (q - 1) * n, gate, C + q * n, gate);
             }
+            // the last coefficient applied, then the partials of |w|^2
+            if (v == 0) step(kPreCopy, cv, nullptr, nullptr, nullptr);
+            else step(kPreAxpy, C + (v - 1) * n, gate, nullptr, nullptr);
             if (reorth && pass == 0 && v > 0) {
-                dots(w, w, nullptr);
-                fold(2, nullptr);
-                for (int64_t q = 0; q < v; ++q) {
-                    dots(w, C + q * n, nullptr);
-                    fold(1, nullptr);
-                }
-                fold(4, nullptr);
+                hipLaunchKernelGGL(k_gs_check, dim3(p), dim3(kBlock), 0, s, w, C, n, v, check);
+                hipLaunchKernelGGL(k_gs_test, dim3(1), dim3(kBlock), 0, s, pairs[cur ^ 1], check,
+                                   (int)p, v, S);
             }
         }
-        dots(w, w, nullptr);
-        fold(3, nullptr);
-        hipLaunchKernelGGL(k_gs_store, dim3(b), dim3(kBlock), 0, s, cv, w, n, v, S,
-                           rng_mult == 0 ? 1 : 0);
-        if (rng_mult != 0)
-            hipLaunchKernelGGL(k_gs_random, dim3(1), dim3(64), 0, s, cv, w, n, v, rng_mult, S);
+        hipLaunchKernelGGL(k_gs_store, dim3(b), dim3(kBlock), 0, s, cv, w, n, v, pairs[cur ^ 1],
+                           (int)p, rng_mult);
     }
     check_launch("gram-schmidt launch");
+}
+
+int64_t gram_schmidt_part_floats(int64_t n, int64_t K) {
+    return (int64_t)parts_for(n) * (4 + K);
 }
 
 void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s) {
