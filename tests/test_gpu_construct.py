@@ -99,3 +99,45 @@ def test_construct_argument_errors(gpu):
     with pytest.raises(SCLError) as e:
         construct.row_normalize(T, 101)
     assert e.value.code == 11
+
+
+def upper_distances(n):
+    """The distance fixture of test_kernel.cpp:24-64: row i stores columns
+    i..n-1 with distance j - i (0 on the diagonal)."""
+    ip, ix, d = [0], [], []
+    for i in range(n):
+        for j in range(i, n):
+            ix.append(j)
+            d.append(float(j - i))
+        ip.append(len(ix))
+    return np.array(ip, np.int64), np.array(ix, np.int64), np.array(d, np.float32)
+
+
+def test_kernel_reference_cases(gpu):
+    # test_kernel.cpp:347-366 (basic: values >= 0), 369-385 (K(0) = 1 for the
+    # Gaussian), 416-432 (row sums > 0), 435-453 (four kernel kinds succeed)
+    for n in (5, 3, 4):
+        ip, ix, d = upper_distances(n)
+        D = CsrMatrix.from_arrays(ip, ix, d, (n, n))
+        vals = construct.kernel_matrix(D, 1.0, "gaussian")
+        assert np.all(vals >= 0.0)
+        assert vals[0] == 1.0 and np.all(vals[ip[:-1]] == 1.0)   # every diagonal entry
+        assert np.allclose(vals, np.exp(-0.5 * d.astype(np.float64) ** 2), rtol=1e-6)
+        assert np.abs(vals - orc.kernel_matrix_f32(d, 1.0, 0)).max() <= 1e-7
+        sums = construct.kernel_row_sums(D, 1.0, "gaussian")
+        assert np.all(sums > 0.0)
+        for kind in ("gaussian", "epanechnikov", "cosine", "linear"):
+            k = construct.KERNELS[kind]
+            got = construct.kernel_row_sums(D, 1.0, kind)
+            ref = orc.kernel_row_sums_f32(ip, d, 1.0, k)
+            assert np.allclose(got, ref, rtol=2e-6, atol=1e-7), kind
+    # closed forms at distance 0 and past the bandwidth
+    ip, ix, d = upper_distances(4)
+    D = CsrMatrix.from_arrays(ip, ix, d, (4, 4))
+    assert np.all(construct.kernel_matrix(D, 1.0, "epanechnikov")[d == 0] == 0.75)
+    assert np.all(construct.kernel_matrix(D, 1.0, "epanechnikov")[d > 1] == 0.0)
+    assert np.all(construct.kernel_matrix(D, 1.0, "uniform")[d == 0] == 0.5)
+    assert np.all(construct.kernel_matrix(D, 1.0, "triangular")[d >= 1] == 0.0)
+    assert np.allclose(construct.kernel_matrix(D, 1.0, "cosine")[d == 0], np.pi / 4, rtol=1e-6)
+    assert np.array_equal(construct.kernel_matrix(D, 2.0, "cauchy"),
+                          orc.kernel_matrix_f32(d, 2.0, 6))

@@ -82,3 +82,48 @@ def test_knn_argument_errors(gpu):
     with pytest.raises(SCLError) as e:
         construct.knn(M, 0)
     assert e.value.code == 10
+
+
+def test_neighbors_reference_cases(gpu):
+    # test_neighbors.cpp:71-90: norms of [1, 2], [3, 4], [5, 6]
+    M = CsrMatrix.from_arrays(np.array([0, 2, 4, 6], np.int64),
+                              np.array([0, 1, 0, 1, 0, 1], np.int64),
+                              np.array([1, 2, 3, 4, 5, 6], np.float32), (3, 2))
+    assert list(construct.neighbor_norms(M)) == [5.0, 25.0, 61.0]
+    # 142-158: an empty matrix; as in the reference (core/sparse.cpp:64-66)
+    # a handle with no rows is refused at creation
+    with pytest.raises(SCLError) as e:
+        CsrMatrix.from_arrays(np.zeros(1, np.int64), np.zeros(0, np.int64),
+                              np.zeros(0, np.float32), (0, 10))
+    assert e.value.code == 10
+    # 166-205 and 348-385: the unit square with explicit zeros stored, k = 2:
+    # the two axis neighbours at distance 1, smaller index first
+    sq = CsrMatrix.from_arrays(np.array([0, 2, 4, 6, 8], np.int64),
+                               np.array([0, 1, 0, 1, 0, 1, 0, 1], np.int64),
+                               np.array([0, 0, 1, 0, 0, 1, 1, 1], np.float32), (4, 2))
+    for gi, gd in (construct.knn(sq, 2), construct.knn(sq, 2, construct.neighbor_norms(sq))):
+        assert gi.tolist() == [[1, 2], [0, 3], [0, 3], [1, 2]]
+        assert np.all(gd == 1.0)
+    # 208-244, 246-268 and 387-425 (seeds 42, 123, 456): 10-30 samples of 5-20
+    # features, density 0.05-0.3, k = 1..5; knn with norms equals knn_simple
+    for seed in (42, 123, 456):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        n, f = int(rng.integers(10, 31)), int(rng.integers(5, 21))
+        ip, ix, v = samples(n, f, float(rng.uniform(0.05, 0.3)), seed)
+        A = CsrMatrix.from_arrays(ip, ix, v, (n, f))
+        norms = construct.neighbor_norms(A)
+        assert np.array_equal(norms, orc.norms_f32(ip, v))
+        ref64 = np.array([float((v[ip[i]:ip[i + 1]].astype(np.float64) ** 2).sum())
+                          for i in range(n)])
+        assert np.allclose(norms, ref64, rtol=1e-6)
+        for k in (1, int(rng.integers(2, min(n - 1, 5) + 1))):
+            si, sd = construct.knn(A, k)
+            wi, wd = construct.knn(A, k, norms)
+            assert np.array_equal(si, wi) and np.array_equal(sd, wd)
+            assert np.all(sd >= 0) and np.all((si >= 0) & (si < n))
+            assert np.all(np.diff(sd, axis=1) >= 0)
+            ri, rd = by_distance_then_index(*orc.knn_f32(ip, ix, v, k))
+            assert np.array_equal(si, ri) and np.array_equal(sd, rd), (seed, k)
+    # 320-346: k larger than the other samples: the rest is -1 / inf
+    gi, gd = construct.knn(A, n + 5)
+    assert np.all(gi[:, n - 1:] == -1) and np.all(np.isinf(gd[:, n - 1:]))
