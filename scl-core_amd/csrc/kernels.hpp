@@ -199,12 +199,17 @@ void launch_stationarity_residual(const int32_t* t_row_ptr, const int32_t* t_col
 // ---- vector-side consumers (walks.hip) ----------------------------------
 // x = fma(a, x, b * t) elementwise.
 void launch_blend(float* x, const float* t, int64_t n, float a, float b, hipStream_t s);
-// y = fma(alpha, y, beta * r); *total = sum |y - x| (device scalar).
+// y = fma(alpha, y, beta * r); *total = sum |y - x| (device scalar), and
+// *done = 1 once *total < tol.  With *done set, y = x and total keeps the
+// converged step's value.
 void launch_rwr_update(float* y, const float* x, const float* r, int64_t n, float alpha,
-                       float beta, float* part, float* total, hipStream_t s);
-// One DPT bookkeeping step; *max_unvisited = max(1 - vis) (device scalar).
+                       float beta, float* part, float* total, float tol, float* done,
+                       hipStream_t s);
+// One DPT bookkeeping step; *max_unvisited = max(1 - vis) (device scalar),
+// *done = 1 once it is below tol.  With *done set, nothing changes.
 void launch_dpt_update(const float* pn, float* vis, float* hit, int64_t n, float it,
-                       float* part, float* max_unvisited, hipStream_t s);
+                       float* part, float* max_unvisited, float tol, float* done,
+                       hipStream_t s);
 // hit /= vis where vis > 0.5, pt = vis > 0.5 ? hit / max : 1.
 void launch_dpt_finish(const float* vis, float* hit, float* pt, int64_t n, float* part,
                        float* max_time, hipStream_t s);

@@ -44,21 +44,31 @@ constexpr int64_t kDefaultHksSteps = 10;
 constexpr float kDefaultLaziness = 0.5f;
 // rows from which the diffusion-map basis is orthogonalised across the GPU
 constexpr int64_t kWideGramSchmidtRows = 16384;
+// steps of a converging walk between two host reads of its convergence flag;
+// the steps after convergence leave the result unchanged on the device
+constexpr int64_t kStepsPerCheck = 8;
 
 // compute_dpt (diffusion.hpp:670-768) on device buffers; pt is device.
 void dpt_device(Session& S, int64_t root, float* pt, int64_t max_iter, float tol) {
     const int64_t n = S.n;
-    DevBuf p(n), pn(n), hit(n), vis(n), part(1024), scal(2);
+    DevBuf p(n), pn(n), hit(n), vis(n), part(1024), scal(3);
+    float* done = scal.get() + 2;
     launch_set_unit(p.get(), n, root, 1.0f, S.s);
     launch_set_unit(vis.get(), n, root, 1.0f, S.s);
     launch_set_unit(hit.get(), n, -1, 0.0f, S.s);
+    launch_fill(done, 1, 0.0f, S.s);
     float* cur = p.get();
     float* nxt = pn.get();
-    for (int64_t it = 1; it <= max_iter; ++it) {
-        S.spmv(cur, nxt);
-        launch_dpt_update(nxt, vis.get(), hit.get(), n, (float)it, part.get(), scal.get(), S.s);
-        if (S.scalar(scal.get()) < tol) break;
-        std::swap(cur, nxt);
+    for (int64_t it = 1; it <= max_iter;) {
+        // the step whose max(1 - visited) falls below tol sets the flag; the
+        // later steps of the batch leave visited and hit as they are
+        for (const int64_t end = std::min(max_iter, it + kStepsPerCheck - 1); it <= end; ++it) {
+            S.spmv(cur, nxt);
+            launch_dpt_update(nxt, vis.get(), hit.get(), n, (float)it, part.get(), scal.get(),
+                              tol, done, S.s);
+            std::swap(cur, nxt);
+        }
+        if (S.scalar(done) != 0.0f) break;
     }
     launch_dpt_finish(vis.get(), hit.get(), pt, n, part.get(), scal.get() + 1, S.s);
 }
@@ -155,24 +165,32 @@ SCL_EXPORT scl_error_t scl_diffusion_random_walk_with_restart(
             }
             const float inv = 1.0f / (float)cnt;
             for (float& r : restart) r *= inv;
-            DevBuf out(n), other(n), rs(n), part(1024), total(1);
+            DevBuf out(n), other(n), rs(n), part(1024), total(2);
+            float* done = total.get() + 1;
             S.upload(rs.get(), restart.data(), n);
             S.upload(out.get(), restart.data(), n);
+            launch_fill(done, 1, 0.0f, S.s);
             const float beta = 1.0f - a;
             // out is the caller's buffer in the reference; cur/nxt swap as its
             // pointer does
             float* cur = out.get();
             float* nxt = other.get();
-            for (int64_t it = 0; it < iters; ++it) {
-                S.spmv(cur, nxt);
-                launch_rwr_update(nxt, cur, rs.get(), n, a, beta, part.get(), total.get(), S.s);
-                if (S.scalar(total.get()) < tl) {
-                    hip_check(hipMemcpyAsync(cur, nxt, (size_t)n * 4, hipMemcpyDeviceToDevice,
+            for (int64_t it = 0; it < iters;) {
+                for (const int64_t end = std::min(iters, it + kStepsPerCheck); it < end; ++it) {
+                    S.spmv(cur, nxt);
+                    launch_rwr_update(nxt, cur, rs.get(), n, a, beta, part.get(), total.get(),
+                                      tl, done, S.s);
+                    std::swap(cur, nxt);
+                }
+                if (S.scalar(done) != 0.0f) {
+                    // the reference copies the converged vector over the
+                    // other buffer and stops: cur holds it (the steps after
+                    // convergence copied it from buffer to buffer)
+                    hip_check(hipMemcpyAsync(nxt, cur, (size_t)n * 4, hipMemcpyDeviceToDevice,
                                              S.s),
                               "copy");
                     break;
                 }
-                std::swap(cur, nxt);
             }
             S.download(scores, out.get(), n);
         }

@@ -55,11 +55,20 @@ __global__ __launch_bounds__(kBlock) void k_blend(float* __restrict__ x,
 }
 
 // y[i] = fma(alpha, y[i], beta * r[i]); partial sums of |y[i] - x[i]|
+// Once *done is set the walk has converged: y takes x unchanged, so the
+// converged vector stays in both buffers whatever the steps after it do.
 __global__ __launch_bounds__(kBlock) void k_rwr_update(float* __restrict__ y,
                                                        const float* __restrict__ x,
                                                        const float* __restrict__ r,
                                                        int64_t n, float alpha, float beta,
-                                                       float* __restrict__ part) {
+                                                       float* __restrict__ part,
+                                                       const float* __restrict__ done) {
+    if (*done != 0.0f) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kBlock)
+            y[i] = x[i];
+        return;
+    }
     float s = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock) {
@@ -73,11 +82,13 @@ __global__ __launch_bounds__(kBlock) void k_rwr_update(float* __restrict__ y,
 }
 
 // One DPT step's bookkeeping (diffusion.hpp:710-733) and partial maxima of
-// 1 - visited.
+// 1 - visited; nothing once *done is set.
 __global__ __launch_bounds__(kBlock) void k_dpt_update(const float* __restrict__ pn,
                                                        float* __restrict__ vis,
                                                        float* __restrict__ hit, int64_t n,
-                                                       float it, float* __restrict__ part) {
+                                                       float it, float* __restrict__ part,
+                                                       const float* __restrict__ done) {
+    if (*done != 0.0f) return;
     float m = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kBlock) {
@@ -134,6 +145,22 @@ __global__ __launch_bounds__(kBlock) void k_fold(const float* __restrict__ part,
         v = is_max ? fmaxf(v, part[i]) : v + part[i];
     v = block_reduce(v, is_max != 0);
     if (threadIdx.x == 0) out[0] = v;
+}
+
+// k_fold, then *done = 1 when the result is below tol; nothing once *done
+// is set (the step that converged keeps its value in out).
+__global__ __launch_bounds__(kBlock) void k_fold_until(const float* __restrict__ part, int m,
+                                                       int is_max, float* __restrict__ out,
+                                                       float tol, float* __restrict__ done) {
+    if (*done != 0.0f) return;
+    float v = 0.0f;
+    for (int i = threadIdx.x; i < m; i += kBlock)
+        v = is_max ? fmaxf(v, part[i]) : v + part[i];
+    v = block_reduce(v, is_max != 0);
+    if (threadIdx.x == 0) {
+        out[0] = v;
+        if (v < tol) *done = 1.0f;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_set_unit(float* __restrict__ x, int64_t n,
@@ -484,19 +511,24 @@ void launch_blend(float* x, const float* t, int64_t n, float a, float b, hipStre
 }
 
 void launch_rwr_update(float* y, const float* x, const float* r, int64_t n, float alpha,
-                       float beta, float* part, float* total, hipStream_t s) {
+                       float beta, float* part, float* total, float tol, float* done,
+                       hipStream_t s) {
     const unsigned p = parts_for(n);
     hipLaunchKernelGGL(k_rwr_update, dim3(p), dim3(kBlock), 0, s, y, x, r, n, alpha, beta,
-                       part);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, 0, total);
+                       part, done);
+    hipLaunchKernelGGL(k_fold_until, dim3(1), dim3(kBlock), 0, s, part, (int)p, 0, total, tol,
+                       done);
     check_launch("rwr update launch");
 }
 
 void launch_dpt_update(const float* pn, float* vis, float* hit, int64_t n, float it,
-                       float* part, float* max_unvisited, hipStream_t s) {
+                       float* part, float* max_unvisited, float tol, float* done,
+                       hipStream_t s) {
     const unsigned p = parts_for(n);
-    hipLaunchKernelGGL(k_dpt_update, dim3(p), dim3(kBlock), 0, s, pn, vis, hit, n, it, part);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(kBlock), 0, s, part, (int)p, 1, max_unvisited);
+    hipLaunchKernelGGL(k_dpt_update, dim3(p), dim3(kBlock), 0, s, pn, vis, hit, n, it, part,
+                       done);
+    hipLaunchKernelGGL(k_fold_until, dim3(1), dim3(kBlock), 0, s, part, (int)p, 1,
+                       max_unvisited, tol, done);
     check_launch("dpt update launch");
 }
 

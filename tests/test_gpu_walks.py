@@ -56,6 +56,35 @@ def test_random_walk_with_restart(gpu):
     assert np.all(diffusion.random_walk_with_restart(P, [-1]) == 0)
 
 
+def test_walks_stopping_inside_a_batch_of_steps(gpu):
+    """The device checks convergence every step and the host reads the flag
+    every few steps: walks that converge on the first, a middle or the last
+    step of such a batch give the restatement's result.  The RWR tolerance
+    is placed mid-way between two consecutive step totals (the device sums
+    them in another order); the DPT test takes a maximum, which is exact."""
+    g, P = op(sym=False)
+    seeds = [5, 17, 17]
+    grid = 10.0 ** np.linspace(-1, -8, 141)
+    stops = {}
+    for tol in grid:
+        stops.setdefault(orc.rwr_f32(g.indptr, g.indices, g.data, seeds, 0.85, 100,
+                                     tol)[1], []).append(tol)
+    checked = 0
+    for it in (3, 8, 9, 11, 16, 17):
+        if len(stops.get(it, [])) < 3:
+            continue
+        tol = float(np.median(stops[it]))
+        ref, n_it = orc.rwr_f32(g.indptr, g.indices, g.data, seeds, 0.85, 100, tol)
+        got = diffusion.random_walk_with_restart(P, seeds, 0.85, 100, tol)
+        assert n_it == it and np.array_equal(got, ref), it
+        checked += 1
+    assert checked >= 3
+    g, P = op(8000, 8, seed=4)
+    for tol in (0.5, 0.2, 0.05, 1e-2, 3e-3):
+        got = diffusion.compute_dpt(P, 77, 100, tol)
+        assert np.array_equal(got, orc.dpt_f32(g.indptr, g.indices, g.data, 77, 100, tol)), tol
+
+
 def test_diffusion_pseudotime(gpu):
     g, P = op(8000, 8, seed=4)
     for root, iters, tol in ((0, 60, 1e-6), (123, 0, 0.0), (4000, 25, 1e-3)):
