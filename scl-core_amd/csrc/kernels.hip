@@ -48,6 +48,37 @@ __device__ __forceinline__ bool block_coords(int n_rowblocks, int rb_per_xcd, in
     return true;
 }
 
+// F = 1: y = P x with one lane per row (rows in the visiting order).  A
+// lane reads its row's indices and values four at a time with 16-byte loads
+// once the position is a multiple of four (col and val 16-byte aligned),
+// then gathers the four x values together; the products are added in
+// stored order, one FMA each, the row-split kernel's chain.
+__global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const float* __restrict__ val,
+                                                      const int32_t* __restrict__ sched,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ y, int32_t n) {
+    const int rr = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (rr >= n) return;
+    const int row = sched ? sched[rr] : rr;
+    int k = row_ptr[row];
+    const int e = row_ptr[row + 1];
+    float acc = 0.0f;
+    for (; k < e && (k & 3); ++k) acc = __builtin_fmaf(val[k], x[col[k]], acc);
+    for (; k + 4 <= e; k += 4) {
+        const int4 c = *reinterpret_cast<const int4*>(col + k);
+        const f4 v = *reinterpret_cast<const f4*>(val + k);
+        const float x0 = x[c.x], x1 = x[c.y], x2 = x[c.z], x3 = x[c.w];
+        acc = __builtin_fmaf(v.x, x0, acc);
+        acc = __builtin_fmaf(v.y, x1, acc);
+        acc = __builtin_fmaf(v.z, x2, acc);
+        acc = __builtin_fmaf(v.w, x3, acc);
+    }
+    for (; k < e; ++k) acc = __builtin_fmaf(val[k], x[col[k]], acc);
+    y[row] = acc;
+}
+
 template <int VEC, int G, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -826,6 +857,14 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     }
     if (shape.rows_per_group > 0) {
         launch_spmm_grouped(P, X, Y, F, shape, s);
+        return;
+    }
+    if (F == 1 && shape.group == 1 && shape.vec == 1 &&
+        ((reinterpret_cast<uintptr_t>(P.col) | reinterpret_cast<uintptr_t>(P.val)) & 15) == 0) {
+        hipLaunchKernelGGL(k_spmv_lane, dim3((unsigned)((P.n + kBlock - 1) / kBlock)),
+                           dim3(kBlock), 0, s, P.row_ptr, P.col, P.val,
+                           shape.schedule ? P.sched : nullptr, X, Y, P.n);
+        check_launch("spmv launch");
         return;
     }
     const int rpb = (kBlock / 64) * (64 / shape.group);
