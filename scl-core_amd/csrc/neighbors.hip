@@ -62,10 +62,10 @@ __global__ __launch_bounds__(kBlock) void k_sum_squared(const int64_t* __restric
     if (lane == 0) norms[row] = r;
 }
 
-constexpr int kQ = 64;   // queries per workgroup
-constexpr int kC = 64;   // candidates per tile
-constexpr int kK = 16;   // features per LDS stage
-constexpr int kPad = 4;  // LDS row padding (floats)
+constexpr int kQ = 64;    // queries per workgroup
+constexpr int kC = 128;   // candidates per tile
+constexpr int kK = 16;    // features per LDS stage
+constexpr int kPad = 4;   // LDS row padding (floats)
 
 // (d, j) before (d', j'): smaller distance, then smaller index
 __device__ __forceinline__ bool before(float d, int j, float d2, int j2) {
@@ -73,14 +73,14 @@ __device__ __forceinline__ bool before(float d, int j, float d2, int j2) {
 }
 
 // One workgroup owns kQ query rows of the dense, zero-padded n_pad x f_pad
-// matrix X and scans every candidate tile.  Thread (ty, tx) computes the
-// 4 x 4 dot products of queries 4ty..4ty+3 with candidates 4tx..4tx+3, each
-// a sequential FMA chain over the features.  Candidates that beat a query's
-// current k-th entry (in (distance, index) order) go to an LDS buffer; after
-// each tile one thread per query sorts its buffer and merges it into the
-// query's sorted list in ld / lj (k entries per query, global memory).  At
-// the end the lists are written out as sqrt(distance) and index, padded
-// with inf / -1.
+// matrix X (n_pad a multiple of kC) and scans every candidate tile.  Thread
+// (ty, tx) computes the 4 x 8 dot products of queries 4ty..4ty+3 with
+// candidates 4tx..4tx+3 and 64+4tx..64+4tx+3, each a sequential FMA chain
+// over the features.  Candidates that beat a query's current k-th entry (in
+// (distance, index) order) go to an LDS buffer; after each tile one thread
+// per query sorts its buffer and merges it into the query's sorted list in
+// ld / lj (k entries per query, global memory).  At the end the lists are
+// written out as sqrt(distance) and index, padded with inf / -1.
 __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
                                                 const float* __restrict__ norms, int64_t n,
                                                 int64_t f_pad, int64_t k,
@@ -104,38 +104,45 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
         thr_j[tid] = 0x7fffffff;
         qn[tid] = q0 + tid < n ? norms[q0 + tid] : 0.0f;
     }
-    // loader mapping: thread tid moves 4 consecutive features of one row
+    // loader mapping: thread tid moves 4 consecutive features of one query
+    // row and of two candidate rows (lr and lr + 64)
     const int lr = tid >> 2, lk = (tid & 3) * 4;
     __syncthreads();
 
     for (int64_t c0 = 0; c0 < n; c0 += kC) {
-        float acc[4][4];
+        float acc[4][8];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) acc[r][c] = 0.0f;
+            for (int c = 0; c < 8; ++c) acc[r][c] = 0.0f;
         for (int64_t k0 = 0; k0 < f_pad; k0 += kK) {
             const f4 a = *reinterpret_cast<const f4*>(X + (q0 + lr) * f_pad + k0 + lk);
-            const f4 b = *reinterpret_cast<const f4*>(X + (c0 + lr) * f_pad + k0 + lk);
+            const f4 b0 = *reinterpret_cast<const f4*>(X + (c0 + lr) * f_pad + k0 + lk);
+            const f4 b1 = *reinterpret_cast<const f4*>(X + (c0 + 64 + lr) * f_pad + k0 + lk);
             As[lk + 0][lr] = a.x;
             As[lk + 1][lr] = a.y;
             As[lk + 2][lr] = a.z;
             As[lk + 3][lr] = a.w;
-            Bs[lk + 0][lr] = b.x;
-            Bs[lk + 1][lr] = b.y;
-            Bs[lk + 2][lr] = b.z;
-            Bs[lk + 3][lr] = b.w;
+            Bs[lk + 0][lr] = b0.x;
+            Bs[lk + 1][lr] = b0.y;
+            Bs[lk + 2][lr] = b0.z;
+            Bs[lk + 3][lr] = b0.w;
+            Bs[lk + 0][64 + lr] = b1.x;
+            Bs[lk + 1][64 + lr] = b1.y;
+            Bs[lk + 2][64 + lr] = b1.z;
+            Bs[lk + 3][64 + lr] = b1.w;
             __syncthreads();
 #pragma unroll
             for (int kk = 0; kk < kK; ++kk) {
                 const f4 av = *reinterpret_cast<const f4*>(&As[kk][ty * 4]);
-                const f4 bv = *reinterpret_cast<const f4*>(&Bs[kk][tx * 4]);
+                const f4 bv0 = *reinterpret_cast<const f4*>(&Bs[kk][tx * 4]);
+                const f4 bv1 = *reinterpret_cast<const f4*>(&Bs[kk][64 + tx * 4]);
                 const float ar[4] = {av.x, av.y, av.z, av.w};
-                const float bc[4] = {bv.x, bv.y, bv.z, bv.w};
+                const float bc[8] = {bv0.x, bv0.y, bv0.z, bv0.w, bv1.x, bv1.y, bv1.z, bv1.w};
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_fmaf(ar[r], bc[c], acc[r][c]);
+                    for (int c = 0; c < 8; ++c) acc[r][c] = __builtin_fmaf(ar[r], bc[c], acc[r][c]);
             }
             __syncthreads();
         }
@@ -146,8 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
             const int64_t q = q0 + ql;
             if (q >= n) continue;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const int64_t j = c0 + tx * 4 + c;
+            for (int c = 0; c < 8; ++c) {
+                const int64_t j = c0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
                 if (j >= n || j == q) continue;
                 float d2 = __builtin_fmaf(-2.0f, acc[r][c], qn[ql] + norms[j]);
                 if (d2 < 0.0f) d2 = 0.0f;
@@ -230,7 +237,7 @@ void launch_sum_squared(const int64_t* ip, const float* v, int64_t n, float* nor
     check_launch("norms launch");
 }
 
-int64_t knn_rows_padded(int64_t n) { return (n + kQ - 1) / kQ * kQ; }
+int64_t knn_rows_padded(int64_t n) { return (n + kC - 1) / kC * kC; }
 int64_t knn_features_padded(int64_t f) { return (f + kK - 1) / kK * kK; }
 
 void launch_knn(const float* X, const float* norms, int64_t n, int64_t f_pad, int64_t k,
