@@ -145,20 +145,39 @@ __global__ void k_symmetrize_serial(const int64_t* __restrict__ ip,
 // vector_sum of transition.hpp:165-179 over x[0, n) on one wave: lane l < 8
 // keeps the partial over l, l + 8, l + 16, ... of the full groups of eight
 // (chunks of 64 loaded by all lanes, handed over with shuffles), the eight
-// partials are added left to right, then the tail in order.  Every lane
-// returns the result.
+// partials are added left to right, then the tail in order.  The partials
+// are serial chains, so the loads run a batch of kSumBatch chunks ahead of
+// the adds (two register buffers).  Every lane returns the result.
+constexpr int kSumBatch = 16;
+
 __device__ float wave_sum8(const float* __restrict__ x, int64_t n) {
     const int lane = threadIdx.x & 63;
     const int64_t full = n / 8 * 8;
     float part = 0.0f;
-    for (int64_t c = 0; c < full; c += 64) {
-        const float val = c + lane < full ? x[c + lane] : 0.0f;
-        const int64_t lim = min((int64_t)64, full - c);
+    float cur[kSumBatch], nxt[kSumBatch];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const float o = __shfl(val, (lane & 7) + 8 * s, 64);
-            if (8 * s < lim) part = part + o;
+    for (int u = 0; u < kSumBatch; ++u) {
+        const int64_t i = 64 * u + lane;
+        cur[u] = i < full ? x[i] : 0.0f;
+    }
+    for (int64_t c = 0; c < full; c += 64 * kSumBatch) {
+#pragma unroll
+        for (int u = 0; u < kSumBatch; ++u) {
+            const int64_t i = c + 64 * (kSumBatch + u) + lane;
+            nxt[u] = i < full ? x[i] : 0.0f;
         }
+#pragma unroll
+        for (int u = 0; u < kSumBatch; ++u) {
+            const int64_t cb = c + 64 * u;
+            const int64_t lim = min((int64_t)64, full - cb);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const float o = __shfl(cur[u], (lane & 7) + 8 * s, 64);
+                if (8 * s < lim) part = part + o;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kSumBatch; ++u) cur[u] = nxt[u];
     }
     float r = 0.0f;
 #pragma unroll
