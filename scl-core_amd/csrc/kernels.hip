@@ -49,10 +49,11 @@ __device__ __forceinline__ bool block_coords(int n_rowblocks, int rb_per_xcd, in
 }
 
 // F = 1: y = P x with one lane per row (rows in the visiting order).  A
-// lane reads its row's indices and values four at a time with 16-byte loads
-// once the position is a multiple of four (col and val 16-byte aligned),
-// then gathers the four x values together; the products are added in
-// stored order, one FMA each, the row-split kernel's chain.
+// lane reads its row's indices and values with 16-byte loads once the
+// position is a multiple of four (col and val 16-byte aligned), sixteen
+// (then eight, then four) at a time, and gathers their x values together;
+// the products are added in stored order, one FMA each, the row-split
+// kernel's chain.
 __global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict__ row_ptr,
                                                       const int32_t* __restrict__ col,
                                                       const float* __restrict__ val,
@@ -66,7 +67,49 @@ __global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict_
     const int e = row_ptr[row + 1];
     float acc = 0.0f;
     for (; k < e && (k & 3); ++k) acc = __builtin_fmaf(val[k], x[col[k]], acc);
-    for (; k + 4 <= e; k += 4) {
+    // sixteen at a time: the four groups' x values are gathered together
+    for (; k + 16 <= e; k += 16) {
+        int4 c[4];
+        f4 v[4];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            c[g] = *reinterpret_cast<const int4*>(col + k + 4 * g);
+            v[g] = *reinterpret_cast<const f4*>(val + k + 4 * g);
+        }
+        float xs[16];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            xs[4 * g + 0] = x[c[g].x];
+            xs[4 * g + 1] = x[c[g].y];
+            xs[4 * g + 2] = x[c[g].z];
+            xs[4 * g + 3] = x[c[g].w];
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            acc = __builtin_fmaf(v[g].x, xs[4 * g + 0], acc);
+            acc = __builtin_fmaf(v[g].y, xs[4 * g + 1], acc);
+            acc = __builtin_fmaf(v[g].z, xs[4 * g + 2], acc);
+            acc = __builtin_fmaf(v[g].w, xs[4 * g + 3], acc);
+        }
+    }
+    // eight at a time: both groups' x values are gathered together
+    for (; k + 8 <= e; k += 8) {
+        const int4 c0 = *reinterpret_cast<const int4*>(col + k);
+        const int4 c1 = *reinterpret_cast<const int4*>(col + k + 4);
+        const f4 v0 = *reinterpret_cast<const f4*>(val + k);
+        const f4 v1 = *reinterpret_cast<const f4*>(val + k + 4);
+        const float x0 = x[c0.x], x1 = x[c0.y], x2 = x[c0.z], x3 = x[c0.w];
+        const float x4 = x[c1.x], x5 = x[c1.y], x6 = x[c1.z], x7 = x[c1.w];
+        acc = __builtin_fmaf(v0.x, x0, acc);
+        acc = __builtin_fmaf(v0.y, x1, acc);
+        acc = __builtin_fmaf(v0.z, x2, acc);
+        acc = __builtin_fmaf(v0.w, x3, acc);
+        acc = __builtin_fmaf(v1.x, x4, acc);
+        acc = __builtin_fmaf(v1.y, x5, acc);
+        acc = __builtin_fmaf(v1.z, x6, acc);
+        acc = __builtin_fmaf(v1.w, x7, acc);
+    }
+    if (k + 4 <= e) {
         const int4 c = *reinterpret_cast<const int4*>(col + k);
         const f4 v = *reinterpret_cast<const f4*>(val + k);
         const float x0 = x[c.x], x1 = x[c.y], x2 = x[c.z], x3 = x[c.w];
@@ -74,6 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict_
         acc = __builtin_fmaf(v.y, x1, acc);
         acc = __builtin_fmaf(v.z, x2, acc);
         acc = __builtin_fmaf(v.w, x3, acc);
+        k += 4;
     }
     for (; k < e; ++k) acc = __builtin_fmaf(val[k], x[col[k]], acc);
     y[row] = acc;
