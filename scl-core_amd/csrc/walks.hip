@@ -459,41 +459,61 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const float* __restrict__ 
 }
 
 // D[i][j] = D[j][i] = sqrt(sum_k (A[i][k] - A[j][k])^2) for i < j, summed in
-// k order (one thread per pair, 16 x 16 pairs per block, k in LDS chunks of
-// 32), D[i][i] = 0.  Only blocks on or above the diagonal work.
+// k order, D[i][i] = 0.  A workgroup takes 64 x 64 pairs (rows bi.., bj..)
+// and a thread 4 x 4 of them, each a sequential chain of FMAs; the two row
+// blocks pass through LDS 16 features at a time, stored feature-major so a
+// thread reads its four rows' values with one 16-byte load.  Only blocks on
+// or above the diagonal work.
 __global__ __launch_bounds__(kBlock) void k_pairwise(const float* __restrict__ A,
                                                      float* __restrict__ D, int64_t n) {
-    constexpr int T = 16, KC = 32;
-    __shared__ float ai[T][KC + 1], aj[T][KC + 1];
+    constexpr int T = 64, KC = 16, PAD = 4;
+    __shared__ float ai[KC][T + PAD], aj[KC][T + PAD];
     const int64_t bi = (int64_t)blockIdx.y * T, bj = (int64_t)blockIdx.x * T;
     if (bj + T <= bi) return;  // wholly below the diagonal
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const int64_t i = bi + ty, j = bj + tx;
-    float s = 0.0f;
+    float s[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s[r][c] = 0.0f;
     for (int64_t k0 = 0; k0 < n; k0 += KC) {
         for (int e = threadIdx.x; e < T * KC; e += kBlock) {
             const int r = e / KC, c = e % KC;
             const int64_t k = k0 + c;
-            ai[r][c] = (bi + r < n && k < n) ? A[(bi + r) * n + k] : 0.0f;
-            aj[r][c] = (bj + r < n && k < n) ? A[(bj + r) * n + k] : 0.0f;
+            ai[c][r] = (bi + r < n && k < n) ? A[(bi + r) * n + k] : 0.0f;
+            aj[c][r] = (bj + r < n && k < n) ? A[(bj + r) * n + k] : 0.0f;
         }
         __syncthreads();
         const int kmax = (int)(n - k0 < KC ? n - k0 : KC);
         for (int c = 0; c < kmax; ++c) {
-            const float d = ai[ty][c] - aj[tx][c];
-            s = __builtin_fmaf(d, d, s);
+            const f4 av = *reinterpret_cast<const f4*>(&ai[c][ty * 4]);
+            const f4 bv = *reinterpret_cast<const f4*>(&aj[c][tx * 4]);
+            const float ar[4] = {av.x, av.y, av.z, av.w};
+            const float bc[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float d = ar[r] - bc[q];
+                    s[r][q] = __builtin_fmaf(d, d, s[r][q]);
+                }
         }
         __syncthreads();
     }
-    if (i < n && j < n) {
-        if (i == j) {
-            D[i * n + i] = 0.0f;
-        } else if (i < j) {
-            const float dist = sqrtf(s);
-            D[i * n + j] = dist;
-            D[j * n + i] = dist;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t i = bi + ty * 4 + r, j = bj + tx * 4 + q;
+            if (i >= n || j >= n) continue;
+            if (i == j) {
+                D[i * n + i] = 0.0f;
+            } else if (i < j) {
+                const float dist = sqrtf(s[r][q]);
+                D[i * n + j] = dist;
+                D[j * n + i] = dist;
+            }
         }
-    }
 }
 
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
@@ -657,7 +677,7 @@ void launch_transpose_square(const float* A, float* B, int64_t n, hipStream_t s)
 
 void launch_pairwise_rows(const float* A, float* D, int64_t n, hipStream_t s) {
     if (n <= 0) return;
-    const unsigned g = (unsigned)((n + 15) / 16);
+    const unsigned g = (unsigned)((n + 63) / 64);
     hipLaunchKernelGGL(k_pairwise, dim3(g, g), dim3(kBlock), 0, s, A, D, n);
     check_launch("pairwise distance launch");
 }
