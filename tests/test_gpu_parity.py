@@ -110,6 +110,25 @@ def test_ragged_rows_empty_long_unsorted(gpu):
     assert np.all(Y[1:50] == 0)
 
 
+@pytest.mark.parametrize("n", [5000, 20_000])
+def test_single_feature_rows_of_every_length(gpu, n):
+    """F = 1 runs one lane per row, reading indices and values 16 bytes at a
+    time once aligned, 16, 8 then 4 at a time, then the tail: rows of 0 to
+    40 nonzeros starting at every offset modulo 4 cover each path, with and
+    without the locality schedule (from 16,384 rows)."""
+    rng = np.random.Generator(np.random.PCG64(n))
+    lens = np.arange(n) % 41
+    rng.shuffle(lens)
+    ip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ix = np.concatenate([np.sort(rng.integers(0, n, l)) for l in lens]).astype(np.int64)
+    d = rng.random(ip[-1]).astype(np.float32)
+    x = synth.normal(n, 1, seed=n)
+    assert np.array_equal(gpu_diffuse(ip, ix, d, x, 3), orc.diffuse_matrix_f32(ip, ix, d, x, 3))
+    P = CsrMatrix.from_arrays(ip, ix, d, (n, n))
+    got = diffusion.diffuse_vector(P, x[:, 0].copy(), 4)
+    assert np.array_equal(got, orc.diffuse_vector_f32(ip, ix, d, x[:, 0], 4))
+
+
 def test_no_nonzeros_and_single_node(gpu):
     Y = gpu_diffuse(np.zeros(6, np.int64), np.zeros(0, np.int64),
                     np.zeros(0, np.float32), np.ones((5, 3), np.float32), 2)
