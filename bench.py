@@ -610,12 +610,21 @@ def main() -> int:
     achieved = B / (ms_per_launch / 1e3) / 1e9
     log(f"{ms_per_step:.3f} ms/diffusion, {ms_per_launch:.4f} ms/launch, {value:.1f} GB/s")
 
-    # the drop-in host-pointer call (PCIe included), once, for reference
+    # the drop-in host-pointer call (PCIe included), for reference: one call
+    # to set up its buffers, then the median of three on fresh copies of X
+    # (X is written in place)
     Xh = X.copy()
-    t0 = time.perf_counter()
     diffusion.diffuse_matrix(A, Xh, t)
-    host_ms = (time.perf_counter() - t0) * 1e3
     same = bool(np.array_equal(Xh, Yd.cpu().numpy()))
+    host_times = []
+    for _ in range(3):
+        Xh = X.copy()
+        t0 = time.perf_counter()
+        diffusion.diffuse_matrix(A, Xh, t)
+        host_times.append((time.perf_counter() - t0) * 1e3)
+    host_ms = float(np.median(host_times))
+    same = same and bool(np.array_equal(Xh, Yd.cpu().numpy()))
+    del Xh
 
     traffic = None
     if TRAFFIC_FILE.exists():

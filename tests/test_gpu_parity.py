@@ -179,6 +179,34 @@ def test_device_api_matches_drop_in(gpu):
     assert torch.equal(Xd.cpu(), torch.from_numpy(X))
 
 
+def test_drop_in_wide_host_matrix(gpu):
+    """A 288 MB host X (copied through the pinned bounce buffers in chunks):
+    bit-identical to the device API for even and odd t, under a tuned
+    row-grouped shape and with MAGIC's default t, and to the oracle on
+    sampled columns."""
+    import torch
+    from scl_amd import torch_ops
+    n, F = 120_000, 600
+    g = synth.knn_graph(n, 10, seed=31).normalized()
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    X = synth.normal(n, F, seed=31)
+    Xd = torch.from_numpy(X).cuda()
+    for t in (2, 3):
+        Y = X.copy()
+        diffusion.diffuse_matrix(P, Y, t)
+        assert np.array_equal(Y, torch_ops.diffuse(P, Xd, t).cpu().numpy()), t
+    cols = np.array([0, 255, 256, 511, 512, 599])
+    assert np.array_equal(Y[:, cols], orc.diffuse_matrix_f32(g.indptr, g.indices, g.data,
+                                                             X[:, cols], 3))
+    torch_ops.set_shape(P, F, 4, 64, 16, nontemporal=True, rows_per_group=4)
+    Z = X.copy()
+    diffusion.diffuse_matrix(P, Z, 3)
+    assert np.array_equal(Z, Y)
+    M = X.copy()
+    diffusion.magic_impute(P, M, 0)
+    assert np.array_equal(M, Y)
+
+
 @pytest.mark.parametrize("symmetric", [False, True])
 def test_transition_matrix_fp64(gpu, symmetric):
     g = synth.knn_graph(5000, 10, seed=21)
