@@ -552,7 +552,7 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
     g.grow = dev_alloc<int32_t>(grow.size(), "grouped rows");
     g.gword = dev_alloc<uint32_t>((size_t)std::max<int64_t>(ne, 1), "grouped words");
     g.gperm = dev_alloc<int32_t>((size_t)std::max<int64_t>(nnz, 1), "grouped perm");
-    g.gval = dev_alloc<float>((size_t)nnz + 64, "grouped values");
+    g.gval = dev_alloc<float>((size_t)nnz + 128, "grouped values");
     hip_check(hipMemcpy(g.gptr, gptr.data(), gptr.size() * 4, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(g.gvptr, gvptr.data(), gvptr.size() * 4, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(g.grow, grow.data(), grow.size() * 4, hipMemcpyHostToDevice), "upload");
@@ -560,7 +560,7 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
         hip_check(hipMemcpy(g.gword, gword.data(), (size_t)ne * 4, hipMemcpyHostToDevice), "upload");
     if (nnz > 0)
         hip_check(hipMemcpy(g.gperm, gperm.data(), (size_t)nnz * 4, hipMemcpyHostToDevice), "upload");
-    hip_check(hipMemset(g.gval, 0, ((size_t)nnz + 64) * sizeof(float)), "clear grouped values");
+    hip_check(hipMemset(g.gval, 0, ((size_t)nnz + 128) * sizeof(float)), "clear grouped values");
     launch_gather_f32(g.gperm, d.val, g.gval, nnz, nullptr);
     hip_check(hipStreamSynchronize(nullptr), "grouped values");
     g.R = R;

@@ -498,7 +498,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
             for (int R : {4, 8}) {
                 if (!ensure_grouped(transition, d, R)) break;
                 for (int vec : {4, 2})
-                    for (int u : {R == 8 ? 4 : 8, R == 8 ? 8 : 16})
+                    for (int u : R == 8 ? std::vector<int>{4, 8, 16} : std::vector<int>{8, 16})
                         for (int occ : {0, 4})
                             for (int nt = 0; nt < 2; ++nt) {
                                 SpmmShape c;

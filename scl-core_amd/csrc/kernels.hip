@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int32_t* __restrict__ gvptr, const float* __restrict__ gval,
     const int32_t* __restrict__ grow, const float* __restrict__ X, float* __restrict__ Y,
     int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd) {
-    static_assert(U * R <= 64, "a chunk's values must fit one per lane");
+    static_assert(U * R <= 128, "a chunk's values must fit two per lane");
     using VT = typename Vec<VEC>::T;
     int slab, gb;
     if (!block_coords(n_gblocks, rb_per_xcd, slab, gb)) return;
@@ -258,8 +258,10 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
 #pragma unroll
         for (int u = 0; u < U; ++u)
             x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * F);
-        // this chunk's values, one per lane (gval is padded by 64)
+        // this chunk's values, one per lane, and the next 64 when a chunk
+        // can hold more (gval is padded by 128)
         const int vl = __builtin_bit_cast(int, gval[vo + lane]);
+        const int vh = U * R > 64 ? __builtin_bit_cast(int, gval[vo + 64 + lane]) : 0;
         int c = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -268,7 +270,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if (m & (1u << r)) {
-                        const float v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(vl, c));
+                        const float v = __builtin_bit_cast(
+                            float, U * R > 64 && c >= 64 ? __builtin_amdgcn_readlane(vh, c - 64)
+                                                         : __builtin_amdgcn_readlane(vl, c));
                         ++c;
                         fma_vec(v, x[u], acc[r]);
                     }
@@ -863,7 +867,7 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
     if (!G || G->R != R || G->n_groups * R < P.n)
         throw std::runtime_error("grouped form of the operator missing for this launch");
     const int lds = occupancy_lds_bytes(shape.wg_per_cu);
-    // U entries per chunk with U * R <= 64 (a chunk's values fit one per lane)
+    // U entries per chunk with U * R <= 128 (a chunk's values fit two per lane)
 #define SCL_GROUPED(V, RR, UU)                                                          \
     (shape.nontemporal                                                                  \
          ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape.xcd_rows, lds, s)     \
@@ -875,7 +879,8 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
         else SCL_GROUPED(1, RR, UU);                   \
     } while (0)
     if (R == 8) {
-        if (shape.unroll >= 8) SCL_GROUPED_V(8, 8);
+        if (shape.unroll >= 16) SCL_GROUPED_V(8, 16);
+        else if (shape.unroll >= 8) SCL_GROUPED_V(8, 8);
         else SCL_GROUPED_V(8, 4);
     } else {
         if (shape.unroll >= 16) SCL_GROUPED_V(4, 16);

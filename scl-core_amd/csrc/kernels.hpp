@@ -31,7 +31,7 @@ struct GroupedCsr {
     int32_t* gptr = nullptr;      // n_groups + 1: first entry of each group
     uint32_t* gword = nullptr;    // n_entries: column | mask << 24
     int32_t* gvptr = nullptr;     // n_groups + 1: first value of each group
-    float* gval = nullptr;        // nnz + 64 (padded for whole-wave loads)
+    float* gval = nullptr;        // nnz + 128 (padded for two whole-wave loads)
     int32_t* gperm = nullptr;     // nnz: gval[i] = val[gperm[i]]
     int32_t* grow = nullptr;      // n_groups * R: the rows, -1 past the end
     uint64_t fingerprint = 0;     // pattern it was built for

@@ -313,13 +313,15 @@ def test_row_grouped_shapes_are_bit_identical(gpu, F):
         X = synth.normal(n, F, seed=F)
         Xd = torch.from_numpy(X).cuda()
         vmax = 4 if F % 4 == 0 else (2 if F % 2 == 0 else 1)
+        # chunks of 8 or 16 entries: up to 128 values per chunk with 8 rows
         for R in (4, 8):
-            for vec in sorted({1, vmax}):
-                torch_ops.set_shape(A, F, vec, 64, 8, rows_per_group=R,
-                                    nontemporal=vec == 1, wg_per_cu=4 if R == 8 else 0)
-                assert torch_ops.get_shape(A, F)["rows_per_group"] == R
-                Y = torch_ops.spmm(A, Xd).cpu().numpy()
-                assert np.array_equal(Y, orc.spmm_step_f32(ip, ix, p, X)), (n, R, vec)
+            for u in (8, 16):
+                for vec in sorted({1, vmax}):
+                    torch_ops.set_shape(A, F, vec, 64, u, rows_per_group=R,
+                                        nontemporal=vec == 1, wg_per_cu=4 if R == 8 else 0)
+                    assert torch_ops.get_shape(A, F)["rows_per_group"] == R
+                    Y = torch_ops.spmm(A, Xd).cpu().numpy()
+                    assert np.array_equal(Y, orc.spmm_step_f32(ip, ix, p, X)), (n, R, u, vec)
         # values change in place (normalisation): the grouped values follow
         diffusion.compute_transition_matrix(A)
         ip, ix, p = A.export()
