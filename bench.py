@@ -65,6 +65,19 @@ def step_bytes(n: int, nnz: int, F: int, vb=4, xb=4, yb=4) -> int:
     return nnz * (vb + 4) + 4 * (n + 1) + n * F * (xb + yb)
 
 
+def host_median_ms(fn, reps: int) -> float:
+    """Median wall-clock milliseconds of fn() over reps calls after one
+    untimed call (host-array entry points: their copies are part of it)."""
+    import numpy as np
+    fn()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(times))
+
+
 def timed(fn, reps: int, stream) -> float:
     """Mean milliseconds of fn() over reps, HIP events on `stream`."""
     import torch
@@ -296,11 +309,7 @@ def leg_wk(steps: int, with_cpu: bool = True) -> dict:
         out["cpu_threads"] = orc.num_threads()
 
     def record(name, gpu_fn, cpu_fn, reps):
-        gpu_fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            gpu_fn()
-        g_ms = (time.perf_counter() - t0) * 1e3 / reps
+        g_ms = host_median_ms(gpu_fn, reps)
         out[name] = {"gpu_ms": round(g_ms, 2)}
         if with_cpu:
             t0 = time.perf_counter()
@@ -309,12 +318,12 @@ def leg_wk(steps: int, with_cpu: bool = True) -> dict:
             out[name].update({"cpu_ms": round(c_ms, 1), "speedup": round(c_ms / g_ms, 1)})
 
     for name, (gpu_fn, cpu_fn) in cases.items():
-        record(name, gpu_fn, cpu_fn, max(1, steps // 5))
+        record(name, gpu_fn, cpu_fn, max(3, steps // 2))
     # dense output: diffusion distance on a 4000-node graph
     gs = synth.knn_graph(4000, 15, seed=7).normalized()
     Ps = CsrMatrix.from_arrays(gs.indptr, gs.indices, gs.data, (4000, 4000))
     record("distance_n4000_t3", lambda: diffusion.diffusion_distance(Ps, 3),
-           lambda: orc.diffusion_distance_f32(gs.indptr, gs.indices, gs.data, 3), 1)
+           lambda: orc.diffusion_distance_f32(gs.indptr, gs.indices, gs.data, 3), 3)
     return out
 
 
@@ -371,12 +380,7 @@ def leg_cl(steps: int, with_cpu: bool = True) -> dict:
             lambda: orc.algebra_spmm_f32(g.indptr, g.indices, g.data, Xa, Ya, 0.5, 2.0)),
     }
     for name, (gpu_fn, cpu_fn) in cases.items():
-        gpu_fn()
-        reps = max(1, steps // 5)
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            gpu_fn()
-        g_ms = (time.perf_counter() - t0) * 1e3 / reps
+        g_ms = host_median_ms(gpu_fn, max(3, steps // 2))
         out[name] = {"gpu_ms": round(g_ms, 2)}
         if with_cpu:
             t0 = time.perf_counter()
@@ -410,16 +414,12 @@ def leg_kn(steps: int, with_cpu: bool = True) -> dict:
         return np.cumsum(ip), c.astype(np.int64), rng.gamma(1.0, 1.0, len(c)).astype(np.float32)
 
     def timed(fn, reps):
-        fn()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            fn()
-        return (time.perf_counter() - t0) * 1e3 / reps
+        return host_median_ms(fn, reps)
 
     for n, f, cpu in ((20_000, 500, with_cpu), (60_000, 500, False)):
         ip, ix, v = samples(n, f, 11 + n)
         M = CsrMatrix.from_arrays(ip, ix, v, (n, f))
-        g_ms = timed(lambda: construct.knn(M, 15), 1)
+        g_ms = timed(lambda: construct.knn(M, 15), 3)
         f_pad = (f + 15) // 16 * 16
         # every pair's dot product over the padded features, 2 flops per FMA
         rec = {"gpu_ms": round(g_ms, 1),
@@ -443,7 +443,7 @@ def leg_kn(steps: int, with_cpu: bool = True) -> dict:
                             lambda: orc.stationary_f32(g.indptr, g.indices, g.data, 1000, 1e-6)),
     }
     for name, (gpu_fn, cpu_fn) in cases.items():
-        rec = {"gpu_ms": round(timed(gpu_fn, max(1, steps // 5)), 2)}
+        rec = {"gpu_ms": round(timed(gpu_fn, max(3, steps // 2)), 2)}
         if with_cpu:
             t0 = time.perf_counter()
             cpu_fn()
