@@ -26,12 +26,20 @@ struct DeviceState {
     hipEvent_t bounce_done[2] = {nullptr, nullptr};
 };
 
-// Host copies at least this large go through the pinned bounce buffers
-// (smaller ones are one chunk, where the host copy and the transfer cannot
-// overlap, and the runtime's own pageable path is as fast).
-constexpr size_t kBounceMin = 64u << 20;
+// Host copies at least this large go through the pinned bounce buffers, in
+// chunks of a quarter of the copy (at least kBounceChunkMin, at most
+// kBounceBytes) so that the host copy of one chunk overlaps the transfer of
+// the previous one; the runtime's own pageable path is as fast only for
+// small copies.
+constexpr size_t kBounceMin = 8u << 20;
+constexpr size_t kBounceChunkMin = 4u << 20;
 constexpr size_t kBounceBytes = 64u << 20;
 constexpr int kCopyThreads = 8;
+
+size_t bounce_chunk(size_t bytes) {
+    const size_t quarter = (bytes / 4 + 4095) / 4096 * 4096;
+    return std::min(kBounceBytes, std::max(kBounceChunkMin, quarter));
+}
 
 // dst = src on up to kCopyThreads host threads (chunks of 1 MiB or more)
 void parallel_memcpy(void* dst, const void* src, size_t bytes) {
@@ -128,9 +136,10 @@ void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s, int de
     // chunk i: wait until buffer i % 2 is free, fill it on the host, queue
     // its copy; the copy engine moves one buffer while the host fills the
     // other
-    for (size_t off = 0, i = 0; off < bytes; off += kBounceBytes, ++i) {
+    const size_t chunk = bounce_chunk(bytes);
+    for (size_t off = 0, i = 0; off < bytes; off += chunk, ++i) {
         const int b = (int)(i & 1);
-        const size_t len = std::min(kBounceBytes, bytes - off);
+        const size_t len = std::min(chunk, bytes - off);
         hip_check(hipEventSynchronize(st.bounce_done[b]), "staging wait");
         parallel_memcpy(st.bounce[b], static_cast<const char*>(src) + off, len);
         hip_check(hipMemcpyAsync(static_cast<char*>(dst) + off, st.bounce[b], len,
@@ -155,9 +164,10 @@ void download_host(void* dst, const void* src, size_t bytes, hipStream_t s, int 
     // empties chunk i - 1 from the other buffer
     size_t prev_off = 0, prev_len = 0;
     int prev_b = -1;
-    for (size_t off = 0, i = 0; off < bytes; off += kBounceBytes, ++i) {
+    const size_t chunk = bounce_chunk(bytes);
+    for (size_t off = 0, i = 0; off < bytes; off += chunk, ++i) {
         const int b = (int)(i & 1);
-        const size_t len = std::min(kBounceBytes, bytes - off);
+        const size_t len = std::min(chunk, bytes - off);
         hip_check(hipEventSynchronize(st.bounce_done[b]), "staging wait");
         hip_check(hipMemcpyAsync(st.bounce[b], static_cast<const char*>(src) + off, len,
                                  hipMemcpyDeviceToHost, s),

@@ -50,9 +50,14 @@ def matrix_from_velocity(graph: SparseMatrixBase) -> np.ndarray:
 def row_normalize(matrix: SparseMatrixBase, n: int = -1) -> np.ndarray:
     """scl_transition_row_normalize_to_stochastic over the first n rows."""
     n = matrix.shape[0] if n < 0 else n
-    ip, _, _ = matrix.export()
-    # rows past the handle's are the library's error to report
-    out = np.empty(int(ip[min(n, len(ip) - 1)] - ip[0]), np.float32)
+    prim = matrix.shape[0] if matrix._is_csr else matrix.shape[1]
+    if n == prim:
+        size = matrix.nnz     # every stored entry: no need to export the arrays
+    else:
+        ip, _, _ = matrix.export()
+        # rows past the handle's are the library's error to report
+        size = int(ip[min(n, len(ip) - 1)] - ip[0])
+    out = np.empty(size, np.float32)
     _lib.check(_lib.load().scl_transition_row_normalize_to_stochastic(
         matrix.handle, n, out.ctypes.data, len(out)))
     return out
