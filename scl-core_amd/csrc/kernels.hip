@@ -254,14 +254,16 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
         uint32_t w[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) w[u] = gword[min(e + u, e1 - 1)];
+        // this chunk's values, one per lane, and the next 64 when a chunk
+        // can hold more (gval is padded by 128); loaded ahead of the gathers
+        // so that waiting for them does not wait for the gathers too
+        const int vl = __builtin_bit_cast(int, gval[vo + lane]);
+        const int vh = U * R > 64 ? __builtin_bit_cast(int, gval[vo + 64 + lane]) : 0;
+        __builtin_amdgcn_sched_barrier(0);
         VT x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
             x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * F);
-        // this chunk's values, one per lane, and the next 64 when a chunk
-        // can hold more (gval is padded by 128)
-        const int vl = __builtin_bit_cast(int, gval[vo + lane]);
-        const int vh = U * R > 64 ? __builtin_bit_cast(int, gval[vo + 64 + lane]) : 0;
         int c = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
