@@ -41,6 +41,30 @@ def test_kernel_weights(gpu, kind):
             assert np.allclose(sums, rsum, rtol=2e-6, atol=1e-7), (kind, h)
 
 
+def test_mid_size_host_copies(gpu):
+    """Host arrays of 8 to 64 MiB cross through the pinned bounce buffers in
+    chunks of a quarter of the copy (at least 4 MiB): 3,000,027 values
+    (12 MB in, 12 MB out; chunks of 4, 4 and 3.4 MB) come back exactly."""
+    n, k = 300_001, 10
+    g = synth.uniform_knn(n, k, seed=41)
+    rng = np.random.Generator(np.random.PCG64(41))
+    d = rng.gamma(2.0, 0.6, len(g.data)).astype(np.float32)
+    ip = g.indptr.copy()
+    ix = g.indices.copy()
+    # 17 more entries on the last row: the copy is not a multiple of 4 KiB
+    extra = np.setdiff1d(np.arange(40), ix[ip[-2]:ip[-1]])[:17]
+    ix = np.concatenate([ix, extra]).astype(np.int64)
+    d = np.concatenate([d, rng.gamma(2.0, 0.6, 17).astype(np.float32)])
+    ip[-1] += 17
+    order = np.argsort(ix[ip[-2]:], kind="stable") + ip[-2]
+    ix[ip[-2]:], d[ip[-2]:] = ix[order], d[order]
+    D = CsrMatrix.from_arrays(ip, ix, d, (n, n))
+    assert D.nnz == len(d) == 3_000_027
+    got = construct.kernel_matrix(D, 1.3, "epanechnikov")
+    assert np.array_equal(got, orc.kernel_matrix_f32(d, 1.3, 1))
+    assert np.array_equal(construct.row_normalize(D), orc.row_normalize_f32(ip, d))
+
+
 def test_row_normalize(gpu):
     ip, ix, d = distances(30_000, 10, seed=32)
     d[ip[5]:ip[6]] = 0.0                     # a zero row
