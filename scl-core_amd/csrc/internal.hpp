@@ -148,8 +148,9 @@ hipStream_t library_stream(int device);
 // device's staging buffers).
 std::mutex& device_mutex(int device);
 // Copies between pageable caller memory and device memory, stream-ordered
-// on s: copies of 64 MiB or more go through the device's pinned bounce
-// buffers (host side on several threads, overlapped with the copy engine).  upload_host
+// on s: copies of 8 MiB or more go through the device's two 64 MiB pinned
+// bounce buffers in chunks (host side on several threads, overlapped with
+// the copy engine).  upload_host
 // returns once the caller's buffer has been read; download_host returns
 // once the data is in the caller's buffer.
 void upload_host(void* dst, const void* src, size_t bytes, hipStream_t s, int device);
