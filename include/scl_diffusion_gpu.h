@@ -75,8 +75,9 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 /* ---------------------------------------------------------------- shapes --
  * Launch shape of the row-split SpMM for one operator (transpose == 0) or its
  * transpose (transpose == 1), or of the fused adjoint + edge-gradient step of
- * the reverse pass (transpose == 2: vec, group and SCL_GPU_SHAPE_NO_SCHEDULE;
- * unroll is 8 and the other flags are ignored), and one feature count:
+ * the reverse pass (transpose == 2: vec, group, SCL_GPU_SHAPE_NO_SCHEDULE and
+ * SCL_GPU_SHAPE_OCCUPANCY; unroll is 8 and the other flags are ignored), and
+ * one feature count:
  *   vec     floats per lane per access: 1, 2 or 4 (must divide n_features)
  *   group   lanes sharing one row: a power of two in 1..64; one launch covers
  *           group * vec features of every row per workgroup column (a slab)
@@ -108,8 +109,8 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           walking the sorted union of their column indices and gathering
  *           each X row once for all the rows that hold it; every row keeps
  *           its sorted-order FMA chain (bit-identical); unroll is the
- *           union entries gathered per chunk (4 rows: 8 or 16, 8 rows: 4 or
- *           8; larger values are clamped).  Needs strictly
+ *           union entries gathered per chunk (4 rows: 8 or 16, 8 rows: 4, 8
+ *           or 16; larger values are clamped).  Needs strictly
  *           increasing column indices in every row and < 2^24 columns;
  *           setting it builds the grouped form on the host (one pass over
  *           the pattern), which follows later value changes

@@ -367,7 +367,6 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
         sh.nontemporal = false;
         sh.xcd_rows = false;
         sh.scalar_rows = false;
-        sh.wg_per_cu = 0;
         SCL_API_CHECK(adjoint_shape_valid(sh, n_features, nullptr, nullptr, nullptr),
                       SCL_ERROR_INVALID_ARGUMENT,
                       "invalid adjoint shape: vec in {1,2,4} dividing n_features, "

@@ -393,6 +393,7 @@ bool adjoint_shape_valid(const SpmmShape& sh, int64_t F, const void* a, const vo
                          const void* c) {
     if (sh.vec != 1 && sh.vec != 2 && sh.vec != 4) return false;
     if (!pow2_in(sh.group, 1, 64) || F % sh.vec != 0) return false;
+    if (sh.wg_per_cu < 0 || sh.wg_per_cu > kShapeOccupancyMask) return false;
     const auto al = reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b) |
                     reinterpret_cast<uintptr_t>(c);
     return al % (4 * (uintptr_t)sh.vec) == 0;
@@ -408,13 +409,16 @@ std::vector<SpmmShape> adjoint_candidate_shapes(int64_t F, const void* a, const 
         for (int g = 4; g <= gmax; g <<= 1) {
             if (g < gmax / 4) continue;             // many passes over the pattern
             if (vec < 4 && g != gmax) continue;     // narrow loads only as one slab
-            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc) {
-                SpmmShape sh;
-                sh.vec = vec;
-                sh.group = g;
-                sh.schedule = sc != 0;
-                if (adjoint_shape_valid(sh, F, a, b, c)) out.push_back(sh);
-            }
+            for (int sc = has_schedule ? 1 : 0; sc >= 0; --sc)
+                // fewer resident workgroups keep fewer rows' gathers in L2
+                for (int occ : {0, 4}) {
+                    SpmmShape sh;
+                    sh.vec = vec;
+                    sh.group = g;
+                    sh.schedule = sc != 0;
+                    sh.wg_per_cu = occ;
+                    if (adjoint_shape_valid(sh, F, a, b, c)) out.push_back(sh);
+                }
         }
     }
     return out;
@@ -426,10 +430,12 @@ void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
     if (PT.n <= 0 || F <= 0) return;
     const bool edge = Xprev != nullptr && dPT != nullptr;
     int vec, gsel;
+    unsigned lds = 0;
     const int32_t* sched = PT.sched;
     if (shape && adjoint_shape_valid(*shape, F, Gs, Gp, Xprev)) {
         vec = shape->vec;
         gsel = shape->group;
+        lds = (unsigned)occupancy_lds_bytes(shape->wg_per_cu);
         if (!shape->schedule) sched = nullptr;
     } else {
         const auto a = reinterpret_cast<uintptr_t>(Gs) | reinterpret_cast<uintptr_t>(Gp) |
@@ -455,7 +461,7 @@ void launch_adjoint_edge(const CsrView& PT, const float* Gs, const float* Xprev,
         AdjFn fn = vec == 4 ? pick_adj<4>(group, nc, edge)
                  : vec == 2 ? pick_adj<2>(group, nc, edge)
                             : pick_adj<1>(group, nc, edge);
-        hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(kBlock), 0, s, PT.row_ptr,
+        hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(kBlock), lds, s, PT.row_ptr,
                            PT.col, PT.val, sched, Gs, Xprev, Gp, dPT,
                            (accumulate || f0 > 0) ? 1 : 0, PT.n, (int32_t)F,
                            (int32_t)f0, (int32_t)Fp);

@@ -124,9 +124,10 @@ def test_unfused_building_blocks_agree_with_fused(gpu):
 
 @pytest.mark.parametrize("F", [64, 128, 300])
 def test_every_adjoint_shape_agrees(gpu, F):
-    """Every (vec, group) of the fused adjoint kernel: dX bit-identical, dP
-    equal up to the order of its per-feature sums, and both within 1e-3 of
-    the FP64 gradient; autotuning picks one of them."""
+    """Every (vec, group) of the fused adjoint kernel, with and without an
+    occupancy cap: dX bit-identical, dP equal up to the order of its
+    per-feature sums, and both within 1e-3 of the FP64 gradient; autotuning
+    picks one of them."""
     n, t = 17_000, 3   # above the size that gets a locality schedule
     g, X, T = _problem(n, 9, F, seed=F)
     P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
@@ -135,10 +136,11 @@ def test_every_adjoint_shape_agrees(gpu, F):
     ref = None
     shapes = [(v, gr, sc) for v in (4, 2, 1) if F % v == 0 for gr in (4, 8, 16, 32, 64)
               for sc in (True, False)]
-    for v, gr, sc in shapes:
-        torch_ops.set_shape(P, F, v, gr, schedule=sc, adjoint=True)
+    for i, (v, gr, sc) in enumerate(shapes):
+        occ = 4 if i % 2 else 0
+        torch_ops.set_shape(P, F, v, gr, schedule=sc, adjoint=True, wg_per_cu=occ)
         got = torch_ops.get_shape(P, F, adjoint=True)
-        assert (got["vec"], got["group"], got["schedule"]) == (v, gr, sc)
+        assert (got["vec"], got["group"], got["schedule"], got["wg_per_cu"]) == (v, gr, sc, occ)
         Xr = Xd.clone().requires_grad_(True)
         vals = torch.from_numpy(g.data.copy()).cuda().requires_grad_(True)
         Y = autograd.diffuse(P, Xr, t, values=vals, checkpoint_every=2)
