@@ -174,8 +174,9 @@ void refresh_grouped_values(DeviceCSR& d, hipStream_t s);
 // Operators with at least this many rows get a locality schedule.
 constexpr int64_t kScheduleMinRows = 16384;
 
-// Cuthill-McKee visiting order of a CSR pattern, sorted by decreasing row
-// length within windows of 64 rows (schedule.cpp).
+// Greedy locality visiting order of a CSR pattern: each next row is the
+// unplaced one with the most pattern edges to the last 512 placed rows
+// (schedule.cpp).
 template <typename I>
 std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 
