@@ -586,7 +586,8 @@ def main() -> int:
                             schedule=not (fl & 2), xcd_rows=bool(fl & 4),
                             scalar_rows=bool(fl & 8), wg_per_cu=(fl >> 8) & 0xF,
                             merge=(fl >> 12) & 0xF,
-                            rows_per_group=(0, 4, 8, 64)[(fl >> 16) & 0x3])
+                            rows_per_group=(torch_ops.TILED if fl & (1 << 18)
+                                            else (0, 4, 8, 64)[(fl >> 16) & 0x3]))
         log(f"shape set to {torch_ops.get_shape(A, F)}")
     elif not args.no_autotune:
         t0 = time.perf_counter()

@@ -67,7 +67,7 @@ SCL_EXPORT scl_error_t scl_gpu_compute_transition_matrix_ex(
 
 /* Host-only (needs no GPU): the locality row order the device uses for
  * operators of >= 16384 rows -- greedy: each next row is the unplaced row
- * with the most pattern edges (either direction) to the last 256 placed
+ * with the most pattern edges (either direction) to the last 512 placed
  * rows.  order_out receives a permutation of 0..primary-1. */
 SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
                                                  scl_index_t* order_out);
@@ -121,7 +121,21 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           for the whole group, and every row reads its neighbours there in
  *           sorted order (bit-identical).  Needs strictly increasing column
  *           indices in every row; setting it builds the staged pattern on
- *           the host (the values stay the operator's own)
+ *           the host (the values stay the operator's own);
+ *           SCL_GPU_SHAPE_TILED (vec 4, transpose 0): LDS tiles -- rows of
+ *           the visiting order are cut into tiles whose column union fits
+ *           the LDS of one workgroup; per slab of group * 4 features
+ *           (group = 4 or 8 lanes per row) the workgroup copies the union's
+ *           X rows into LDS once, with direct global-to-LDS loads, and each
+ *           row (one group of lanes) reads its neighbours there in stored
+ *           order (bit-identical).  SCL_GPU_SHAPE_OCCUPANCY(b), b = 1 or 2,
+ *           is the LDS budget (1: 160 KiB tiles, one workgroup per CU; 2: 80
+ *           KiB tiles, two); unroll (4, 8 or 16) is the slabs one
+ *           workgroup computes.  Needs strictly increasing column indices in
+ *           every row, rows of at most 480-1920 nonzeros (budget and group)
+ *           and columns * n_features < 2^32; setting it builds the tiled
+ *           form on the host (one pass over the pattern plus a sort per
+ *           tile), which follows later value changes
  * Any row-split SpMM shape gives bit-identical results; only the speed
  * differs.  The
  * adjoint shape leaves dX bit-identical; dP's per-nonzero sums over features
@@ -134,6 +148,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_MERGE_PATH(m) (((m) & 0xF) << 12)
 #define SCL_GPU_SHAPE_ROW_GROUPS(r) ((((r) / 4) & 0x3) << 16)
 #define SCL_GPU_SHAPE_STAGED_GROUPS (3 << 16)
+#define SCL_GPU_SHAPE_TILED (1 << 18)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

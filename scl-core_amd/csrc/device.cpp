@@ -263,6 +263,17 @@ void DeviceCSR::release_grouped() {
     dev_free(staged.upos);
     dev_free(staged.grow);
     staged = StagedCsr{};
+    for (TiledCsr& t : tiled) {
+        dev_free(t.info);
+        dev_free(t.ucol);
+        dev_free(t.trow);
+        dev_free(t.trbeg);
+        dev_free(t.trlen);
+        dev_free(t.tslot);
+        dev_free(t.tval);
+        dev_free(t.tperm);
+        t = TiledCsr{};
+    }
     // shapes that read the grouped form go with it
     for (auto* m : {&shapes, &t_shapes})
         for (auto it = m->begin(); it != m->end();)
@@ -353,7 +364,9 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->adj_shapes.clear();
         }
     }
-    if (slot->grouped[0].R || slot->grouped[1].R || slot->staged.built) {
+    bool any_tiled = false;
+    for (const TiledCsr& t : slot->tiled) any_tiled = any_tiled || t.lpr;
+    if (slot->grouped[0].R || slot->grouped[1].R || slot->staged.built || any_tiled) {
         // same pattern: re-gather the grouped values (the staged form holds
         // no values); else drop the forms
         const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
@@ -361,9 +374,13 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
         for (const GroupedCsr& g : slot->grouped)
             if (g.R && g.fingerprint != fp) keep = false;
         if (slot->staged.built && slot->staged.fingerprint != fp) keep = false;
+        for (const TiledCsr& t : slot->tiled)
+            if (t.lpr && t.fingerprint != fp) keep = false;
         if (keep) {
             for (GroupedCsr& g : slot->grouped)
                 if (g.R) launch_gather_f32(g.gperm, slot->val, g.gval, nnz, nullptr);
+            for (TiledCsr& t : slot->tiled)
+                if (t.lpr) launch_gather_pad_f32(t.tperm, slot->val, t.tval, t.n_slots, nullptr);
             hip_check(hipStreamSynchronize(nullptr), "refresh grouped values");
         } else {
             if (slot->pins > 0)
@@ -484,7 +501,7 @@ bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d) {
                   "upload");
     hip_check(hipMemcpy(g.upos, upos.data(), upos.size() * 4, hipMemcpyHostToDevice), "upload");
     g.n_groups = n_groups;
-    g.n_union = (int64_t)ucol.size();
+    g.n_union = (int64_t)ucol.size() - 1;
     g.fingerprint = fp;
     g.built = true;
     return true;
@@ -493,6 +510,158 @@ bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d) {
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s) {
     for (GroupedCsr& g : d.grouped)
         if (g.R) launch_gather_f32(g.gperm, d.val, g.gval, d.nnz, s);
+    for (TiledCsr& t : d.tiled)
+        if (t.lpr) launch_gather_pad_f32(t.tperm, d.val, t.tval, t.n_slots, s);
+}
+
+bool ensure_tiled(scl_sparse_matrix* h, DeviceCSR& d, int lpr, int budget) {
+    if (!(lpr == 4 || lpr == 8) || !(budget == 1 || budget == 2)) return false;
+    TiledCsr& g = d.tiled[tiled_index(lpr, budget)];
+    const int64_t p = h->primary(), q = h->secondary();
+    const uint64_t fp = pattern_fingerprint(p, h->indptr, h->indices);
+    if (g.lpr == lpr && g.budget == budget && g.fingerprint == fp) return true;
+    const int64_t smax = tiled_union_max(lpr, budget);
+    // padding entries read the last LDS slot, which holds zeros
+    const uint16_t zero_unit = (uint16_t)((tiled_slot_capacity(lpr, budget) - 1) * lpr);
+    // a tile's padded entries, rounded up to a multiple of 8 at its end
+    const int64_t emax = tiled_entries_max(budget) - 4;
+    const int64_t rmax = kTiledThreads / lpr;
+    for (int64_t i = 0; i < p; ++i) {
+        const int64_t len = h->indptr[i + 1] - h->indptr[i];
+        if (len > smax || ((len + 3) & ~int64_t(3)) > emax) return false;
+        for (int64_t k = h->indptr[i] + 1; k < h->indptr[i + 1]; ++k)
+            if (h->indices[k] <= h->indices[k - 1]) return false;
+    }
+    if (d.pins > 0)
+        fail(SCL_ERROR_INVALID_ARGUMENT,
+             "a diffusion plan holds this operator: its tiled form cannot be rebuilt");
+    // tiles follow the device's visiting order (the locality schedule)
+    std::vector<int32_t> order((size_t)p);
+    if (d.sched) {
+        hip_check(hipMemcpy(order.data(), d.sched, (size_t)p * 4, hipMemcpyDeviceToHost),
+                  "download schedule");
+    } else {
+        for (int64_t i = 0; i < p; ++i) order[i] = (int32_t)i;
+    }
+    // greedy cut: a row joins the current tile while rows, union slots and
+    // padded entries all fit
+    std::vector<int32_t> stamp((size_t)std::max<int64_t>(q, 1), -1);
+    std::vector<int64_t> cut{0};
+    {
+        int64_t nu = 0, ne = 0, nr = 0;
+        int32_t t = 0;
+        for (int64_t i = 0; i < p; ++i) {
+            const int32_t row = order[i];
+            const int64_t b = h->indptr[row], e = h->indptr[row + 1];
+            const int64_t pad = (e - b + 3) & ~int64_t(3);
+            int64_t fresh = 0;
+            for (int64_t k = b; k < e; ++k) fresh += stamp[h->indices[k]] != t;
+            if (nr > 0 && (nr + 1 > rmax || nu + fresh > smax || ne + pad > emax)) {
+                cut.push_back(i);
+                ++t;
+                nu = ne = nr = 0;
+                fresh = e - b;
+            }
+            for (int64_t k = b; k < e; ++k) stamp[h->indices[k]] = t;
+            nu += fresh;
+            ne += pad;
+            ++nr;
+        }
+        if (p > 0) cut.push_back(p);
+    }
+    const int64_t n_tiles = (int64_t)cut.size() - 1;
+    std::vector<int32_t> info((size_t)(n_tiles + 1) * 4, 0), ucol, trow((size_t)p),
+        trbeg((size_t)p), trlen((size_t)p), tperm;
+    std::vector<uint16_t> tslot;
+    std::vector<int32_t> rows, cols;
+    std::vector<int32_t> place((size_t)std::max<int64_t>(q, 1), -1);
+    ucol.reserve((size_t)h->nnz / 2 + 16);
+    tslot.reserve((size_t)h->nnz + (size_t)p * 3);
+    tperm.reserve((size_t)h->nnz + (size_t)p * 3);
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        info[t * 4 + 0] = (int32_t)ucol.size();
+        info[t * 4 + 2] = (int32_t)tslot.size();
+        info[t * 4 + 3] = (int32_t)cut[t];
+        rows.assign(order.begin() + cut[t], order.begin() + cut[t + 1]);
+        // longest rows first, so the rows of one wave have similar lengths
+        std::stable_sort(rows.begin(), rows.end(), [&](int32_t a, int32_t b) {
+            return h->indptr[a + 1] - h->indptr[a] > h->indptr[b + 1] - h->indptr[b];
+        });
+        cols.clear();
+        for (int32_t r : rows)
+            for (int64_t k = h->indptr[r]; k < h->indptr[r + 1]; ++k)
+                cols.push_back((int32_t)h->indices[k]);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        for (size_t u = 0; u < cols.size(); ++u) place[cols[u]] = (int32_t)u;
+        info[t * 4 + 1] = (int32_t)cols.size();
+        const size_t e0 = tslot.size();
+        for (size_t j = 0; j < rows.size(); ++j) {
+            const int32_t r = rows[j];
+            const int64_t at = cut[t] + (int64_t)j;
+            trow[at] = r;
+            trbeg[at] = (int32_t)(tslot.size() - e0);
+            trlen[at] = (int32_t)(h->indptr[r + 1] - h->indptr[r]);
+            for (int64_t k = h->indptr[r]; k < h->indptr[r + 1]; ++k) {
+                tslot.push_back((uint16_t)(place[h->indices[k]] * lpr));
+                tperm.push_back((int32_t)k);
+            }
+            while ((tslot.size() - e0) % 4) {
+                tslot.push_back(zero_unit);
+                tperm.push_back(-1);
+            }
+        }
+        while ((tslot.size() - e0) % 8) {
+            tslot.push_back(zero_unit);
+            tperm.push_back(-1);
+        }
+        for (int32_t c : cols) place[c] = -1;
+        ucol.insert(ucol.end(), cols.begin(), cols.end());
+        if (ucol.size() > (size_t)INT32_MAX || tslot.size() > (size_t)INT32_MAX) return false;
+    }
+    info[n_tiles * 4 + 0] = (int32_t)ucol.size();
+    info[n_tiles * 4 + 2] = (int32_t)tslot.size();
+    info[n_tiles * 4 + 3] = (int32_t)p;
+    for (TiledCsr* t : {&g}) {
+        dev_free(t->info);
+        dev_free(t->ucol);
+        dev_free(t->trow);
+        dev_free(t->trbeg);
+        dev_free(t->trlen);
+        dev_free(t->tslot);
+        dev_free(t->tval);
+        dev_free(t->tperm);
+        *t = TiledCsr{};
+    }
+    const size_t ns = std::max<size_t>(tslot.size(), 8);
+    g.info = dev_alloc<int32_t>(info.size(), "tiled info");
+    ucol.push_back(0);  // spare entry: the kernel's clamped reads stay inside
+    g.ucol = dev_alloc<int32_t>(ucol.size(), "tiled columns");
+    g.trow = dev_alloc<int32_t>(std::max<size_t>(trow.size(), 1), "tiled rows");
+    g.trbeg = dev_alloc<int32_t>(std::max<size_t>(trbeg.size(), 1), "tiled row starts");
+    g.trlen = dev_alloc<int32_t>(std::max<size_t>(trlen.size(), 1), "tiled row lengths");
+    g.tslot = dev_alloc<uint16_t>(ns, "tiled slots");
+    g.tval = dev_alloc<float>(ns, "tiled values");
+    g.tperm = dev_alloc<int32_t>(ns, "tiled permutation");
+    auto up = [](void* dst, const void* src, size_t bytes) {
+        if (bytes) hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload tiles");
+    };
+    up(g.info, info.data(), info.size() * 4);
+    up(g.ucol, ucol.data(), ucol.size() * 4);
+    up(g.trow, trow.data(), trow.size() * 4);
+    up(g.trbeg, trbeg.data(), trbeg.size() * 4);
+    up(g.trlen, trlen.data(), trlen.size() * 4);
+    up(g.tslot, tslot.data(), tslot.size() * 2);
+    up(g.tperm, tperm.data(), tperm.size() * 4);
+    g.n_tiles = n_tiles;
+    g.n_union = (int64_t)ucol.size() - 1;
+    g.n_slots = (int64_t)tslot.size();
+    launch_gather_pad_f32(g.tperm, d.val, g.tval, g.n_slots, nullptr);
+    hip_check(hipStreamSynchronize(nullptr), "tiled values");
+    g.lpr = lpr;
+    g.budget = budget;
+    g.fingerprint = fp;
+    return true;
 }
 
 bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {

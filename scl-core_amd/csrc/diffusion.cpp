@@ -21,7 +21,7 @@ constexpr int64_t kInt32Max = std::numeric_limits<int32_t>::max();
 
 CsrView view_of(const DeviceCSR& d) {
     return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge,
-                   d.grouped, &d.staged};
+                   d.grouped, &d.staged, d.tiled};
 }
 
 CsrView transposed_view_of(const DeviceCSR& d) {
@@ -355,10 +355,13 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
     sh.merge = (flags >> kShapeMergeShift) & kShapeMergeMask;
     const int gcode = (flags >> kShapeGroupShift) & kShapeGroupMask;
+    const bool tiled = (flags & kShapeTiled) != 0;
     SCL_API_CHECK(gcode == 0 || transpose == 0, SCL_ERROR_INVALID_ARGUMENT,
                   "grouped form: code 1 (4 rows), 2 (8 rows) or 3 (LDS-staged groups), "
                   "forward direction only");
-    sh.rows_per_group = gcode == 3 ? kStagedRows : gcode ? 4 * gcode : 0;
+    SCL_API_CHECK(!tiled || (transpose == 0 && gcode == 0), SCL_ERROR_INVALID_ARGUMENT,
+                  "tiled form: forward direction only, not with a grouped form");
+    sh.rows_per_group = tiled ? kTiledRows : gcode == 3 ? kStagedRows : gcode ? 4 * gcode : 0;
     SCL_API_CHECK(sh.merge <= kMergeMax && (sh.merge == 0 || transpose != 2),
                   SCL_ERROR_INVALID_ARGUMENT,
                   "merge-path code must be 0..8, and 0 for the fused adjoint");
@@ -384,7 +387,12 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
         if (sh.merge > 0)
             ensure_merge_scratch(d, merge_waves(transpose ? d.n_sec : d.n, d.nnz, sh.merge),
                                  n_features);
-        if (sh.rows_per_group == kStagedRows) {
+        if (sh.rows_per_group == kTiledRows) {
+            if (!ensure_tiled(transition, d, sh.group, sh.wg_per_cu))
+                fail(SCL_ERROR_INVALID_ARGUMENT,
+                     "tiled form needs strictly increasing column indices in every row "
+                     "and rows that fit one tile");
+        } else if (sh.rows_per_group == kStagedRows) {
             if (!ensure_staged(transition, d))
                 fail(SCL_ERROR_INVALID_ARGUMENT,
                      "staged form needs strictly increasing column indices in every row");
@@ -440,8 +448,9 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
                  (sh.scalar_rows ? kShapeScalarRows : 0) |
                  (sh.wg_per_cu << kShapeOccupancyShift) |
                  (sh.merge << kShapeMergeShift) |
-                 ((sh.rows_per_group == kStagedRows ? 3 : sh.rows_per_group / 4)
-                  << kShapeGroupShift);
+                 (sh.rows_per_group == kTiledRows ? kShapeTiled
+                  : ((sh.rows_per_group == kStagedRows ? 3 : sh.rows_per_group / 4)
+                     << kShapeGroupShift));
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }
@@ -493,6 +502,25 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
                     c.rows_per_group = kStagedRows;
                     c.nontemporal = nt != 0;
                     if (spmm_shape_valid(c, n_features, X_dev, Y_dev)) cands.push_back(c);
+                }
+            // LDS tiles: 4 or 8 lanes per row, one workgroup per CU (two
+            // measured slower at every width tried)
+            for (int lpr : {4, 8})
+                for (int budget : {1}) {
+                    if (!ensure_tiled(transition, d, lpr, budget)) continue;
+                    for (int chunk : {16})
+                        for (int nt = 0; nt < 2; ++nt) {
+                            SpmmShape c;
+                            c.vec = 4;
+                            c.group = lpr;
+                            c.unroll = chunk;
+                            c.wg_per_cu = budget;
+                            c.rows_per_group = kTiledRows;
+                            c.nontemporal = nt != 0;
+                            if (spmm_shape_valid(c, n_features, X_dev, Y_dev) &&
+                                (uint64_t)d.n_sec * (uint64_t)n_features <= 0xFFFFFFFFull)
+                                cands.push_back(c);
+                        }
                 }
             for (int R : {4, 8}) {
                 if (!ensure_grouped(transition, d, R)) break;

@@ -37,7 +37,7 @@ int64_t n_checkpoints(int64_t t, int64_t every) { return (t + every - 1) / every
 
 CsrView fwd_view(const DeviceCSR& d) {
     return CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched, &d.merge,
-                   d.grouped, &d.staged};
+                   d.grouped, &d.staged, d.tiled};
 }
 CsrView adj_view(const DeviceCSR& d) {
     return CsrView{d.n_sec, d.n, d.nnz, d.t_row_ptr, d.t_col, d.t_val, d.t_sched,

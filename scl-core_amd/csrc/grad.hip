@@ -377,6 +377,16 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ i
     if (p < m) dst[p] = src[idx[p]];
 }
 
+__global__ __launch_bounds__(kBlock) void k_gather_pad(const int32_t* __restrict__ idx,
+                                                       const float* __restrict__ src,
+                                                       float* __restrict__ dst, int64_t m) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p < m) {
+        const int32_t i = idx[p];
+        dst[p] = i >= 0 ? src[i] : -0.0f;
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -510,6 +520,14 @@ void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t
                        hipStream_t s) {
     if (m <= 0) return;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((m + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, idx, src, dst, m);
+    check_launch("gather launch");
+}
+
+void launch_gather_pad_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                           hipStream_t s) {
+    if (m <= 0) return;
+    hipLaunchKernelGGL(k_gather_pad, dim3((unsigned)((m + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, idx, src, dst, m);
     check_launch("gather launch");
 }

@@ -107,6 +107,9 @@ struct DeviceCSR {
     GroupedCsr grouped[2];
     // LDS-staged groups of kStagedRows rows (pattern only: no values)
     StagedCsr staged;
+    // LDS-tiled forms, [tiled_index(lpr, budget)] (values refreshed with the
+    // operator's)
+    TiledCsr tiled[4];
 
     ~DeviceCSR();
     void release_grouped();
@@ -167,6 +170,10 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R);
 // false when the pattern cannot be staged (a row with unsorted or repeated
 // column indices).
 bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d);
+// Builds d.tiled[tiled_index(lpr, budget)] for the handle's current pattern
+// when missing or stale; false when the pattern cannot be tiled (a row with
+// unsorted or repeated column indices, or too long for one tile).
+bool ensure_tiled(scl_sparse_matrix* h, DeviceCSR& d, int lpr, int budget);
 // Re-gathers the grouped forms' values from d.val (after the device values
 // changed in place), stream-ordered on s.
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s);

@@ -573,8 +573,15 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     if (s.merge < 0 || s.merge > kMergeMax || (s.merge > 0 && s.group != 64)) return false;
     // grouped form: lanes over features, 4 or 8 rows per wave, not with merge
     if (s.rows_per_group != 0 && s.rows_per_group != kStagedRows &&
+        s.rows_per_group != kTiledRows &&
         (!(s.rows_per_group == 4 || s.rows_per_group == 8) || s.group != 64 || s.merge))
         return false;
+    // tiled form: 4 or 8 lanes per row, an LDS budget of 1 or 2 workgroups
+    // per CU, unroll = slabs per workgroup
+    if (s.rows_per_group == kTiledRows)
+        return s.vec == 4 && (s.group == 4 || s.group == 8) &&
+               (s.wg_per_cu == 1 || s.wg_per_cu == 2) && !s.merge && !s.scalar_rows &&
+               (s.unroll == 4 || s.unroll == 8 || s.unroll == 16);
     // staged form: 64 lanes x 4 features, its own LDS (no occupancy cap)
     if (s.rows_per_group == kStagedRows &&
         (s.vec != 4 || s.group != 64 || s.merge || s.wg_per_cu || s.scalar_rows))
@@ -862,6 +869,191 @@ void launch_spmm_staged(const CsrView& P, const float* X, float* Y, int64_t F,
     check_launch("staged spmm launch");
 }
 
+// ------------------------------------------------------------- LDS tiles --
+// One workgroup of kTiledThreads lanes computes one tile (a run of rows of
+// the visiting order whose column union fits LDS) for a chunk of slabs of
+// LPR * 4 features.  Per slab, the waves copy the union's slab of X rows into
+// LDS (one 16-byte piece per lane, 64 / LPR rows per wave load), then every
+// lane group of LPR lanes walks one row's (union slot, value) entries from
+// LDS in stored order and reads the neighbour's 4 features there:
+// acc = fma(p, x, acc), the row-split chain bit for bit.  X leaves the
+// vector memory path once per tile and slab instead of once per stored
+// nonzero.  The next slab's pieces are loaded into registers while the
+// current slab is computed, and written to LDS after it.
+//
+// Rows are padded to a multiple of four entries with (zero slot, -0.0):
+// fma(-0.0, +0.0, acc) == acc for every acc (-0.0 + -0.0 = -0.0 and
+// +0.0 + -0.0 = +0.0), so the padding needs no guard.  The zero slot is the
+// last LDS slot; unions are capped one wave load short of it.
+//
+// Workgroups are numbered so that XCD x walks its contiguous range of tiles
+// in blocks of as many tiles as it runs at once, and each block chunk by
+// chunk: the tiles in flight on one XCD are neighbours in the visiting order
+// working on the same slabs, so their shared X rows meet in that XCD's L2.
+template <int LPR, int B>
+struct TiledLds {
+    static constexpr int kSlotBytes = LPR * 16;
+    static constexpr int kSlots = tiled_slot_capacity(LPR, B);
+    static constexpr int kEntries = tiled_entries_max(B);
+    static constexpr int kSlotsPerLoad = 64 / LPR;  // one 1 KiB wave load
+    static constexpr int kLoadsPerWave =
+        (tiled_union_max(LPR, B) / kSlotsPerLoad + kTiledThreads / 64 - 1) / (kTiledThreads / 64);
+    static constexpr int kBytes = kSlots * kSlotBytes + kEntries * 6;
+    static_assert(kBytes <= 163840 / B, "tile exceeds the LDS budget");
+    static_assert(kSlots % kSlotsPerLoad == 0 && kEntries % 8 == 0, "tile layout");
+    static_assert(tiled_union_max(LPR, B) + kSlotsPerLoad == kSlots, "zero slot");
+    static_assert(kSlots * kSlotBytes / 16 <= 65536, "slot units fit 16 bits");
+};
+
+template <int LPR, int B, bool NT>
+__global__ __launch_bounds__(kTiledThreads) void k_spmm_tiled(
+    const int4* __restrict__ info, const int32_t* __restrict__ ucol,
+    const int32_t* __restrict__ trow, const int32_t* __restrict__ trbeg,
+    const int32_t* __restrict__ trlen, const uint16_t* __restrict__ tslot,
+    const float* __restrict__ tval, const float* __restrict__ X, float* __restrict__ Y,
+    int32_t F, int32_t n_tiles, int32_t tiles_per_xcd, int32_t block_tiles,
+    int32_t chunk_slabs, int32_t n_chunks, int32_t n_slabs) {
+    using L = TiledLds<LPR, B>;
+    constexpr int kWaves = kTiledThreads / 64;
+    __shared__ __attribute__((aligned(16))) char lds[L::kBytes];
+    char* xs = lds;
+    uint16_t* ms = reinterpret_cast<uint16_t*>(lds + L::kSlots * L::kSlotBytes);
+    float* mv = reinterpret_cast<float*>(lds + L::kSlots * L::kSlotBytes + L::kEntries * 2);
+    // XCD x receives hardware ids x, x + 8, ...: its k-th workgroup takes
+    // block k / (block_tiles n_chunks) of its tile range, chunk by chunk
+    const int xcd = blockIdx.x % kNumXcd, k = blockIdx.x / kNumXcd;
+    const int per_block = block_tiles * n_chunks;
+    const int blk = k / per_block, rem = k - blk * per_block;
+    const int chunk = rem / block_tiles;
+    const int local = blk * block_tiles + (rem - chunk * block_tiles);
+    const int tile = xcd * tiles_per_xcd + local;
+    if (local >= tiles_per_xcd || tile >= n_tiles) return;  // whole workgroup
+    const int tid = threadIdx.x;
+    const int4 ti = info[tile];
+    const int4 tn = info[tile + 1];
+    const int u_off = ti.x, n_u = ti.y, e_off = ti.z, r_off = ti.w;
+    const int ne = tn.z - e_off, nr = tn.w - r_off;
+    // the tile's entries (ne is a multiple of 8) and the zero slot
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(tslot + e_off);
+        uint4* dst = reinterpret_cast<uint4*>(ms);
+        for (int p = tid; p < ne / 8; p += kTiledThreads) dst[p] = src[p];
+        const f4* vs = reinterpret_cast<const f4*>(tval + e_off);
+        f4* vd = reinterpret_cast<f4*>(mv);
+        for (int p = tid; p < ne / 4; p += kTiledThreads) vd[p] = vs[p];
+        if (tid < LPR)
+            *reinterpret_cast<f4*>(xs + (L::kSlots - 1) * L::kSlotBytes + tid * 16) =
+                f4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    const int g = tid / LPR, sub = tid % LPR;
+    int row = 0, rb = 0, rl = 0;
+    if (g < nr) {
+        row = trow[r_off + g];
+        rb = trbeg[r_off + g];
+        rl = (trlen[r_off + g] + 3) & ~3;
+    }
+    // X row offsets of the slots this wave stages (32-bit: the launcher
+    // checks columns * F < 2^32); slots past the union stage row 0
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int n_loads = (n_u + L::kSlotsPerLoad - 1) / L::kSlotsPerLoad;
+    uint32_t roff[L::kLoadsPerWave];
+#pragma unroll
+    for (int i = 0; i < L::kLoadsPerWave; ++i) {
+        const int ld = wave + kWaves * i;
+        const int slot = ld * L::kSlotsPerLoad + lane / LPR;
+        // clamped read (ucol holds one spare entry past the last union), so
+        // the loads carry no branch and issue together
+        const uint32_t c = (uint32_t)ucol[u_off + min(slot, max(n_u - 1, 0))];
+        roff[i] = (ld < n_loads && slot < n_u) ? c * (uint32_t)F : 0u;
+    }
+    f4 stg[L::kLoadsPerWave];
+    // every wave issues all its loads (past the union they read row 0 and
+    // are not written to LDS): no branch between the loads, so they are all
+    // in flight together
+    auto fetch = [&](int slab) {
+        // lanes past the last feature fetch feature 0 (never read)
+        const int fs = slab * (LPR * 4) + (lane % LPR) * 4;
+        const float* Xs = X + (fs < F ? fs : 0);
+#pragma unroll
+        for (int i = 0; i < L::kLoadsPerWave; ++i)
+            stg[i] = *reinterpret_cast<const f4*>(Xs + roff[i]);
+    };
+    const int s_beg = chunk * chunk_slabs;
+    const int s_end = min(s_beg + chunk_slabs, n_slabs);
+    const char* xl = xs + sub * 16;
+    fetch(s_beg);
+    for (int slab = s_beg; slab < s_end; ++slab) {
+#pragma unroll
+        for (int i = 0; i < L::kLoadsPerWave; ++i) {
+            const int ld = wave + kWaves * i;
+            if (ld < n_loads) *reinterpret_cast<f4*>(xs + ld * 1024 + lane * 16) = stg[i];
+        }
+        if (slab + 1 < s_end) fetch(slab + 1);
+        __syncthreads();
+        const int f = slab * (LPR * 4) + sub * 4;
+        if (g < nr && f < F) {
+            float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int e = 0; e < rl; e += 4) {
+                // slots in 16-byte units: lane offset + 16 * unit
+                const uint2 sl = *reinterpret_cast<const uint2*>(ms + rb + e);
+                const f4 v = *reinterpret_cast<const f4*>(mv + rb + e);
+                const f4 x0 = *reinterpret_cast<const f4*>(xl + ((sl.x & 0xFFFFu) << 4));
+                const f4 x1 = *reinterpret_cast<const f4*>(xl + ((sl.x >> 16) << 4));
+                const f4 x2 = *reinterpret_cast<const f4*>(xl + ((sl.y & 0xFFFFu) << 4));
+                const f4 x3 = *reinterpret_cast<const f4*>(xl + ((sl.y >> 16) << 4));
+                fma_vec(v.x, x0, acc);
+                fma_vec(v.y, x1, acc);
+                fma_vec(v.z, x2, acc);
+                fma_vec(v.w, x3, acc);
+            }
+            store_vec<4, NT>(Y + (size_t)row * F + f, acc);
+        }
+        __syncthreads();
+    }
+}
+
+template <int LPR, int B, bool NT>
+void launch_tiled_t(const TiledCsr& T, const float* X, float* Y, int64_t F, int chunk,
+                    hipStream_t s) {
+    const int64_t n_slabs = (F + LPR * 4 - 1) / (LPR * 4);
+    const int64_t n_chunks = (n_slabs + chunk - 1) / chunk;
+    const int64_t tpx = (T.n_tiles + kNumXcd - 1) / kNumXcd;
+    // tiles one XCD runs at once: its 32 CUs, B workgroups each
+    const int64_t block = std::min<int64_t>(32 * B, tpx);
+    const int64_t blocks = (tpx + block - 1) / block;
+    const int64_t nb = kNumXcd * blocks * block * n_chunks;
+    if (nb > INT32_MAX) throw std::runtime_error("tiled spmm grid too large");
+    hipLaunchKernelGGL((k_spmm_tiled<LPR, B, NT>), dim3((unsigned)nb), dim3(kTiledThreads), 0, s,
+                       reinterpret_cast<const int4*>(T.info), T.ucol, T.trow, T.trbeg, T.trlen,
+                       T.tslot, T.tval, X, Y, (int32_t)F, (int32_t)T.n_tiles, (int32_t)tpx,
+                       (int32_t)block, (int32_t)chunk, (int32_t)n_chunks, (int32_t)n_slabs);
+}
+
+void launch_spmm_tiled(const CsrView& P, const float* X, float* Y, int64_t F,
+                       const SpmmShape& shape, hipStream_t s) {
+    const int lpr = shape.group, budget = shape.wg_per_cu;
+    const TiledCsr* T = P.tiled ? &P.tiled[tiled_index(lpr, budget)] : nullptr;
+    if (!T || T->lpr != lpr || T->budget != budget)
+        throw std::runtime_error("tiled form of the operator missing for this launch");
+    if ((uint64_t)P.n_sec * (uint64_t)F > 0xFFFFFFFFull)
+        throw std::runtime_error("tiled form: columns x features must stay below 2^32");
+    if (T->n_tiles == 0) return;
+    const int chunk = std::max(shape.unroll, 1);
+#define SCL_TILED(LP, BB)                                                               \
+    (shape.nontemporal ? launch_tiled_t<LP, BB, true>(*T, X, Y, F, chunk, s)            \
+                       : launch_tiled_t<LP, BB, false>(*T, X, Y, F, chunk, s))
+    if (lpr == 8) {
+        if (budget == 2) SCL_TILED(8, 2);
+        else SCL_TILED(8, 1);
+    } else {
+        if (budget == 2) SCL_TILED(4, 2);
+        else SCL_TILED(4, 1);
+    }
+#undef SCL_TILED
+    check_launch("tiled spmm launch");
+}
+
 void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
                          const SpmmShape& shape, hipStream_t s) {
     const int R = shape.rows_per_group;
@@ -904,6 +1096,10 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     }
     if (shape.rows_per_group == kStagedRows) {
         launch_spmm_staged(P, X, Y, F, shape, s);
+        return;
+    }
+    if (shape.rows_per_group == kTiledRows) {
+        launch_spmm_tiled(P, X, Y, F, shape, s);
         return;
     }
     if (shape.rows_per_group > 0) {

@@ -56,6 +56,48 @@ struct StagedCsr {
     uint64_t fingerprint = 0;  // pattern it was built for
 };
 
+// LDS-tiled form of the forward operator (the "tiled" SpMM shapes).  Rows
+// are cut, in visiting order, into tiles whose column union fits the LDS of
+// a workgroup; a workgroup stages the union's slab of X rows once per slab
+// (lpr * 4 features: 16 or 32) with direct global-to-LDS loads, and each of
+// its lane groups (lpr lanes, one row of the tile) reads its neighbours
+// there in stored order.  Within a tile the rows run longest first, so the
+// rows of a wave have similar lengths.  A tile's entries (one union slot
+// and one value per stored nonzero, each row padded to a multiple of four)
+// are copied to LDS once per workgroup and serve every slab it computes.
+constexpr int kTiledRows = 128;   // SpmmShape::rows_per_group code of the form
+constexpr int kTiledThreads = 512;
+// LDS slots, union slots and padded entries of one tile: lpr 4 or 8 lanes
+// per row, budget 1 or 2 workgroups per CU (160 or 80 KiB of LDS each).  The
+// union stops one wave load (64 / lpr slots) short of the capacity so that
+// the last slot stays zero (the padding entries read it).
+constexpr int tiled_slot_capacity(int lpr, int budget) {
+    return (lpr == 8 ? 960 : 1920) / budget;
+}
+constexpr int tiled_union_max(int lpr, int budget) {
+    return tiled_slot_capacity(lpr, budget) - 64 / lpr;
+}
+constexpr int tiled_entries_max(int budget) { return 6400 / budget; }
+struct TiledCsr {
+    int lpr = 0;                  // 0: not built
+    int budget = 0;
+    int64_t n_tiles = 0;
+    int64_t n_union = 0;
+    int64_t n_slots = 0;          // padded entries
+    int32_t* info = nullptr;      // (n_tiles + 1) x 4: union offset, union size,
+                                  // entry offset, first row (sentinel last)
+    int32_t* ucol = nullptr;      // n_union: each tile's columns, ascending
+    int32_t* trow = nullptr;      // n: rows in tile order
+    int32_t* trbeg = nullptr;     // n: first entry of the row (tile-local)
+    int32_t* trlen = nullptr;     // n: stored entries of the row
+    uint16_t* tslot = nullptr;    // n_slots: union slot of every entry, in
+                                  // 16-byte LDS units (lpr units per slot)
+    float* tval = nullptr;        // n_slots: its value (-0.0 in padding)
+    int32_t* tperm = nullptr;     // n_slots: position in (col, val), -1 padding
+    uint64_t fingerprint = 0;     // pattern it was built for
+};
+inline int tiled_index(int lpr, int budget) { return (lpr == 8 ? 2 : 0) + (budget == 2 ? 1 : 0); }
+
 struct CsrView {
     int32_t n;            // rows of the applied operator
     int32_t n_sec;        // columns
@@ -67,6 +109,7 @@ struct CsrView {
     const MergeScratch* merge = nullptr;  // for merge-path shapes
     const GroupedCsr* grouped = nullptr;  // [0]: R = 4, [1]: R = 8
     const StagedCsr* staged = nullptr;    // LDS-staged groups of kStagedRows rows
+    const TiledCsr* tiled = nullptr;      // [tiled_index(lpr, budget)]
 };
 
 // Launch shape of the row-split SpMM (see kernels.hip).
@@ -83,7 +126,10 @@ struct SpmmShape {
     int merge = 0;      // 0: row-split; m > 0: merge path, 256 << (m - 1)
                         // path items (row ends + nonzeros) per wave
     int rows_per_group = 0;  // 0: one row per lane group; 4 or 8: grouped form;
-                             // kStagedRows: LDS-staged form
+                             // kStagedRows: LDS-staged form; kTiledRows:
+                             // LDS-tiled form (group = lanes per row, 4 or 8;
+                             // wg_per_cu = LDS budget, 1 or 2 workgroups per
+                             // CU; unroll = slabs per workgroup)
 };
 
 // Flags of the C ABI's shape calls (scl_gpu_spmm_set_shape).
@@ -104,6 +150,8 @@ constexpr int kMergeMax = 8;
 // 3 = LDS-staged groups of kStagedRows rows)
 constexpr int kShapeGroupShift = 16;
 constexpr int kShapeGroupMask = 0x3;
+// bit 18: LDS-tiled form (see TiledCsr)
+constexpr int kShapeTiled = 1 << 18;
 // Waves a merge-path launch with code m uses for n rows and nnz nonzeros.
 int64_t merge_waves(int64_t n, int64_t nnz, int m);
 
@@ -156,6 +204,9 @@ void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_
                         hipStream_t s);
 void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
                        hipStream_t s);
+// dst[p] = idx[p] >= 0 ? src[idx[p]] : -0.0 for p < m.
+void launch_gather_pad_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
+                           hipStream_t s);
 
 // Input / output storage of one reduced-precision SpMM step.
 enum class LowpIo { F32_TO_BF16, BF16_TO_BF16, BF16_TO_F32, F32_TO_F32 };

@@ -56,7 +56,7 @@ struct Session {
         if (P->wrapped) P->generation++;
         d = &device_csr(P, dev);
         v = CsrView{d->n, d->n_sec, d->nnz, d->row_ptr, d->col, d->val, d->sched, &d->merge,
-                    d->grouped, &d->staged};
+                    d->grouped, &d->staged, d->tiled};
         s = library_stream(dev);
     }
 
@@ -78,9 +78,10 @@ struct Session {
         CsrView w = v;
         w.val = val;
         w.grouped = nullptr;
+        w.tiled = nullptr;
         SpmmShape sh = shape(F, X, Y);
-        // the staged form reads the view's values; the grouped forms hold
-        // copies of the operator's
+        // the staged form reads the view's values; the grouped and tiled
+        // forms hold copies of the operator's
         if (sh.rows_per_group && sh.rows_per_group != kStagedRows)
             sh = spmm_default_shape(F, X, Y);
         launch_spmm_f32(w, X, Y, F, sh, s);

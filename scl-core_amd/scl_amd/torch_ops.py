@@ -15,6 +15,7 @@ from .sparse import SparseMatrixBase
 
 # rows per group of the shape flags' group code (bits 16..17)
 _ROWS_PER_GROUP = (0, 4, 8, 64)
+TILED = 128  # rows_per_group code of the LDS-tiled form
 
 def _stream_ptr(t: torch.Tensor):
     return torch.cuda.current_stream(t.device).cuda_stream
@@ -132,7 +133,8 @@ def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False,
             "nontemporal": bool(f.value & 1), "schedule": not (f.value & 2),
             "xcd_rows": bool(f.value & 4), "scalar_rows": bool(f.value & 8),
             "wg_per_cu": (f.value >> 8) & 0xF, "merge": (f.value >> 12) & 0xF,
-            "rows_per_group": _ROWS_PER_GROUP[(f.value >> 16) & 0x3]}
+            "rows_per_group": (TILED if f.value & (1 << 18)
+                               else _ROWS_PER_GROUP[(f.value >> 16) & 0x3])}
 
 
 def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
@@ -142,10 +144,14 @@ def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
               wg_per_cu: int = 0, merge: int = 0, rows_per_group: int = 0) -> None:
     """merge=m > 0 selects the merge-path SpMM with 256 << (m - 1) path items
     per wave (group 64 only); rows_per_group=4 or 8 the row-grouped form, 64
-    the LDS-staged groups (vec 4, group 64)."""
+    the LDS-staged groups (vec 4, group 64), TILED (128) the LDS tiles
+    (vec 4; group = lanes per row, 4 or 8; wg_per_cu = LDS budget, 1 or 2;
+    unroll = slabs per workgroup)."""
+    code = ((1 << 18) if rows_per_group == TILED
+            else _ROWS_PER_GROUP.index(rows_per_group) << 16)
     flags = ((1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
              | (8 if scalar_rows else 0) | ((wg_per_cu & 0xF) << 8) | ((merge & 0xF) << 12)
-             | (_ROWS_PER_GROUP.index(rows_per_group) << 16))
+             | code)
     _lib.check(_lib.load().scl_gpu_spmm_set_shape(
         P.handle, n_features, _direction(transpose, adjoint), vec, group, unroll, flags))
 
