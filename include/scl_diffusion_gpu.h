@@ -59,8 +59,9 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,
  * FP64 degrees d_i = sum_j a_ij (degrees_out: host array of rows doubles, or
  * NULL) and the FP64 stationarity residual r = || P^T pi - pi ||_1 with
  * pi = d / sum(d) evaluated with the normalised fp32 values (residual_out:
- * host double, or NULL).  r vanishes, up to rounding, for symmetric A with
- * symmetric == 0. */
+ * host double, or NULL; a rectangular matrix with residual_out set is
+ * SCL_ERROR_DIMENSION_MISMATCH).  r vanishes, up to rounding, for symmetric A
+ * with symmetric == 0. */
 SCL_EXPORT scl_error_t scl_gpu_compute_transition_matrix_ex(
     scl_sparse_t adjacency, int symmetric, double* degrees_out,
     double* residual_out);
@@ -233,6 +234,17 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
     scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
 
+/* The same with P's pattern and the values values_dev (nnz floats, the
+ * handle's stored order, device memory) in place of the handle's values,
+ * which stay untouched: for callers that differentiate with respect to the
+ * values (torch.autograd).  values_dev == NULL means the handle's values.
+ * Shapes that read a grouped or tiled copy of the handle's values fall back
+ * to the default shape. */
+SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_values_device(
+    scl_sparse_t transition, const scl_real_t* values_dev, const scl_real_t* X_dev,
+    scl_real_t* Y_dev, scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
 /* Edge gradient of one step on its own (CSR order):
  * dP_dev[k] (+)= <G_dev[i,:], Xprev_dev[j,:]> for every nonzero k = (i, j);
  * accumulate == 0 overwrites dP_dev.  The reverse pass above computes this
@@ -256,6 +268,23 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
     const scl_real_t* dY_dev, scl_real_t* dX_dev, scl_real_t* dP_dev,
     void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
+/* The reverse pass with P's pattern and the values values_dev (as in
+ * scl_gpu_diffuse_forward_ckpt_values_device); its workspace, from the
+ * query below, holds nnz more floats (the values in transposed order). */
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_workspace(
+    scl_sparse_t transition, scl_index_t n_nodes, scl_index_t n_features,
+    scl_index_t every, int want_dp, scl_size_t* bytes);
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_device(
+    scl_sparse_t transition, const scl_real_t* values_dev, const scl_real_t* X_dev,
+    const scl_real_t* ckpt_dev, const scl_real_t* dY_dev, scl_real_t* dX_dev,
+    scl_real_t* dP_dev, void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream);
+
+/* A counter that changes whenever the handle's values or pattern may have
+ * changed (any value update through the library, and every host-pointer call
+ * on a handle made by scl_sparse_wrap, whose arrays the caller may edit). */
+SCL_EXPORT scl_error_t scl_gpu_sparse_generation(scl_sparse_t matrix, uint64_t* out);
 
 /* ----------------------------------------------------------------- plans --
  * A fixed sequence recorded once into a hipGraph and replayed with one

@@ -260,6 +260,30 @@ def stationarity_residual_f64(indptr, indices, vals, deg):
                                                 d.ctypes.data)
 
 
+def ulp_distance(a, b) -> np.ndarray:
+    """Per-element distance in fp32 ulps between two float32 arrays (the
+    number of representable floats between them; +0 and -0 coincide)."""
+    def ordered(x):
+        i = _f32(x).ravel().view(np.int32).astype(np.int64)
+        return np.where(i < 0, -(i & 0x7FFFFFFF), i)
+    return np.abs(ordered(a) - ordered(b))
+
+
+def ulps_bound_fp32_transition(indptr, indices, symmetric=False) -> np.ndarray:
+    """First-order bound, in ulps, of how far the reference's FP32
+    normalisation (scl/kernel/diffusion.hpp:452-603: sequential fp32 row sum
+    d_i, then a_ij / d_i or a_ij d_i^-1/2 d_j^-1/2) can lie from the
+    correctly rounded FP64 result: a sum of L terms is off by at most
+    (L - 1) u relative, each further operation by u, and a relative error e
+    is at most 2^24 e ulps.  Row mode: L_i + 2; symmetric: (L_i + L_j)/2 + 4."""
+    ip, ix = _i64(indptr), _i64(indices)
+    lens = np.diff(ip)
+    row = np.repeat(np.arange(len(lens)), lens)
+    if symmetric:
+        return (lens[row] + lens[ix] + 1) // 2 + 4
+    return lens[row] + 2
+
+
 def rel_err(a, b) -> float:
     """max|a-b| / max(max|a|, max|b|) (reference precision.hpp:102-120)."""
     a = _f32(a).ravel()

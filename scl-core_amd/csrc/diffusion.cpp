@@ -95,6 +95,10 @@ void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
     if (symmetric && A->rows != A->cols)
         fail(SCL_ERROR_DIMENSION_MISMATCH,
              "Symmetric normalisation needs a square matrix");
+    // the residual pairs the degree of row i with transposed row i: square only
+    if (residual_out && A->rows != A->cols)
+        fail(SCL_ERROR_DIMENSION_MISMATCH,
+             "The stationarity residual needs a square matrix");
     const int dev = current_device();
     std::lock_guard<std::mutex> call_lk(device_mutex(dev));
     std::lock_guard<std::mutex> lk(A->mu);

@@ -50,11 +50,24 @@ SpmmShape fwd_shape(DeviceCSR& d, int64_t F, const void* a, const void* b) {
     return spmm_default_shape(F, a, b);
 }
 
-size_t backward_bytes(int64_t n, int64_t F, int64_t nnz, int64_t every, bool want_dp) {
+size_t backward_bytes(int64_t n, int64_t F, int64_t nnz, int64_t every, bool want_dp,
+                      bool values = false) {
     const size_t plane = round_up((size_t)n * (size_t)F * sizeof(float));
     size_t b = plane * (size_t)(2 + (every - 1));
     if (want_dp) b += round_up((size_t)nnz * sizeof(float));
+    if (values) b += round_up((size_t)nnz * sizeof(float));  // transposed values
     return b;
+}
+
+// A forward view over the operator's pattern with the caller's values
+// (stored order).  The grouped and tiled forms hold copies of the handle's
+// values, so a shape that reads one falls back to the default.
+void with_values(CsrView& v, SpmmShape& sh, const float* values, int64_t F, const void* a,
+                 const void* b) {
+    v.val = values;
+    v.grouped = nullptr;
+    v.tiled = nullptr;
+    if (sh.rows_per_group && sh.rows_per_group != kStagedRows) sh = spmm_default_shape(F, a, b);
 }
 
 }  // namespace
@@ -116,6 +129,15 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream) {
+    return scl_gpu_diffuse_forward_ckpt_values_device(transition, nullptr, X_dev, Y_dev,
+                                                      ckpt_dev, work_dev, n_nodes,
+                                                      n_features, n_steps, every, stream);
+}
+
+SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_values_device(
+    scl_sparse_t transition, const scl_real_t* values_dev, const scl_real_t* X_dev,
+    scl_real_t* Y_dev, scl_real_t* ckpt_dev, scl_real_t* work_dev, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream) {
     SCL_API_CHECK_NULL(transition, "Transition matrix is null");
     SCL_API_CHECK(X_dev && Y_dev, SCL_ERROR_NULL_POINTER, "X or Y is null");
     SCL_API_CHECK(n_nodes > 0 && n_features > 0 && n_steps > 0 && every > 0,
@@ -135,6 +157,7 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_forward_ckpt_device(
             DeviceCSR& d = device_csr(transition, dev);
             v = fwd_view(d);
             sh = fwd_shape(d, n_features, X_dev, Y_dev);
+            if (values_dev) with_values(v, sh, values_dev, n_features, X_dev, Y_dev);
             if (!spmm_shape_valid(sh, n_features, ckpt_dev ? ckpt_dev : Y_dev, work_dev ? work_dev : Y_dev))
                 sh = spmm_default_shape(n_features, X_dev, Y_dev);
         }
@@ -167,10 +190,36 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_workspace(
     SCL_API_CATCH
 }
 
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_workspace(
+    scl_sparse_t transition, scl_index_t n_nodes, scl_index_t n_features,
+    scl_index_t every, int want_dp, scl_size_t* bytes) {
+    SCL_API_CHECK_NULL(transition, "Transition matrix is null");
+    SCL_API_CHECK_NULL(bytes, "Output pointer is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0 && every > 0,
+                  SCL_ERROR_INVALID_ARGUMENT, "Dimensions must be positive");
+    SCL_API_TRY
+        check_square(transition, n_nodes, n_features);
+        *bytes = backward_bytes(n_nodes, n_features, transition->nnz, every, want_dp != 0,
+                                true);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
 SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
     scl_sparse_t transition, const scl_real_t* X_dev, const scl_real_t* ckpt_dev,
     const scl_real_t* dY_dev, scl_real_t* dX_dev, scl_real_t* dP_dev,
     void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
+    scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream) {
+    return scl_gpu_diffuse_backward_values_device(transition, nullptr, X_dev, ckpt_dev, dY_dev,
+                                                  dX_dev, dP_dev, work_dev, work_bytes,
+                                                  n_nodes, n_features, n_steps, every,
+                                                  stream);
+}
+
+SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_device(
+    scl_sparse_t transition, const scl_real_t* values_dev, const scl_real_t* X_dev,
+    const scl_real_t* ckpt_dev, const scl_real_t* dY_dev, scl_real_t* dX_dev,
+    scl_real_t* dP_dev, void* work_dev, scl_size_t work_bytes, scl_index_t n_nodes,
     scl_index_t n_features, scl_index_t n_steps, scl_index_t every, void* stream) {
     SCL_API_CHECK_NULL(transition, "Transition matrix is null");
     SCL_API_CHECK(X_dev && dY_dev && dX_dev && work_dev, SCL_ERROR_NULL_POINTER,
@@ -185,9 +234,11 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
         check_square(transition, n_nodes, n_features);
         const int64_t n = n_nodes, F = n_features, t = n_steps;
         const bool want_dp = dP_dev != nullptr && transition->nnz > 0;
-        if (work_bytes < backward_bytes(n, F, transition->nnz, every, want_dp))
+        const bool own_values = values_dev != nullptr && transition->nnz > 0;
+        if (work_bytes < backward_bytes(n, F, transition->nnz, every, want_dp, own_values))
             fail(SCL_ERROR_INVALID_ARGUMENT,
-                 "work buffer smaller than scl_gpu_diffuse_backward_workspace");
+                 own_values ? "work buffer smaller than scl_gpu_diffuse_backward_values_workspace"
+                            : "work buffer smaller than scl_gpu_diffuse_backward_workspace");
         const int dev = current_device();
         hipStream_t s = static_cast<hipStream_t>(stream);
         CsrView fv, av;
@@ -217,8 +268,19 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
         auto rslot = [&](int64_t r) {
             return reinterpret_cast<float*>(w + plane_b * (size_t)(1 + r));
         };
-        float* dPT = want_dp ? reinterpret_cast<float*>(w + plane_b * (size_t)(1 + every))
-                             : nullptr;
+        char* tail = w + plane_b * (size_t)(1 + every);
+        float* dPT = nullptr;
+        if (want_dp) {
+            dPT = reinterpret_cast<float*>(tail);
+            tail += round_up((size_t)transition->nnz * sizeof(float));
+        }
+        if (own_values) {
+            // the caller's values in transposed order for the adjoint steps
+            float* tv = reinterpret_cast<float*>(tail);
+            launch_gather_f32(t_src, values_dev, tv, transition->nnz, s);
+            av.val = tv;
+            with_values(fv, sh, values_dev, F, gbuf[0], rslot(1));
+        }
         if (!spmm_shape_valid(sh, F, gbuf[0], rslot(1)))
             sh = spmm_default_shape(F, gbuf[0], rslot(1));
         const int64_t M = (t + every - 1) / every;
@@ -243,6 +305,16 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_device(
             }
         }
         if (want_dp) launch_scatter_f32(t_src, dPT, dP_dev, transition->nnz, s);
+        SCL_API_RETURN_OK;
+    SCL_API_CATCH
+}
+
+SCL_EXPORT scl_error_t scl_gpu_sparse_generation(scl_sparse_t matrix, uint64_t* out) {
+    SCL_API_CHECK_NULL(matrix, "Matrix is null");
+    SCL_API_CHECK_NULL(out, "Output pointer is null");
+    SCL_API_TRY
+        std::lock_guard<std::mutex> lk(matrix->mu);
+        *out = matrix->generation;
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

@@ -133,6 +133,13 @@ SIGNATURES = {
     "scl_gpu_plan_destroy": (c_err, [PP]),
     "scl_gpu_diffuse_backward_device": (
         c_err, [P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
+    "scl_gpu_diffuse_forward_ckpt_values_device": (
+        c_err, [P, P, P, P, P, P, c_index, c_index, c_index, c_index, P]),
+    "scl_gpu_diffuse_backward_values_workspace": (
+        c_err, [P, c_index, c_index, c_index, ctypes.c_int, ctypes.POINTER(c_size)]),
+    "scl_gpu_diffuse_backward_values_device": (
+        c_err, [P, P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
+    "scl_gpu_sparse_generation": (c_err, [P, ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lock = threading.Lock()

@@ -28,15 +28,24 @@ def _stream(t: torch.Tensor):
     return torch.cuda.current_stream(t.device).cuda_stream
 
 
+def _generation(P) -> int:
+    g = ctypes.c_uint64(0)
+    _lib.check(_lib.load().scl_gpu_sparse_generation(P.handle, ctypes.byref(g)))
+    return g.value
+
+
 class _Diffuse(torch.autograd.Function):
+    """With ``values`` the forward and the reverse pass read P's pattern with
+    those values (saved for backward, so an in-place edit between the two
+    trips autograd's version check); P's own values are never written.
+    Without them both read P's values, and backward refuses to run when P
+    changed in between."""
+
     @staticmethod
     def forward(ctx, X, values, P, n_steps, every):
         lib = _lib.load()
-        if values is not None:
-            v = values.detach().contiguous()
-            _lib.check(lib.scl_gpu_sparse_set_values_device(P.handle, v.data_ptr(),
-                                                            _stream(X)))
         Xc = X.detach().contiguous()
+        v = values.detach().contiguous() if values is not None else None
         n, F = Xc.shape
         n_ckpt = (n_steps + every - 1) // every - 1
         Y = torch.empty_like(Xc)
@@ -44,34 +53,42 @@ class _Diffuse(torch.autograd.Function):
             if n_ckpt > 0 else None
         work = torch.empty((2, n, F), dtype=Xc.dtype, device=Xc.device) \
             if n_steps >= 2 else None
-        _lib.check(lib.scl_gpu_diffuse_forward_ckpt_device(
-            P.handle, Xc.data_ptr(), Y.data_ptr(),
+        _lib.check(lib.scl_gpu_diffuse_forward_ckpt_values_device(
+            P.handle, None if v is None else v.data_ptr(), Xc.data_ptr(), Y.data_ptr(),
             None if ckpt is None else ckpt.data_ptr(),
             None if work is None else work.data_ptr(), n, F, n_steps, every,
             _stream(Xc)))
         del work
         ctx.P, ctx.n_steps, ctx.every = P, n_steps, every
-        ctx.has_values = values is not None
-        ctx.save_for_backward(Xc, ckpt if ckpt is not None else Xc.new_empty(0))
+        ctx.has_values = v is not None
+        ctx.generation = None if v is not None else _generation(P)
+        ctx.save_for_backward(Xc, ckpt if ckpt is not None else Xc.new_empty(0),
+                              v if v is not None else Xc.new_empty(0))
         ctx.n_ckpt = n_ckpt
         return Y
 
     @staticmethod
     def backward(ctx, dY):
         lib = _lib.load()
-        Xc, ckpt = ctx.saved_tensors
+        Xc, ckpt, v = ctx.saved_tensors
         n, F = Xc.shape
         P = ctx.P
+        if not ctx.has_values and _generation(P) != ctx.generation:
+            raise RuntimeError("the operator changed between forward and backward; pass "
+                               "values= to differentiate through a changing operator")
         want_dp = ctx.has_values and ctx.needs_input_grad[1]
         dYc = dY.contiguous()
         nbytes = ctypes.c_size_t(0)
-        _lib.check(lib.scl_gpu_diffuse_backward_workspace(
-            P.handle, n, F, ctx.every, 1 if want_dp else 0, ctypes.byref(nbytes)))
+        query = (lib.scl_gpu_diffuse_backward_values_workspace if ctx.has_values
+                 else lib.scl_gpu_diffuse_backward_workspace)
+        _lib.check(query(P.handle, n, F, ctx.every, 1 if want_dp else 0,
+                         ctypes.byref(nbytes)))
         work = torch.empty(nbytes.value, dtype=torch.uint8, device=Xc.device)
         dX = torch.empty_like(Xc)
         dP = torch.empty(P.nnz, dtype=Xc.dtype, device=Xc.device) if want_dp else None
-        _lib.check(lib.scl_gpu_diffuse_backward_device(
-            P.handle, Xc.data_ptr(), ckpt.data_ptr() if ctx.n_ckpt > 0 else None,
+        _lib.check(lib.scl_gpu_diffuse_backward_values_device(
+            P.handle, v.data_ptr() if ctx.has_values else None, Xc.data_ptr(),
+            ckpt.data_ptr() if ctx.n_ckpt > 0 else None,
             dYc.data_ptr(), dX.data_ptr(), None if dP is None else dP.data_ptr(),
             work.data_ptr(), nbytes.value, n, F, ctx.n_steps, ctx.every, _stream(Xc)))
         return (dX if ctx.needs_input_grad[0] else None, dP, None, None, None)
@@ -82,9 +99,9 @@ def diffuse(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
             ) -> torch.Tensor:
     """Differentiable ``P^t X``.
 
-    ``values`` (nnz float32 on X's device, stored order of P) replaces P's
-    entries and receives ``dL/dP`` when it requires grad; without it P is a
-    constant and only ``dL/dX`` is produced.
+    ``values`` (nnz float32 on X's device, stored order of P) stands in for
+    P's entries (P itself is not modified) and receives ``dL/dP`` when it
+    requires grad; without it P is a constant and only ``dL/dX`` is produced.
     """
     if X.dtype != torch.float32 or X.dim() != 2 or not X.is_cuda:
         raise ValueError("expected a (n, F) float32 device tensor")

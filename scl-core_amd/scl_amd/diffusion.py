@@ -31,16 +31,18 @@ def compute_transition_matrix(adjacency: SparseMatrixBase, symmetric: bool = Fal
 
 
 def compute_transition_matrix_ex(adjacency: SparseMatrixBase,
-                                 symmetric: bool = False
-                                 ) -> Tuple[np.ndarray, float]:
-    """Normalises in place and returns (FP64 degrees, FP64 stationarity residual)."""
+                                 symmetric: bool = False, residual: bool = True
+                                 ) -> Tuple[np.ndarray, Optional[float]]:
+    """Normalises in place and returns (FP64 degrees, FP64 stationarity
+    residual); the residual (square matrices only) is None when not asked
+    for."""
     rows = adjacency.shape[0] if adjacency._is_csr else adjacency.shape[1]
     deg = np.empty(rows, np.float64)
     res = ctypes.c_double(0.0)
     _lib.check(_lib.load().scl_gpu_compute_transition_matrix_ex(
         adjacency.handle, 1 if symmetric else 0, deg.ctypes.data,
-        ctypes.addressof(res)))
-    return deg, res.value
+        ctypes.addressof(res) if residual else None))
+    return deg, (res.value if residual else None)
 
 
 def diffuse_matrix(transition: SparseMatrixBase, X: np.ndarray, n_steps: int,

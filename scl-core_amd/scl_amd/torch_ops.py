@@ -167,3 +167,14 @@ def spmm(P: SparseMatrixBase, X: torch.Tensor, transpose: bool = False
             P.handle, X.data_ptr(), Y.data_ptr(), n, F, 1 if transpose else 0,
             _stream_ptr(X)))
     return Y
+
+
+def set_values(P: SparseMatrixBase, values: torch.Tensor) -> None:
+    """Replaces P's values (nnz float32 on the device, stored order) in its
+    device copies and host arrays (scl_gpu_sparse_set_values_device)."""
+    v = values.detach().contiguous()
+    if v.dtype != torch.float32 or v.numel() != P.nnz or not v.is_cuda:
+        raise ValueError("expected nnz float32 values on the device")
+    with torch.cuda.device(v.device):
+        _lib.check(_lib.load().scl_gpu_sparse_set_values_device(
+            P.handle, v.data_ptr(), _stream_ptr(v)))

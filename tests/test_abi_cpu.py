@@ -132,6 +132,10 @@ def test_diffusion_argument_checks_come_first():
     assert lib.scl_diffusion_compute_transition_matrix(None, 0) == 4
     rect = CsrMatrix.from_arrays([0, 1, 2], [0, 2], [1, 1], (2, 3))
     assert lib.scl_diffusion_compute_transition_matrix(rect.handle, 1) == 11
+    # the stationarity residual pairs degree i with transposed row i: square only
+    res = ctypes.c_double(0.0)
+    assert lib.scl_gpu_compute_transition_matrix_ex(rect.handle, 0, None,
+                                                    ctypes.addressof(res)) == 11
     # device entry points validate too
     g = lib.scl_gpu_diffuse_matrix_device
     assert g(P.handle, None, X.ctypes.data, None, 3, 2, 1, None) == 4

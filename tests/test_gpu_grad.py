@@ -155,3 +155,46 @@ def test_every_adjoint_shape_agrees(gpu, F):
     torch_ops.autotune(P, Xd, adjoint=True)
     best = torch_ops.get_shape(P, F, adjoint=True)
     assert (best["vec"], best["group"], best["schedule"]) in shapes
+
+
+def test_two_forwards_with_different_values_then_one_backward(gpu):
+    """Each forward reads its own values tensor (saved for backward); P's
+    host and device values are never written, so two forwards on one P with
+    different values, then one backward through both, give each forward's
+    own FP64 gradient."""
+    n, F, t = 3000, 24, 4
+    g, X, T = _problem(n, 9, F, seed=41)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    before = P.export()[2].copy()
+    rng = np.random.default_rng(42)
+    other = (g.data * rng.uniform(0.5, 1.5, g.nnz)).astype(np.float32)
+    dev = torch.device("cuda:0")
+    Xd = torch.from_numpy(X).to(dev)
+    Td = torch.from_numpy(T).to(dev)
+    va = torch.from_numpy(g.data.copy()).to(dev).requires_grad_(True)
+    vb = torch.from_numpy(other).to(dev).requires_grad_(True)
+    Xa = Xd.clone().requires_grad_(True)
+    Xb = Xd.clone().requires_grad_(True)
+    Ya = autograd.diffuse(P, Xa, t, values=va)
+    Yb = autograd.diffuse(P, Xb, t, values=vb)
+    L = 0.5 * ((Ya - Td) ** 2).sum() + 0.5 * ((Yb - Td) ** 2).sum()
+    L.backward()
+    for vals, Xg, vg in ((g.data, Xa.grad, va.grad), (other, Xb.grad, vb.grad)):
+        _, _, dX64, dP64 = orc.grad_f64(g.indptr, g.indices, vals, X, T, t)
+        ex = orc.rel_err(Xg.cpu().numpy(), dX64)
+        ep = orc.rel_err(vg.cpu().numpy(), dP64)
+        assert ex <= 1e-3 and ep <= 1e-3, (ex, ep)
+    assert np.array_equal(P.export()[2], before)
+    # and the forward with P's own values still matches the plain kernel
+    assert torch.equal(torch_ops.diffuse(P, Xd, t), Ya.detach())
+
+
+def test_constant_operator_changed_before_backward_is_refused(gpu):
+    n, F = 2000, 8
+    g, X, T = _problem(n, 6, F, seed=43)
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Xd = torch.from_numpy(X).cuda().requires_grad_(True)
+    Y = autograd.diffuse(P, Xd, 3)
+    torch_ops.set_values(P, torch.from_numpy(g.data * 0.5).cuda())
+    with pytest.raises(RuntimeError, match="changed between forward and backward"):
+        Y.sum().backward()

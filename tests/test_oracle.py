@@ -327,3 +327,30 @@ def test_rounded_input_restatement_by_hand():
             orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, 1, fmt, scale_exp=k,
                                  round_input=True),
             orc.spmm_step_f32(g.indptr, g.indices, g.data, Xq))
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_fp64_normalisation_within_fp32_formula_error(symmetric):
+    """The FP64 normalisation (device path) and the reference's FP32 formula
+    differ only by the FP32 formula's own rounding: every entry lies within
+    the first-order bound of that error.  A wrong term would not: dividing by
+    the wrong row's degree moves entries by far more than the bound."""
+    g = synth.knn_graph(20_000, 15, seed=31)
+    e64, _ = orc.transition_f64(g.indptr, g.indices, g.data, symmetric)
+    r32 = orc.transition_f32(g.indptr, g.indices, g.data, symmetric)
+    bound = orc.ulps_bound_fp32_transition(g.indptr, g.indices, symmetric)
+    ulps = orc.ulp_distance(e64, r32)
+    assert np.all(ulps <= bound) and ulps.max() >= 1
+    if not symmetric:   # planted mistake: the next row's degree
+        lens = np.diff(g.indptr)
+        rs = np.add.reduceat(g.data.astype(np.float64), g.indptr[:-1])
+        wrong = (g.data / np.repeat(np.roll(rs, -1), lens)).astype(np.float32)
+        assert np.any(orc.ulp_distance(wrong, r32) > bound)
+
+
+def test_ulp_distance_counts_representable_steps():
+    a = np.array([1.0, -1.0, 0.0, -0.0, 1.0], np.float32)
+    b = np.array([np.nextafter(np.float32(1), np.float32(2)), -1.0, -0.0,
+                  np.float32(1e-45), np.nextafter(np.nextafter(np.float32(1), np.float32(0)),
+                                                  np.float32(0))], np.float32)
+    assert orc.ulp_distance(a, b).tolist() == [1, 0, 0, 1, 2]
