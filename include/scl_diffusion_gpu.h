@@ -45,6 +45,23 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
     scl_real_t* work_dev, scl_index_t n_nodes, scl_index_t n_features,
     scl_index_t n_steps, void* stream);
 
+/* Y_dev = P^t X_dev like scl_gpu_diffuse_matrix_device, for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * It allocates nothing: the intermediate iterate (n_steps >= 2) lives in a
+ * fixed array of SCL_GPU_NOALLOC_SCRATCH_FLOATS floats (64 MiB) that the
+ * library holds on every GPU it runs on until the process ends; larger
+ * n_nodes * n_features with n_steps >= 2 is SCL_ERROR_INVALID_ARGUMENT.  The
+ * operator must already be on the current device and unchanged since
+ * (scl_gpu_sparse_prepare, a set_shape or an earlier device call), else
+ * SCL_ERROR_INVALID_ARGUMENT.  All calls on one GPU share its array: calls
+ * that overlap on one GPU (other streams or threads) race.  Results equal
+ * scl_gpu_diffuse_matrix_device's bit for bit. */
+#define SCL_GPU_NOALLOC_SCRATCH_FLOATS ((size_t)16 * 1024 * 1024)
+SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device_noalloc(
+    scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
+    scl_index_t n_nodes, scl_index_t n_features, scl_index_t n_steps, void* stream);
+
 /* One step Y_dev = op(P) X_dev with op(P) = P (transpose == 0) or P^T
  * (transpose != 0, applied through a cached transposed pattern). */
 SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,

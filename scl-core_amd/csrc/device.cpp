@@ -401,6 +401,13 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
     return *slot;
 }
 
+DeviceCSR* device_csr_current(scl_sparse_matrix* h, int device) {
+    const auto it = h->dev.find(device);
+    if (it == h->dev.end() || !it->second || it->second->generation != h->generation)
+        return nullptr;
+    return it->second.get();
+}
+
 void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
     (void)s;
     if (d.t_row_ptr && d.t_generation == d.generation) return;

@@ -190,6 +190,9 @@ std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 // Device copy of a handle, refreshed when its generation changed.
 // Caller holds h->mu.
 DeviceCSR& device_csr(scl_sparse_matrix* h, int device);
+// The device copy when it exists and matches the handle's generation, else
+// null; never uploads or allocates.  Caller holds h->mu.
+DeviceCSR* device_csr_current(scl_sparse_matrix* h, int device);
 // Builds the transposed pattern on the device if missing or stale.
 void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s);
 

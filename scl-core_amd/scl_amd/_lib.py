@@ -140,6 +140,8 @@ SIGNATURES = {
     "scl_gpu_diffuse_backward_values_device": (
         c_err, [P, P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
     "scl_gpu_sparse_generation": (c_err, [P, ctypes.POINTER(ctypes.c_uint64)]),
+    "scl_gpu_diffuse_matrix_device_noalloc": (
+        c_err, [P, P, P, c_index, c_index, c_index, P]),
 }
 
 _lock = threading.Lock()
