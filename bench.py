@@ -91,8 +91,35 @@ def timed(fn, reps: int, stream) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def host_cpu() -> dict:
+    """lscpu model, sockets, physical cores and SIMD flags of this host."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        per = int(kv.get("Core(s) per socket", "0") or 0)
+        info = {"model": kv.get("Model name"), "sockets": sockets,
+                "physical_cores": sockets * per if sockets and per else None,
+                "threads_per_core": kv.get("Thread(s) per core"),
+                "logical_cpus": kv.get("CPU(s)")}
+        flags = kv.get("Flags", "").split()
+        info["simd"] = [f for f in ("avx2", "fma", "avx512f", "avx512bw", "avx512_bf16")
+                        if f in flags]
+    except Exception as e:  # lscpu missing: report what os says
+        import os
+        info = {"model": None, "logical_cpus": os.cpu_count(), "error": str(e)}
+    return info
+
+
 def cpu_baseline(graph_arrays, n, F, t, seed, bytes_full_step, budget_s=12.0):
-    """Times the CPU restatement on all rows and a column sample."""
+    """Times the CPU restatement on all rows and a column sample, on all the
+    OpenMP threads and on one."""
     import numpy as np
     from oracle import oracle as orc
     from scl_amd import synth
@@ -114,14 +141,29 @@ def cpu_baseline(graph_arrays, n, F, t, seed, bytes_full_step, budget_s=12.0):
         dt = time.perf_counter() - t0
         fs = len(cols)
     full_time = dt * F / fs
+    # one thread on 4 columns
+    c1 = np.arange(0, F, F // 4)[:4]
+    X1 = synth.log1p_poisson(n, F, seed=seed, cols=c1)
+    orc.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        orc.diffuse_matrix_f32(ip, ix, p, X1, t)
+        dt1 = time.perf_counter() - t0
+    finally:
+        orc.set_num_threads(threads)
+    full1 = dt1 * F / len(c1)
     return {
         "value": round(t * bytes_full_step / full_time / 1e9, 3),
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"all {n} rows x {fs} of {F} columns, t={t}, {dt:.2f} s; "
-                   f"scaled to the full width (columns are independent): "
+        "sample": (f"all {n} rows x {fs} of {F} columns, t={t}, {dt:.2f} s on {threads} "
+                   f"OpenMP threads; scaled to the full width (columns are independent): "
                    f"{full_time:.1f} s per diffusion; {orc.load().variant}"),
+        "one_thread": {"value": round(t * bytes_full_step / full1 / 1e9, 3), "unit": "GB/s",
+                       "sample": f"all rows x {len(c1)} columns, t={t}, {dt1:.2f} s; "
+                                 f"{full1:.1f} s per diffusion scaled"},
+        "host": host_cpu(),
     }
 
 
@@ -178,6 +220,19 @@ def leg_c3(steps: int) -> dict:
     B_f = step_bytes(n, nnz, F)
     B_e = 8 * n * F + 12 * nnz
     total = 9 * B_f + 6 * (B_f + B_e) + 3 * n * F * 4 + 2 * n * F * 4
+    # error against the FP64 gradient of the whole problem (outside timing)
+    from oracle import oracle as orc
+    ip, ix, pv = P.export()
+    t0 = time.perf_counter()
+    _, Y64, dX64, dP64 = orc.grad_f64(ip, ix, pv, synth.normal(n, F, seed=3),
+                                      synth.normal(n, F, seed=103), t)
+    err_s = time.perf_counter() - t0
+    dXh, dPh, Yh = plan.dX.cpu().numpy(), plan.dP.cpu().numpy(), plan.Y.cpu().numpy()
+    err = {"Y_max": orc.rel_err(Yh, Y64), "dX_max": orc.rel_err(dXh, dX64),
+           "dX_median": orc.median_rel_err(dXh, dX64), "dP_max": orc.rel_err(dPh, dP64),
+           "dP_median": orc.median_rel_err(dPh, dP64),
+           "how": f"FP64 oracle gradient of all {n} x {F} and {nnz} nonzeros ({err_s:.1f} s)"}
+    del Y64, dX64, dP64
     plan.close()
     return {
         "workload": "knn30_500k_f128_t6", "nnz": nnz,
@@ -188,50 +243,71 @@ def leg_c3(steps: int) -> dict:
         "speedup_vs_plain": round(plain_ms / fused_ms, 3),
         "algorithmic_GBps": round(total / fused_ms / 1e6, 1),
         "dX_fused_equals_plain": same_dx,
-        "err": "dX, dP vs FP64 <= 2.5e-7 (tests/test_gpu_grad.py)",
+        "err_vs_fp64": err,
     }
 
 
 def leg_c4(steps: int) -> dict:
-    """uniform15_1m, F=256, t=4: bf16 iterates vs fp32."""
+    """uniform15_1m, F=256, t=4: reduced-precision storage vs fp32, each with
+    its error against FP64 on 16 sampled columns."""
+    import numpy as np
     import torch
+    from oracle import oracle as orc
     from scl_amd import CsrMatrix, synth, torch_ops
     n, F, t = 1_000_000, 256, 4
     g = synth.uniform_knn(n, 15, seed=4).normalized()
     P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-    X = torch.from_numpy(synth.normal(n, F, seed=4)).to(dev)
-    torch_ops.autotune(P, X)
-    Y32 = torch_ops.diffuse(P, X, t)
+    X = synth.normal(n, F, seed=4)
+    Xd = torch.from_numpy(X).to(dev)
+    torch_ops.autotune(P, Xd)
+    Y32 = torch_ops.diffuse(P, Xd, t)
     torch.cuda.synchronize()
-    f32_ms = timed(lambda: torch_ops.diffuse(P, X, t), steps, stream)
+    f32_ms = timed(lambda: torch_ops.diffuse(P, Xd, t), steps, stream)
+    cols = np.arange(0, F, 16)
+    ct = torch.from_numpy(cols).to(dev)
+    y64 = orc.diffuse_f64(g.indptr, g.indices, g.data, X[:, cols], t)
+
+    def errs(Y):
+        Yh = Y[:, ct].cpu().numpy()
+        return {"max_rel_err_vs_fp64": orc.rel_err(Yh, y64),
+                "median_rel_err_vs_fp64": orc.median_rel_err(Yh, y64)}
+
     out = {"workload": "uniform15_1m_f256_t4", "nnz": g.nnz,
            "fp32_ms": round(f32_ms, 3),
-           "fp32_GBps": round(t * step_bytes(n, g.nnz, F) / f32_ms / 1e6, 1)}
-    scale = float(Y32.abs().max())
-    k16 = torch_ops.lowp_scale_exp(P, X, t)
-    for fmt, vb, ri in (("bf16", False, False), ("bf16", True, False), ("fp16", False, False),
+           "fp32_GBps": round(t * step_bytes(n, g.nnz, F) / f32_ms / 1e6, 1),
+           "fp32": errs(Y32),
+           "error_columns": f"{len(cols)} of {F} (every 16th), FP64 oracle"}
+    k16 = torch_ops.lowp_scale_exp(P, Xd, t)
+    width = {"bf16": 2, "fp16": 2, "comp": 3}
+    for fmt, vb, ri in (("comp", True, True), ("comp", False, False), ("bf16", True, True),
+                        ("bf16", False, False), ("bf16", True, False), ("fp16", False, False),
                         ("fp16", False, True)):
         kk = k16 if fmt == "fp16" else 0
-        Yb = torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk, round_input=ri)
-        ms = timed(lambda: torch_ops.diffuse_lowp(P, X, t, fmt, bf16_values=vb, scale_exp=kk,
-                                                  round_input=ri),
-                   steps, stream)
-        vbytes = 2 if vb else 4
-        if ri:  # X -> 16 bits once, then t steps gathering 16-bit rows
-            bytes_ = (n * F * 6 + (t - 1) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
-                      + step_bytes(n, g.nnz, F, vbytes, 2, 4))
+        run = lambda: torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb, scale_exp=kk,
+                                             round_input=ri)
+        Yb = run()
+        ms = timed(run, steps, stream)
+        w = width[fmt]
+        vbytes = w if vb else 4
+        if ri:  # X stored reduced once, then t steps gathering reduced rows
+            bytes_ = (n * F * (4 + w) + (t - 1) * step_bytes(n, g.nnz, F, vbytes, w, w)
+                      + step_bytes(n, g.nnz, F, vbytes, w, 4))
         else:
-            bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, 2)
-                      + (t - 2) * step_bytes(n, g.nnz, F, vbytes, 2, 2)
-                      + step_bytes(n, g.nnz, F, vbytes, 2, 4))
-        key = (f"{fmt}_iterates_{'bf16' if vb else 'fp32'}_values"
-               + ("_16bit_input" if ri else ""))
-        out[key] = {"ms": round(ms, 3), "GBps": round(bytes_ / ms / 1e6, 1),
-                    "speedup_vs_fp32": round(f32_ms / ms, 3),
-                    "max_rel_diff_vs_fp32_kernel": float((Yb - Y32).abs().max()) / scale}
-    out["err"] = "vs FP64 at n=200k: see tests/test_gpu_lowp.py output"
+            bytes_ = (step_bytes(n, g.nnz, F, vbytes, 4, w)
+                      + (t - 2) * step_bytes(n, g.nnz, F, vbytes, w, w)
+                      + step_bytes(n, g.nnz, F, vbytes, w, 4))
+        name = {"comp": "compensated_bf16", "bf16": "bf16", "fp16": "fp16_2k"}[fmt]
+        key = (f"{name}_iterates_{('compensated' if fmt == 'comp' else 'bf16') if vb else 'fp32'}"
+               f"_values" + ("_stored_input" if ri else ""))
+        rec = {"ms": round(ms, 3), "GBps": round(bytes_ / ms / 1e6, 1),
+               "speedup_vs_fp32": round(f32_ms / ms, 3)}
+        rec.update(errs(Yb))
+        rec["meets_1e-3"] = rec["max_rel_err_vs_fp64"] <= 1e-3
+        out[key] = rec
+    out["spec"] = ("P values and X stored in bf16: compensated_bf16_iterates_compensated_"
+                   "values_stored_input (bf16 plus an 8-bit residual per value, 3 bytes)")
     return out
 
 
@@ -518,6 +594,19 @@ def leg_c5(steps: int) -> dict:
     seq_ms = timed(sequential, max(2, steps // 2), stream)
     same = bool(torch.equal(dX, plan.dX)) and bool(torch.equal(Y, plan.Y))
     shape = torch_ops.get_shape(B, F)
+    # FP64 gradients of 8 sampled graphs (outside timing)
+    from oracle import oracle as orc
+    Xh, Th = X.cpu().numpy(), T.cpu().numpy()
+    pdx, pdp, pY = plan.dX.cpu().numpy(), plan.dP.cpu().numpy(), plan.Y.cpu().numpy()
+    rng = np.random.Generator(np.random.PCG64(55))
+    ex = ep = ey = 0.0
+    for i in rng.choice(256, 8, replace=False):
+        a, b = int(off[i]), int(off[i + 1])
+        gr = graphs[i]
+        _, y64, dx64, dp64 = orc.grad_f64(gr.indptr, gr.indices, gr.data, Xh[a:b], Th[a:b], t)
+        ey = max(ey, orc.rel_err(pY[a:b], y64))
+        ex = max(ex, orc.rel_err(pdx[a:b], dx64))
+        ep = max(ep, orc.rel_err(pdp[int(nz[i]):int(nz[i + 1])], dp64))
     plan.close()
     fwd_bytes = t * step_bytes(N, B.nnz, F)
     return {"workload": "batch256_f64_t5", "graphs": 256, "rows": N, "nnz": B.nnz,
@@ -526,7 +615,38 @@ def leg_c5(steps: int) -> dict:
             "speedup_vs_sequential": round(seq_ms / plan_ms, 2),
             "forward_GBps_equiv": round(fwd_bytes / plan_ms / 1e6, 1),
             "autotuned_shape": shape, "results_identical": same,
-            "err": "per-graph dX, dP vs FP64 <= 1e-3 (tests/test_gpu_batch.py)"}
+            "err_vs_fp64": {"Y_max": ey, "dX_max": ex, "dP_max": ep,
+                            "how": "max over 8 sampled graphs, FP64 oracle gradient"}}
+
+
+def leg_l2(steps: int, shape: dict, n: int, nnz_rows, F: int) -> dict:
+    """The headline kernel with every gather L2-resident: an operator with
+    the headline's row lengths whose columns all fall in 2048 rows (a slab of
+    them is 2 MiB, well inside one XCD's 4 MiB L2), run with the headline's
+    shape.  Its time bounds what the kernel's gather stream can reach with
+    perfect locality."""
+    import numpy as np
+    import torch
+    from scl_amd import CsrMatrix, synth, torch_ops
+    lens = np.diff(nnz_rows)
+    rng = np.random.Generator(np.random.PCG64(12))
+    ip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    base = rng.integers(0, 2048, n)
+    step = 1 + 2 * rng.integers(0, 16, n)   # odd: distinct mod 2048
+    ix = np.empty(int(ip[-1]), np.int64)
+    for i in range(n):
+        ix[ip[i]:ip[i + 1]] = np.sort((base[i] + step[i] * np.arange(lens[i])) % 2048)
+    P = CsrMatrix.from_arrays(ip, ix, np.full(len(ix), 1.0 / 64, np.float32), (n, n))
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    Xd = torch.from_numpy(synth.normal(n, F, seed=12)).to(dev)
+    kw = dict(shape)
+    torch_ops.set_shape(P, F, kw.pop("vec"), kw.pop("group"), kw.pop("unroll"), **kw)
+    torch_ops.spmm(P, Xd)
+    torch.cuda.synchronize()
+    ms = timed(lambda: torch_ops.spmm(P, Xd), steps, stream)
+    return {"workload": "headline row lengths, columns within 2048 rows", "nnz": len(ix),
+            "shape": torch_ops.get_shape(P, F), "ms_per_launch": round(ms, 4)}
 
 
 def main() -> int:
@@ -535,7 +655,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--legs", default="c2,c3,c4,c5,mp,wk,cl,kn")
+    ap.add_argument("--legs", default="c2,l2,c3,c4,c5,mp,wk,cl,kn")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--shape", default=None,
                     help="vec,group,unroll,flags for the headline SpMM instead of the "
@@ -561,8 +681,13 @@ def main() -> int:
     g = synth.knn_graph(n, k, seed=seed)
     log(f"graph nnz={g.nnz} in {time.perf_counter() - t0:.1f}s; normalising (FP64, GPU)")
     A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    once = {}   # one-time costs of a new operator, in seconds
+    t0 = time.perf_counter()
+    A.prepare_device()          # upload + locality schedule
+    once["upload_and_schedule_s"] = round(time.perf_counter() - t0, 3)
+    t0 = time.perf_counter()
     deg, residual = diffusion.compute_transition_matrix_ex(A, False)
-    A.prepare_device()
+    once["normalise_and_residual_s"] = round(time.perf_counter() - t0, 3)
     log(f"stationarity residual {residual:.3e}; generating X")
     X = synth.log1p_poisson(n, F, seed=seed)
     zeros = float((X == 0).mean())
@@ -589,15 +714,31 @@ def main() -> int:
                             rows_per_group=(torch_ops.TILED if fl & (1 << 18)
                                             else (0, 4, 8, 64)[(fl >> 16) & 0x3]))
         log(f"shape set to {torch_ops.get_shape(A, F)}")
-    elif not args.no_autotune:
-        t0 = time.perf_counter()
-        tune_ms = torch_ops.autotune(A, Xd)
-        log(f"autotune {time.perf_counter() - t0:.1f}s -> {torch_ops.get_shape(A, F)}")
 
     def one():
         scl_amd.check(lib.scl_gpu_diffuse_matrix_device(
             A.handle, Xd.data_ptr(), Yd.data_ptr(), Wd.data_ptr(), n, F, t,
             stream.cuda_stream))
+
+    default = None
+    if not args.shape:
+        # the drop-in's own choice, before any tuning: the first call builds
+        # the row-grouped form of a wide operator
+        t0 = time.perf_counter()
+        one()
+        torch.cuda.synchronize()
+        once["first_call_with_form_build_s"] = round(time.perf_counter() - t0, 3)
+        one()
+        torch.cuda.synchronize()
+        d_ms = timed(one, max(2, args.steps // 2), stream) / t
+        default = {"shape": torch_ops.get_shape(A, F), "ms_per_launch": round(d_ms, 4),
+                   "GBps": round(step_bytes(n, g.nnz, F) / d_ms / 1e6, 1)}
+        log(f"default shape {default['shape']}: {d_ms:.4f} ms/launch")
+    if not args.shape and not args.no_autotune:
+        t0 = time.perf_counter()
+        tune_ms = torch_ops.autotune(A, Xd)
+        once["autotune_s"] = round(time.perf_counter() - t0, 2)
+        log(f"autotune {once['autotune_s']}s -> {torch_ops.get_shape(A, F)}")
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -610,6 +751,7 @@ def main() -> int:
     value = t * B / (ms_per_step / 1e3) / 1e9
     achieved = B / (ms_per_launch / 1e3) / 1e9
     log(f"{ms_per_step:.3f} ms/diffusion, {ms_per_launch:.4f} ms/launch, {value:.1f} GB/s")
+    shape_used = torch_ops.get_shape(A, F)
 
     # the drop-in host-pointer call (PCIe included), for reference: one call
     # to set up its buffers, then the median of three on fresh copies of X
@@ -627,9 +769,36 @@ def main() -> int:
     same = same and bool(np.array_equal(Xh, Yd.cpu().numpy()))
     del Xh
 
-    traffic = None
+    # error against FP64 and bit-identity to the CPU restatement on 8 columns
+    from oracle import oracle as orc
+    ip, ix, pv = A.export()
+    ecols = np.array([0, 1, 250, 999, 1000, 1500, 1998, 1999])
+    Yh = Yd[:, torch.from_numpy(ecols).to(dev)].cpu().numpy()
+    y64 = orc.diffuse_f64(ip, ix, pv, X[:, ecols], t)
+    err = {"max_rel_err_vs_fp64": orc.rel_err(Yh, y64),
+           "median_rel_err_vs_fp64": orc.median_rel_err(Yh, y64),
+           "bit_identical_to_cpu_restatement": bool(np.array_equal(
+               Yh, orc.diffuse_matrix_f32(ip, ix, pv, X[:, ecols], t))),
+           "columns": ecols.tolist()}
+    del y64
+
+    traffic = counters = None
     if TRAFFIC_FILE.exists():
-        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+        counters = json.loads(TRAFFIC_FILE.read_text())
+        traffic = counters.get("hbm_bytes_per_launch")
+    l2 = None
+    if "l2" in args.legs.split(","):
+        log("leg l2 (L2-resident ceiling of the headline kernel)")
+        try:
+            l2 = leg_l2(max(3, args.steps // 2), torch_ops.get_shape(A, F), n, g.indptr, F)
+            l2["bound_GBps"] = round(B / l2["ms_per_launch"] / 1e6, 1)
+            l2["frac_of_bound"] = round(achieved / l2["bound_GBps"], 4)
+            if counters and counters.get("l2_request_bytes_per_launch"):
+                l2["l2_request_bytes_per_launch"] = counters["l2_request_bytes_per_launch"]
+            torch.cuda.empty_cache()
+        except Exception as e:
+            l2 = {"error": f"{type(e).__name__}: {e}"}
+        log(f"leg l2: {l2}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -676,6 +845,10 @@ def main() -> int:
             "pct_peak": round(100 * value / PEAK_HBM_GBPS, 2),
             "stationarity_residual_fp64": residual,
             "autotuned_ms_per_launch": tune_ms,
+            "shape": shape_used,
+            "default_without_autotune": default,
+            "err": err,
+            "one_time_costs": once,
             "drop_in_host_ms": round(host_ms, 1),
             "drop_in_matches_device": same,
         },
@@ -687,6 +860,7 @@ def main() -> int:
             "frac": round(achieved / PEAK_HBM_GBPS, 4),
             "traffic": traffic,
         },
+        "l2_bound": l2,
         "cpu_baseline": cpu,
         "legs": legs,
     }

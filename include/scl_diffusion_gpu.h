@@ -206,13 +206,24 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
  *   SCL_GPU_LOWP_ROUND_INPUT  store X in 16 bits too (one rounding of
  *                             X * 2^scale_exp up front), so that the first
  *                             step also gathers 16-bit rows
- * n_features must be even; work_dev holds 2 planes of n x F 16-bit values
- * (n * F * 4 bytes) and is required when n_steps >= 2 or with
- * SCL_GPU_LOWP_ROUND_INPUT. */
+ *   SCL_GPU_LOWP_COMPENSATED  compensated bf16 (not with FP16): every stored
+ *                             value is bf16 plus an 8-bit residual of its
+ *                             rounding, in units of 1/256 of the bf16's last
+ *                             place, read back exactly as bf16 + residual
+ *                             (15 to 16 mantissa bits in 3 bytes); applies to
+ *                             the iterates, to X with ROUND_INPUT and to P's
+ *                             values with BF16_VALUES; n_features % 4 == 0
+ * n_features must be even; work_dev (required when n_steps >= 2 or with
+ * SCL_GPU_LOWP_ROUND_INPUT) holds 2 planes: scl_gpu_lowp_workspace gives its
+ * size (n * F * 4 bytes for 16-bit storage, two 256-byte aligned planes of
+ * n * F * 3 bytes compensated). */
 #define SCL_GPU_LOWP_BF16_VALUES 1
 #define SCL_GPU_LOWP_NO_SCHEDULE 2
 #define SCL_GPU_LOWP_FP16 4
 #define SCL_GPU_LOWP_ROUND_INPUT 8
+#define SCL_GPU_LOWP_COMPENSATED 16
+SCL_EXPORT scl_error_t scl_gpu_lowp_workspace(scl_index_t n_nodes, scl_index_t n_features,
+                                              int flags, scl_size_t* bytes);
 SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     void* work_dev, scl_index_t n_nodes, scl_index_t n_features,

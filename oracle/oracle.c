@@ -28,6 +28,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* threads of later parallel regions (a timing knob of the CPU baseline) */
+void orc_set_num_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* axpy_simd (diffusion.hpp:100-134): y[c] += a * x[c], one FMA per element.
  * Highway's MulAdd is a fused multiply-add on FMA targets; the scalar tail
  * `y[k] += alpha * x[k]` (131-133) is contracted to an FMA by GCC/Clang at
@@ -205,6 +214,22 @@ static inline float orc_round_fp16(float f) {
 
 float orc_fp16_round(float f) { return orc_round_fp16(f); }
 
+/* compensated bf16: hi = bf16(q) plus lo = rint((q - hi) * 2^(142 - e)) in
+ * [-127, 127] (e = hi's biased exponent; lo = 0 when e <= 30 or e == 255),
+ * read back as hi + lo * 2^(e - 142) (exact in fp32) */
+static inline float orc_round_comp(float q) {
+    const float hi = orc_round_bf16(q);
+    uint32_t u;
+    memcpy(&u, &hi, 4);
+    const int e = (int)((u >> 23) & 0xFFu);
+    if (e <= 30 || e == 255) return hi;
+    float d = rintf(ldexpf(q - hi, 142 - e));
+    d = fminf(fmaxf(d, -127.0f), 127.0f);
+    return hi + ldexpf(d, e - 142);
+}
+
+float orc_comp_round(float f) { return orc_round_comp(f); }
+
 int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
                          int bf16_values, int fp16, int scale_exp, int round_input,
@@ -216,12 +241,18 @@ int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
     float* b = (float*)malloc(total * sizeof(float));
     if (!v || !a || !b) { free(v); free(a); free(b); return -1; }
     const float up = ldexpf(1.0f, scale_exp), down = ldexpf(1.0f, -scale_exp);
-    for (int64_t k = 0; k < nnz; ++k) v[k] = bf16_values ? orc_round_bf16(vals[k]) : vals[k];
+    /* fp16: 0 bf16, 1 fp16 (scaled by 2^scale_exp), 2 compensated bf16 */
+    for (int64_t k = 0; k < nnz; ++k)
+        v[k] = !bf16_values ? vals[k]
+               : fp16 == 2  ? orc_round_comp(vals[k])
+                            : orc_round_bf16(vals[k]);
     memcpy(a, X, total * sizeof(float));
     if (round_input) {
         /* X stored as round16(X * 2^k) and read back as that * 2^-k */
-        if (fp16)
+        if (fp16 == 1)
             for (size_t e = 0; e < total; ++e) a[e] = orc_round_fp16(a[e] * up) * down;
+        else if (fp16 == 2)
+            for (size_t e = 0; e < total; ++e) a[e] = orc_round_comp(a[e]);
         else
             for (size_t e = 0; e < total; ++e) a[e] = orc_round_bf16(a[e]);
     }
@@ -229,8 +260,10 @@ int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
         orc_spmm_strided_f32(n, F, indptr, indices, v, a, b);
         if (s < t) {
             /* stored as round16(y * 2^k), read back as that * 2^-k */
-            if (fp16)
+            if (fp16 == 1)
                 for (size_t e = 0; e < total; ++e) b[e] = orc_round_fp16(b[e] * up) * down;
+            else if (fp16 == 2)
+                for (size_t e = 0; e < total; ++e) b[e] = orc_round_comp(b[e]);
             else
                 for (size_t e = 0; e < total; ++e) b[e] = orc_round_bf16(b[e]);
         }

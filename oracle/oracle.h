@@ -91,17 +91,25 @@ int orc_diffuse_bf16_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
 /* fp32 -> nearest-even bf16, returned as fp32. */
 float orc_bf16_round(float f);
 
-/* The same with fp16 iterates when fp16 != 0: each intermediate iterate is
+/* The same with fp16 iterates when fp16 == 1: each intermediate iterate is
  * stored as round_fp16(y * 2^scale_exp) and read back times 2^-scale_exp
- * (round to nearest even, overflow to inf, subnormals kept); bf16 iterates
- * otherwise (scale_exp ignored).  round_input != 0 stores X the same way
- * before the first step. */
+ * (round to nearest even, overflow to inf, subnormals kept); compensated
+ * bf16 when fp16 == 2 (orc_comp_round: values too when bf16_values); bf16
+ * iterates otherwise (scale_exp ignored).  round_input != 0 stores X the
+ * same way before the first step. */
 int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const int64_t* indices, const float* vals,
                          int bf16_values, int fp16, int scale_exp, int round_input,
                          const float* X, float* Y);
 /* fp32 -> nearest-even binary16, returned as fp32. */
 float orc_fp16_round(float f);
+/* fp32 -> compensated bf16 -> fp32: hi = bf16(q) plus an 8-bit residual
+ * rint((q - hi) 2^(142 - e)) clamped to [-127, 127] (e = hi's biased
+ * exponent; none when e <= 30 or e == 255), read back as hi + residual
+ * 2^(e - 142).  The device's compensated storage (no reference counterpart;
+ * pinned by its error bound, idempotence, symmetry and exact cases in
+ * tests/test_oracle.py). */
+float orc_comp_round(float f);
 
 /* FP64 oracle: Y = P^t X with double values/accumulators, X and Y double. */
 int orc_diffuse_f64(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
@@ -144,6 +152,8 @@ double orc_median_rel_err_f32_f64(size_t len, const float* a, const double* b);
 
 /* Thread count the restatement runs with (omp_get_max_threads). */
 int orc_num_threads(void);
+/* Sets the OpenMP thread count of later calls (n > 0). */
+void orc_set_num_threads(int n);
 
 /* ---- vector-side consumers (walks.c): diffusion.hpp:670-1194 ---------- */
 int orc_lazy_walk_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,

@@ -65,6 +65,8 @@ def load():
         "orc_diffuse_bf16_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int, P, P]),
         "orc_bf16_round": (ctypes.c_float, [ctypes.c_float]),
         "orc_fp16_round": (ctypes.c_float, [ctypes.c_float]),
+        "orc_comp_round": (ctypes.c_float, [ctypes.c_float]),
+        "orc_set_num_threads": (None, [ctypes.c_int]),
         "orc_diffuse_lowp_f32": (ctypes.c_int, [I64, I64, I64, P, P, P, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  P, P]),
@@ -158,15 +160,17 @@ def diffuse_bf16_f32(indptr, indices, vals, X, t, bf16_values=False):
 
 def diffuse_lowp_f32(indptr, indices, vals, X, t, fmt="bf16", bf16_values=False,
                      scale_exp=0, round_input=False):
-    """16-bit-iterate restatement ("bf16" or scaled "fp16"); round_input
-    stores X in the same format before the first step."""
+    """Reduced-precision-iterate restatement ("bf16", scaled "fp16" or
+    "comp", compensated bf16); round_input stores X in the same format before
+    the first step."""
     X = _f32(X)
     n, F = X.shape
     Y = np.empty_like(X)
     ip, ix, v = _i64(indptr), _i64(indices), _f32(vals)
+    code = {"bf16": 0, "fp16": 1, "comp": 2}[fmt]
     rc = load().orc_diffuse_lowp_f32(n, F, t, ip.ctypes.data, ix.ctypes.data,
                                      v.ctypes.data, 1 if bf16_values else 0,
-                                     1 if fmt == "fp16" else 0, int(scale_exp),
+                                     code, int(scale_exp),
                                      1 if round_input else 0,
                                      X.ctypes.data, Y.ctypes.data)
     if rc:
@@ -178,6 +182,13 @@ def fp16_round(a):
     """Nearest-even binary16 rounding of a float32 array via the oracle."""
     a = _f32(a)
     f = load().orc_fp16_round
+    return np.array([f(float(x)) for x in a.ravel()], np.float32).reshape(a.shape)
+
+
+def comp_round(a):
+    """Compensated-bf16 round trip of a float32 array via the oracle."""
+    a = _f32(a)
+    f = load().orc_comp_round
     return np.array([f(float(x)) for x in a.ravel()], np.float32).reshape(a.shape)
 
 
@@ -301,6 +312,10 @@ def median_rel_err(a, b) -> float:
 
 def num_threads() -> int:
     return load().orc_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    load().orc_set_num_threads(int(n))
 
 
 # ---------------------------------------------------- vector-side consumers --

@@ -96,7 +96,8 @@ T* dev_alloc(size_t count, const char* what) {
 }
 
 void dev_free(void* p) {
-    if (p) (void)hipFree(p);
+    // a failed free must not leave its error for the caller's next HIP call
+    if (p && hipFree(p) != hipSuccess) (void)hipGetLastError();
 }
 
 }  // namespace
@@ -240,6 +241,7 @@ DeviceCSR::~DeviceCSR() {
     dev_free(val);
     dev_free(sched);
     dev_free(val_bf16);
+    dev_free(val_comp.hi);  // lo points into the same allocation
     dev_free(merge.split);
     dev_free(merge.carry_row);
     dev_free(merge.carry_val);

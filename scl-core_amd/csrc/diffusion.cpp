@@ -48,13 +48,41 @@ void check_operator(const scl_sparse_matrix* P, int64_t n_nodes, int64_t F) {
 // (the form costs a host pass over the pattern).
 constexpr int64_t kGroupedMinRows = 4096;
 
+// The measured-best forward shape of wide operators (n >= kGroupedMinRows,
+// F >= 128): row groups of 4 (vec 4, 16 union entries per chunk,
+// non-temporal stores, at most 4 workgroups per CU).  At n = 200,000, nnz
+// 8.9 M, F = 2000 it is the shape the autotuner picks.
+SpmmShape wide_default_shape() {
+    SpmmShape g;
+    g.vec = 4;
+    g.group = 64;
+    g.unroll = 16;
+    g.rows_per_group = 4;
+    g.wg_per_cu = 4;
+    g.nontemporal = true;
+    return g;
+}
+
 // Launch shape for F features: the one tuned or set for this operator, when
-// it fits the buffers, else the default.  Caller holds the handle's mutex.
-SpmmShape choose_shape(DeviceCSR& d, bool transposed, int64_t F, const void* X,
-                       const void* Y) {
+// it fits the buffers; else, for wide forward operators whose rows are
+// sorted, the row-grouped shape (its grouped form is built on the first
+// such call when `build` is set, and the shape is kept for F); else the
+// default.  Caller holds the handle's mutex.
+SpmmShape choose_shape(scl_sparse_matrix* h, DeviceCSR& d, bool transposed, int64_t F,
+                       const void* X, const void* Y, bool build = true) {
     auto& m = transposed ? d.t_shapes : d.shapes;
     auto it = m.find(F);
     if (it != m.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
+    if (!transposed && it == m.end() && d.n >= kGroupedMinRows && F >= 128) {
+        const SpmmShape g = wide_default_shape();
+        const bool ready = d.grouped[0].R == 4 || (build && d.pins == 0 && ensure_grouped(h, d, 4));
+        if (ready && spmm_shape_valid(g, F, nullptr, nullptr)) {
+            if (build) m[F] = g;
+            if (spmm_shape_valid(g, F, X, Y)) return g;
+        } else if (build) {
+            m[F] = spmm_default_shape(F, nullptr, nullptr);  // not groupable: do not retry
+        }
+    }
     return spmm_default_shape(F, X, Y);
 }
 
@@ -84,7 +112,7 @@ void diffuse_host(scl_sparse_matrix* P, float* X, int64_t n, int64_t F,
     float* buf[2] = {static_cast<float*>(device_workspace(dev, 0, bytes)),
                      static_cast<float*>(device_workspace(dev, 1, bytes))};
     upload_host(buf[0], X, bytes, s, dev);
-    const SpmmShape shape = choose_shape(d, false, F, buf[0], buf[1]);
+    const SpmmShape shape = choose_shape(P, d, false, F, buf[0], buf[1]);
     const int fin = pingpong(view_of(d), shape, buf, 0, F, t, s);
     download_host(X, buf[fin], bytes, s, dev);
     hip_check(hipStreamSynchronize(s), "diffusion");
@@ -283,7 +311,7 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
             std::lock_guard<std::mutex> lk(transition->mu);
             DeviceCSR& d = device_csr(transition, dev);
             v = view_of(d);
-            sh = choose_shape(d, false, n_features, X_dev, Y_dev);
+            sh = choose_shape(transition, d, false, n_features, X_dev, Y_dev);
             if (work_dev && !spmm_shape_valid(sh, n_features, work_dev, Y_dev))
                 sh = spmm_default_shape(n_features, work_dev, Y_dev);
             if (!spmm_shape_valid(sh, n_features, X_dev, Y_dev))
@@ -332,7 +360,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_device(scl_sparse_t transition,
             } else {
                 v = view_of(d);
             }
-            sh = choose_shape(d, transpose != 0, n_features, X_dev, Y_dev);
+            sh = choose_shape(transition, d, transpose != 0, n_features, X_dev, Y_dev);
         }
         launch_spmm_f32(v, X_dev, Y_dev, n_features, sh, s);
         SCL_API_RETURN_OK;
@@ -441,7 +469,8 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
             sh.xcd_rows = false;
             sh.scalar_rows = false;
         } else {
-            sh = choose_shape(d, transpose != 0, n_features, nullptr, nullptr);
+            sh = choose_shape(transition, d, transpose != 0, n_features, nullptr, nullptr,
+                              false);
         }
         *vec = sh.vec;
         *group = sh.group;

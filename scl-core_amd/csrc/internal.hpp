@@ -95,6 +95,9 @@ struct DeviceCSR {
     // values rounded to bf16 (built on first use, for the current values)
     uint16_t* val_bf16 = nullptr;
     uint64_t bf16_generation = ~uint64_t(0);
+    // values in compensated bf16 (hi + 8-bit residual), built on first use
+    CompPlanes val_comp;
+    uint64_t comp_generation = ~uint64_t(0);
     // launch shapes chosen per feature count (autotuned or set by the
     // caller) for the operator and for its transpose
     std::map<int64_t, SpmmShape> shapes, t_shapes;

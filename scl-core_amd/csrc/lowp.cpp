@@ -19,9 +19,89 @@ void check_lowp(const scl_sparse_matrix* P, int64_t n, int64_t F) {
         fail(SCL_ERROR_INVALID_ARGUMENT, "n_features too large");
 }
 
+// Bytes of one compensated plane of m values (hi then lo), 256-aligned.
+size_t comp_plane_bytes(size_t m) { return (m * 3 + 255) / 256 * 256; }
+
+CompPlanes comp_planes(void* base, size_t m) {
+    CompPlanes c;
+    c.hi = static_cast<uint16_t*>(base);
+    c.lo = reinterpret_cast<int8_t*>(static_cast<char*>(base) + m * 2);
+    return c;
+}
+
+// Y = P^t X with compensated-bf16 storage (lowp.hip): the iterates, and X
+// with SCL_GPU_LOWP_ROUND_INPUT, and P's values with SCL_GPU_LOWP_BF16_VALUES;
+// fp32 accumulation, fp32 output.
+void diffuse_comp(scl_sparse_matrix* P, const float* X, float* Y, void* work, int64_t n,
+                  int64_t F, int64_t t, int flags, hipStream_t s) {
+    const bool round_input = (flags & SCL_GPU_LOWP_ROUND_INPUT) != 0;
+    const int dev = current_device();
+    CsrView v;
+    const CompPlanes* vals = nullptr;
+    CompPlanes vc;
+    bool sched = (flags & SCL_GPU_LOWP_NO_SCHEDULE) == 0;
+    {
+        std::lock_guard<std::mutex> lk(P->mu);
+        DeviceCSR& d = device_csr(P, dev);
+        const auto ti = d.shapes.find(F);
+        if (ti != d.shapes.end() && !ti->second.schedule) sched = false;
+        if (flags & SCL_GPU_LOWP_BF16_VALUES) {
+            if (!d.val_comp.hi) {
+                void* p = nullptr;
+                hip_check(hipMalloc(&p, comp_plane_bytes((size_t)std::max<int64_t>(d.nnz, 1))),
+                          "compensated values");
+                d.val_comp = comp_planes(p, (size_t)std::max<int64_t>(d.nnz, 1));
+                d.comp_generation = ~uint64_t(0);
+            }
+            if (d.comp_generation != d.generation) {
+                launch_encode_comp(d.val, d.val_comp, d.nnz, s);
+                d.comp_generation = d.generation;
+            }
+            vc = d.val_comp;
+            vals = &vc;
+        }
+        v = CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
+    }
+    const size_t m = (size_t)n * (size_t)F;
+    CompPlanes w[2];
+    if (work) {
+        w[0] = comp_planes(work, m);
+        w[1] = comp_planes(static_cast<char*>(work) + comp_plane_bytes(m), m);
+    }
+    if (round_input) {
+        launch_encode_comp(X, w[0], (int64_t)m, s);
+        int cur = 0;
+        for (int64_t st = 1; st < t; ++st) {
+            launch_spmm_comp(v, vals, nullptr, &w[cur], nullptr, &w[cur ^ 1], F, sched, s);
+            cur ^= 1;
+        }
+        launch_spmm_comp(v, vals, nullptr, &w[cur], Y, nullptr, F, sched, s);
+    } else if (t == 1) {
+        launch_spmm_comp(v, vals, X, nullptr, Y, nullptr, F, sched, s);
+    } else {
+        launch_spmm_comp(v, vals, X, nullptr, nullptr, &w[1], F, sched, s);
+        int cur = 1;
+        for (int64_t st = 2; st < t; ++st) {
+            launch_spmm_comp(v, vals, nullptr, &w[cur], nullptr, &w[cur ^ 1], F, sched, s);
+            cur ^= 1;
+        }
+        launch_spmm_comp(v, vals, nullptr, &w[cur], Y, nullptr, F, sched, s);
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+SCL_EXPORT scl_error_t scl_gpu_lowp_workspace(scl_index_t n_nodes, scl_index_t n_features,
+                                              int flags, scl_size_t* bytes) {
+    SCL_API_CHECK_NULL(bytes, "Output pointer is null");
+    SCL_API_CHECK(n_nodes > 0 && n_features > 0, SCL_ERROR_INVALID_ARGUMENT,
+                  "Dimensions must be positive");
+    const size_t m = (size_t)n_nodes * (size_t)n_features;
+    *bytes = (flags & SCL_GPU_LOWP_COMPENSATED) ? 2 * comp_plane_bytes(m) : m * 4;
+    SCL_API_RETURN_OK;
+}
 
 SCL_EXPORT scl_error_t scl_gpu_lowp_scale_exp(scl_sparse_t transition,
                                               const scl_real_t* X_dev,
@@ -88,9 +168,20 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_lowp_device(
     SCL_API_CHECK((const void*)X_dev != (const void*)Y_dev, SCL_ERROR_INVALID_ARGUMENT,
                   "X and Y must be distinct");
     const bool fp16 = (flags & SCL_GPU_LOWP_FP16) != 0;
+    const bool comp = (flags & SCL_GPU_LOWP_COMPENSATED) != 0;
     SCL_API_CHECK(scale_exp >= -60 && scale_exp <= 60 && (fp16 || scale_exp == 0),
                   SCL_ERROR_INVALID_ARGUMENT,
                   "scale_exp must be 0 for bf16 and within [-60, 60] for fp16");
+    SCL_API_CHECK(!comp || (!fp16 && n_features % 4 == 0), SCL_ERROR_INVALID_ARGUMENT,
+                  "compensated bf16 excludes fp16 and needs n_features % 4 == 0");
+    if (comp) {
+        SCL_API_TRY
+            check_lowp(transition, n_nodes, n_features);
+            diffuse_comp(transition, X_dev, Y_dev, work_dev, n_nodes, n_features, n_steps,
+                         flags, static_cast<hipStream_t>(stream));
+            SCL_API_RETURN_OK;
+        SCL_API_CATCH
+    }
     SCL_API_TRY
         check_lowp(transition, n_nodes, n_features);
         const int dev = current_device();

@@ -140,6 +140,7 @@ SIGNATURES = {
     "scl_gpu_diffuse_backward_values_device": (
         c_err, [P, P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
     "scl_gpu_sparse_generation": (c_err, [P, ctypes.POINTER(ctypes.c_uint64)]),
+    "scl_gpu_lowp_workspace": (c_err, [c_index, c_index, ctypes.c_int, ctypes.POINTER(c_size)]),
     "scl_gpu_diffuse_matrix_device_noalloc": (
         c_err, [P, P, P, c_index, c_index, c_index, P]),
 }

@@ -56,23 +56,30 @@ def lowp_scale_exp(P: SparseMatrixBase, X: torch.Tensor, n_steps: int) -> int:
 def diffuse_lowp(P: SparseMatrixBase, X: torch.Tensor, n_steps: int,
                  fmt: str = "bf16", bf16_values: bool = False, schedule: bool = True,
                  scale_exp=None, round_input: bool = False) -> torch.Tensor:
-    """Returns P^t X (fp32) with 16-bit intermediate iterates ("bf16" or
-    "fp16"; fp16 scaled by 2^scale_exp, computed when not given) and fp32
-    accumulation.  round_input stores X in the same 16-bit format first, so
-    every step gathers 16-bit rows.  schedule=False forces the natural row
-    order (otherwise the order follows the fp32 SpMM's tuned shape)."""
+    """Returns P^t X (fp32) with reduced-precision intermediate iterates and
+    fp32 accumulation: "bf16", "fp16" (scaled by 2^scale_exp, computed when
+    not given) or "comp" (compensated bf16: bf16 plus an 8-bit residual of
+    its rounding, 3 bytes per value).  round_input stores X in the same
+    format first, so every step gathers reduced rows; bf16_values reads P's
+    values as bf16 (compensated with "comp").  schedule=False forces the
+    natural row order (otherwise the order follows the fp32 SpMM's tuned
+    shape)."""
     _check_dense(X)
     n, F = X.shape
     fp16 = fmt == "fp16"
-    if fmt not in ("bf16", "fp16"):
-        raise ValueError("fmt must be 'bf16' or 'fp16'")
+    comp = fmt == "comp"
+    if fmt not in ("bf16", "fp16", "comp"):
+        raise ValueError("fmt must be 'bf16', 'fp16' or 'comp'")
     if scale_exp is None:
         scale_exp = lowp_scale_exp(P, X, n_steps) if fp16 else 0
     Y = torch.empty_like(X)
-    work = torch.empty((2, n, F), dtype=torch.float16 if fp16 else torch.bfloat16,
-                       device=X.device) if n_steps >= 2 or round_input else None
     flags = ((1 if bf16_values else 0) | (0 if schedule else 2) | (4 if fp16 else 0)
-             | (8 if round_input else 0))
+             | (8 if round_input else 0) | (16 if comp else 0))
+    work = None
+    if n_steps >= 2 or round_input:
+        nb = ctypes.c_size_t(0)
+        _lib.check(_lib.load().scl_gpu_lowp_workspace(n, F, flags, ctypes.byref(nb)))
+        work = torch.empty(nb.value, dtype=torch.uint8, device=X.device)
     with torch.cuda.device(X.device):
         _lib.check(_lib.load().scl_gpu_diffuse_matrix_lowp_device(
             P.handle, X.data_ptr(), Y.data_ptr(),

@@ -1,8 +1,9 @@
-"""GPU parity of the 16-bit-iterate diffusion (bf16, and fp16 with a
-power-of-two scale) against its CPU restatement (oracle orc_diffuse_lowp_f32:
-the same fp32 FMA chains in stored order, the same nearest-even rounding of
-every intermediate iterate), bit for bit, plus the error against FP64 stated
-for the uniform15 workload shape."""
+"""GPU parity of the reduced-precision diffusion (bf16, fp16 with a
+power-of-two scale, and compensated bf16: bf16 plus an 8-bit residual)
+against its CPU restatement (oracle orc_diffuse_lowp_f32: the same fp32 FMA
+chains in stored order, the same rounding of every stored value), bit for
+bit, plus the error against FP64 stated for the uniform15 workload shape.
+The precision bar is 1e-3 relative to FP64 (max-norm convention)."""
 from __future__ import annotations
 
 import numpy as np
@@ -63,6 +64,26 @@ def test_rounded_input_matches_restatement(gpu, fmt, F, t, vb):
     assert np.array_equal(Y.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("F,t,vb,ri", [(4, 1, False, False), (8, 2, True, False),
+                                       (12, 3, True, True), (64, 4, False, True),
+                                       (256, 4, True, True), (256, 3, False, False),
+                                       (300, 2, True, True), (1000, 4, True, False)])
+def test_compensated_path_matches_restatement(gpu, F, t, vb, ri):
+    """Compensated bf16 iterates (and X with round_input, P's values with
+    bf16_values): bit-identical to the restatement, for F a multiple of 8
+    (8 values per lane) and of 4 only (4 per lane), t = 1 included."""
+    n = 20_000 if F <= 256 else 5_000
+    g = synth.uniform_knn(n, 15, seed=F + t + 3).normalized()
+    X = synth.normal(n, F, seed=F + 3) * np.float32(5.0)
+    X[0, 0], X[1, 0] = np.float32(1e-35), np.float32(-3e38)   # tiny and huge
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    Y = torch_ops.diffuse_lowp(P, torch.from_numpy(X).cuda(), t, "comp", bf16_values=vb,
+                               round_input=ri)
+    ref = orc.diffuse_lowp_f32(g.indptr, g.indices, g.data, X, t, "comp", bf16_values=vb,
+                               round_input=ri)
+    assert np.array_equal(Y.cpu().numpy(), ref, equal_nan=True)
+
+
 def test_bf16_error_vs_fp64_uniform15(gpu):
     """n = 200,000 (the 1M-cell shape scaled down 5x), k = 15 uniform
     neighbours, F = 256 N(0,1), t = 4."""
@@ -76,7 +97,8 @@ def test_bf16_error_vs_fp64_uniform15(gpu):
     y64q = orc.diffuse_f64(g.indptr, g.indices, g.data, orc.bf16_round(X[:, cols]), t)
     rows = []
     for fmt, vb, ri in (("bf16", False, False), ("bf16", True, False), ("fp16", False, False),
-                        ("fp16", True, False), ("fp16", False, True)):
+                        ("fp16", True, False), ("fp16", False, True), ("comp", True, True),
+                        ("bf16", True, True)):
         Y = torch_ops.diffuse_lowp(P, Xd, t, fmt, bf16_values=vb,
                                    round_input=ri)[:, torch.from_numpy(cols).cuda()]
         Yh = Y.cpu().numpy()
@@ -92,7 +114,10 @@ def test_bf16_error_vs_fp64_uniform15(gpu):
         print(f"{fmt} iterates, {'bf16' if vb else 'fp32'} values: max {emax:.2e} median {emed:.2e}")
     print(f"bf16 iterates on bf16-rounded input vs FP64 of that input: {eq:.2e}")
     assert e32 <= 1e-5
-    assert all(emax < 1e-2 for _, _, emax, _ in rows)
-    # fp16 iterates with fp32 values meet the 1e-3 bar, also with X in fp16
+    # P values and X stored in compensated bf16 meet the 1e-3 bar with a
+    # wide margin; fp16 iterates with fp32 values meet it too, also with X
+    # in fp16; plain bf16 storage does not (its rows are reported above)
+    assert rows[5][2] <= 1e-4
     assert rows[2][2] <= 1e-3
     assert rows[4][2] <= 1e-3
+    assert rows[6][2] > 1e-3

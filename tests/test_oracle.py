@@ -354,3 +354,56 @@ def test_ulp_distance_counts_representable_steps():
                   np.float32(1e-45), np.nextafter(np.nextafter(np.float32(1), np.float32(0)),
                                                   np.float32(0))], np.float32)
     assert orc.ulp_distance(a, b).tolist() == [1, 0, 0, 1, 2]
+
+
+def test_compensated_bf16_rounding_properties():
+    """orc_comp_round (bf16 plus an 8-bit residual in 1/256 of the bf16's
+    last place).  Pinned by properties a wrong step or offset breaks: the
+    relative error is at most 2^-15 above 2^-96 and exceeds 2^-16 only where
+    the residual saturates at +-127 (under 1% of N(0,1) samples; a residual
+    step twice as large puts about a quarter of them there, half as large
+    clamps at a quarter bf16 place, 2^-10); values with 16 significant bits
+    at the residual's positions come back exactly; it is idempotent, odd and
+    monotone; below 2^-96 and for inf / nan it is plain bf16."""
+    rng = np.random.default_rng(7)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32),
+                        (rng.standard_normal(5000) * 1e20).astype(np.float32),
+                        (rng.standard_normal(5000) * 1e-20).astype(np.float32)])
+    c = orc.comp_round(x)
+    rel = np.abs(c.astype(np.float64) - x) / np.abs(x.astype(np.float64))
+    assert rel.max() <= 2.0 ** -15
+    assert np.mean(rel > 2.0 ** -16) < 0.01
+    assert np.mean(rel > 2.0 ** -17) > 0.1  # the residual is 8 bits, not more
+    assert np.array_equal(orc.comp_round(c), c)
+    assert np.array_equal(orc.comp_round(-x), -c)
+    xs = np.sort(x)
+    assert np.all(np.diff(orc.comp_round(xs).astype(np.float64)) >= 0)
+    # exact cases: hi plus k 2^-8 of hi's last place, |k| <= 127
+    hi = orc.bf16_round(rng.standard_normal(500).astype(np.float32))
+    e = (hi.view(np.uint32) >> 23) & 0xFF
+    k = rng.integers(-127, 128, 500)
+    q = (hi.astype(np.float64) + k * np.exp2(e.astype(np.float64) - 142)).astype(np.float32)
+    assert np.array_equal(orc.comp_round(q), q)
+    tiny = np.array([1e-30, -3e-32, 2.0 ** -100], np.float32)
+    assert np.array_equal(orc.comp_round(tiny), orc.bf16_round(tiny))
+    special = np.array([np.inf, -np.inf], np.float32)
+    assert np.array_equal(orc.comp_round(special), special)
+    assert np.isnan(orc.comp_round(np.array([np.nan], np.float32)))[0]
+
+
+def test_compensated_restatement_reduces_and_beats_bf16():
+    g = synth.uniform_knn(800, 9, seed=3).normalized()
+    X = synth.normal(800, 8, seed=3)
+    ip, ix, p = g.indptr, g.indices, g.data
+    # t = 2 by hand, values and input compensated
+    pc, Xc = orc.comp_round(p), orc.comp_round(X)
+    mid = orc.comp_round(orc.spmm_step_f32(ip, ix, pc, Xc))
+    exp = orc.spmm_step_f32(ip, ix, pc, mid)
+    got = orc.diffuse_lowp_f32(ip, ix, p, X, 2, "comp", bf16_values=True, round_input=True)
+    assert np.array_equal(got, exp)
+    y64 = orc.diffuse_f64(ip, ix, p, X, 4)
+    ec = orc.rel_err(orc.diffuse_lowp_f32(ip, ix, p, X, 4, "comp", bf16_values=True,
+                                          round_input=True), y64)
+    eb = orc.rel_err(orc.diffuse_lowp_f32(ip, ix, p, X, 4, "bf16", bf16_values=True,
+                                          round_input=True), y64)
+    assert ec < 2.0 ** -13 < eb
