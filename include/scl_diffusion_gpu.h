@@ -207,11 +207,12 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
  *                             X * 2^scale_exp up front), so that the first
  *                             step also gathers 16-bit rows
  *   SCL_GPU_LOWP_COMPENSATED  compensated bf16 (not with FP16): every stored
- *                             value is bf16 plus an 8-bit residual of its
- *                             rounding, in units of 1/256 of the bf16's last
- *                             place, read back exactly as bf16 + residual
- *                             (15 to 16 mantissa bits in 3 bytes); applies to
- *                             the iterates, to X with ROUND_INPUT and to P's
+ *                             value is rounded once to 24 bits (15 mantissa
+ *                             bits, nearest even) and kept as a bf16 plane
+ *                             (its upper 16 bits) plus an 8-bit plane with
+ *                             the mantissa bits bf16 drops, read back
+ *                             exactly (3 bytes per value); applies to the
+ *                             iterates, to X with ROUND_INPUT and to P's
  *                             values with BF16_VALUES; n_features % 4 == 0
  * n_features must be even; work_dev (required when n_steps >= 2 or with
  * SCL_GPU_LOWP_ROUND_INPUT) holds 2 planes: scl_gpu_lowp_workspace gives its

@@ -214,18 +214,18 @@ static inline float orc_round_fp16(float f) {
 
 float orc_fp16_round(float f) { return orc_round_fp16(f); }
 
-/* compensated bf16: hi = bf16(q) plus lo = rint((q - hi) * 2^(142 - e)) in
- * [-127, 127] (e = hi's biased exponent; lo = 0 when e <= 30 or e == 255),
- * read back as hi + lo * 2^(e - 142) (exact in fp32) */
+/* compensated bf16: q rounded once to 24 bits (round to nearest even at
+ * bit 8 of the fp32 word; nan kept quiet), stored as its upper 16 bits (a
+ * bf16) and the next 8, and read back as that 24-bit word shifted left 8 */
 static inline float orc_round_comp(float q) {
-    const float hi = orc_round_bf16(q);
     uint32_t u;
-    memcpy(&u, &hi, 4);
-    const int e = (int)((u >> 23) & 0xFFu);
-    if (e <= 30 || e == 255) return hi;
-    float d = rintf(ldexpf(q - hi, 142 - e));
-    d = fminf(fmaxf(d, -127.0f), 127.0f);
-    return hi + ldexpf(d, e - 142);
+    memcpy(&u, &q, 4);
+    const uint32_t w = ((u & 0x7fffffffu) > 0x7f800000u) ? ((u >> 8) | 0x4000u)
+                                                         : ((u + 0x7fu + ((u >> 8) & 1u)) >> 8);
+    const uint32_t r = w << 8;
+    float f;
+    memcpy(&f, &r, 4);
+    return f;
 }
 
 float orc_comp_round(float f) { return orc_round_comp(f); }

@@ -103,11 +103,11 @@ int orc_diffuse_lowp_f32(int64_t n, int64_t F, int64_t t, const int64_t* indptr,
                          const float* X, float* Y);
 /* fp32 -> nearest-even binary16, returned as fp32. */
 float orc_fp16_round(float f);
-/* fp32 -> compensated bf16 -> fp32: hi = bf16(q) plus an 8-bit residual
- * rint((q - hi) 2^(142 - e)) clamped to [-127, 127] (e = hi's biased
- * exponent; none when e <= 30 or e == 255), read back as hi + residual
- * 2^(e - 142).  The device's compensated storage (no reference counterpart;
- * pinned by its error bound, idempotence, symmetry and exact cases in
+/* fp32 -> compensated bf16 -> fp32: rounded once to 24 bits (15 mantissa
+ * bits, nearest even; nan kept quiet) and read back exactly.  The device's
+ * compensated storage (a bf16 plane of the upper 16 bits plus an 8-bit
+ * plane of the next 8; no reference counterpart; pinned by its error bound,
+ * rounding direction, idempotence, symmetry and exact cases in
  * tests/test_oracle.py). */
 float orc_comp_round(float f);
 

@@ -219,21 +219,24 @@ enum class LowpIo { F32_TO_BF16, BF16_TO_BF16, BF16_TO_F32, F32_TO_F32 };
 void launch_spmm_lowp(const CsrView& P, const uint16_t* vals_bf16, LowpIo io,
                       const void* X, void* Y, int64_t F, bool use_sched, bool fp16,
                       float in_scale, float out_scale, hipStream_t s);
-// Compensated bf16 storage of m values: hi = bf16(q) and an 8-bit residual
-// in units of 1/256 of hi's last place (lowp.hip); q is read back as
-// hi + lo * 2^(exponent(hi) - 142), 15 to 16 mantissa bits in 3 bytes.
+// Compensated bf16 storage of m values (lowp.hip): each value rounded once
+// to 24 bits (15 mantissa bits), stored as hi = its upper 16 bits (a bf16)
+// and lo = the next 8 (the mantissa bits bf16 drops); read back exactly as
+// the fp32 word hi << 16 | lo << 8.
 struct CompPlanes {
     uint16_t* hi = nullptr;  // m
-    int8_t* lo = nullptr;    // m
+    uint8_t* lo = nullptr;   // m
 };
-// dst = compensated(src), m values.
+// dst = compensated(src), m values in two planes (P's values).
 void launch_encode_comp(const float* src, const CompPlanes& dst, int64_t m, hipStream_t s);
-// One step Y = P X with fp32 accumulation; X read from Xc when set (else
-// fp32 X), Y written to Yc when set (else fp32 Y), P's values read from
-// vals when set (else P.val).  n_features % 4 == 0; plane pointers aligned
-// to 8 elements' width.
-void launch_spmm_comp(const CsrView& P, const CompPlanes* vals, const void* X,
-                      const CompPlanes* Xc, void* Y, const CompPlanes* Yc, int64_t F,
+// Packed compensated iterates: groups of 4 consecutive values in 12 bytes
+// (the 4 hi halves, then the 4 lo bytes), 3 m / 4 uint32 words.
+void launch_pack_comp(const float* src, uint32_t* dst, int64_t m, hipStream_t s);
+// One step Y = P X with fp32 accumulation; X read from the packed Xc when set
+// (else fp32 X), Y written to the packed Yc when set (else fp32 Y), P's
+// values read from vals when set (else P.val).  n_features % 4 == 0.
+void launch_spmm_comp(const CsrView& P, const CompPlanes* vals, const float* X,
+                      const uint32_t* Xc, float* Y, uint32_t* Yc, int64_t F,
                       bool use_sched, hipStream_t s);
 // partial[b] = max |x| over a grid-stride slice, for b < n_partial blocks.
 void launch_absmax(const float* x, int64_t m, float* partial, int n_partial, hipStream_t s);

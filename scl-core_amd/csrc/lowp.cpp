@@ -19,13 +19,13 @@ void check_lowp(const scl_sparse_matrix* P, int64_t n, int64_t F) {
         fail(SCL_ERROR_INVALID_ARGUMENT, "n_features too large");
 }
 
-// Bytes of one compensated plane of m values (hi then lo), 256-aligned.
+// Bytes of one compensated plane of m values (3 bytes each), 256-aligned.
 size_t comp_plane_bytes(size_t m) { return (m * 3 + 255) / 256 * 256; }
 
 CompPlanes comp_planes(void* base, size_t m) {
     CompPlanes c;
     c.hi = static_cast<uint16_t*>(base);
-    c.lo = reinterpret_cast<int8_t*>(static_cast<char*>(base) + m * 2);
+    c.lo = reinterpret_cast<uint8_t*>(static_cast<char*>(base) + m * 2);
     return c;
 }
 
@@ -35,6 +35,8 @@ CompPlanes comp_planes(void* base, size_t m) {
 void diffuse_comp(scl_sparse_matrix* P, const float* X, float* Y, void* work, int64_t n,
                   int64_t F, int64_t t, int flags, hipStream_t s) {
     const bool round_input = (flags & SCL_GPU_LOWP_ROUND_INPUT) != 0;
+    if ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y)) % 16 != 0)
+        fail(SCL_ERROR_INVALID_ARGUMENT, "compensated bf16 needs 16-byte aligned X and Y");
     const int dev = current_device();
     CsrView v;
     const CompPlanes* vals = nullptr;
@@ -63,29 +65,29 @@ void diffuse_comp(scl_sparse_matrix* P, const float* X, float* Y, void* work, in
         v = CsrView{d.n, d.n_sec, d.nnz, d.row_ptr, d.col, d.val, d.sched};
     }
     const size_t m = (size_t)n * (size_t)F;
-    CompPlanes w[2];
+    uint32_t* w[2] = {nullptr, nullptr};
     if (work) {
-        w[0] = comp_planes(work, m);
-        w[1] = comp_planes(static_cast<char*>(work) + comp_plane_bytes(m), m);
+        w[0] = static_cast<uint32_t*>(work);
+        w[1] = reinterpret_cast<uint32_t*>(static_cast<char*>(work) + comp_plane_bytes(m));
     }
     if (round_input) {
-        launch_encode_comp(X, w[0], (int64_t)m, s);
+        launch_pack_comp(X, w[0], (int64_t)m, s);
         int cur = 0;
         for (int64_t st = 1; st < t; ++st) {
-            launch_spmm_comp(v, vals, nullptr, &w[cur], nullptr, &w[cur ^ 1], F, sched, s);
+            launch_spmm_comp(v, vals, nullptr, w[cur], nullptr, w[cur ^ 1], F, sched, s);
             cur ^= 1;
         }
-        launch_spmm_comp(v, vals, nullptr, &w[cur], Y, nullptr, F, sched, s);
+        launch_spmm_comp(v, vals, nullptr, w[cur], Y, nullptr, F, sched, s);
     } else if (t == 1) {
         launch_spmm_comp(v, vals, X, nullptr, Y, nullptr, F, sched, s);
     } else {
-        launch_spmm_comp(v, vals, X, nullptr, nullptr, &w[1], F, sched, s);
+        launch_spmm_comp(v, vals, X, nullptr, nullptr, w[1], F, sched, s);
         int cur = 1;
         for (int64_t st = 2; st < t; ++st) {
-            launch_spmm_comp(v, vals, nullptr, &w[cur], nullptr, &w[cur ^ 1], F, sched, s);
+            launch_spmm_comp(v, vals, nullptr, w[cur], nullptr, w[cur ^ 1], F, sched, s);
             cur ^= 1;
         }
-        launch_spmm_comp(v, vals, nullptr, &w[cur], Y, nullptr, F, sched, s);
+        launch_spmm_comp(v, vals, nullptr, w[cur], Y, nullptr, F, sched, s);
     }
 }
 

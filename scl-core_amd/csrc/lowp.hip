@@ -229,98 +229,59 @@ __global__ __launch_bounds__(kBlock) void k_from_bf16(const uint16_t* __restrict
 }
 
 // --------------------------------------------------- compensated bf16 --
-// A value q is stored as hi = bf16(q) (round to nearest even) plus an 8-bit
-// residual lo = round((q - hi) / 2^(e - 142)), clamped to [-127, 127], where
-// e is hi's biased exponent (so 2^(e - 142) is 1/256 of hi's last place);
-// lo = 0 when e <= 30 (|hi| < 2^-96) or hi is inf/nan.  Read back as
-// hi + lo * 2^(e - 142), which is exact in fp32 (hi has 8 significant bits,
-// lo 7 more bits below them): 15 to 16 bits of mantissa in 3 bytes.
-// q - hi is exact (Sterbenz), so encode and decode are exact operations up
-// to the one rounding of the residual.
-__device__ __forceinline__ float comp_decode(uint16_t h, int8_t l) {
-    const int e = (h >> 7) & 0xFF;
-    return bf16_to_f32(h) + __builtin_ldexpf((float)l, e - 142);
+// A value q is rounded once to 24 bits (sign, 8 exponent bits, 15 mantissa
+// bits: round to nearest even at bit 8 of the fp32 word; nan stays a quiet
+// nan) and stored as two planes: hi, the upper 16 bits (a bf16), and lo, the
+// next 8 bits, which carry the mantissa bits bf16 drops.  Read back as the
+// fp32 word hi << 16 | lo << 8: exact, one byte permute per value, so a
+// gathered value costs what a bf16 one does in VALU work and 3 bytes
+// instead of 2 in memory.
+__device__ __forceinline__ float comp_decode(uint16_t h, uint8_t l) {
+    return __uint_as_float(((uint32_t)h << 16) | ((uint32_t)l << 8));
 }
 
-__device__ __forceinline__ void comp_encode(float q, uint16_t& h, int8_t& l) {
-    h = f32_to_bf16(q);
-    const int e = (h >> 7) & 0xFF;
-    int r = 0;
-    if (e > 30 && e < 255) {
-        const float d = __builtin_rintf(__builtin_ldexpf(q - bf16_to_f32(h), 142 - e));
-        r = (int)fminf(fmaxf(d, -127.0f), 127.0f);
-    }
-    l = (int8_t)r;
+__device__ __forceinline__ void comp_encode(float q, uint16_t& h, uint8_t& l) {
+    const uint32_t u = __float_as_uint(q);
+    const uint32_t w = (u & 0x7fffffffu) > 0x7f800000u
+                           ? (u >> 8) | 0x4000u
+                           : (u + 0x7fu + ((u >> 8) & 1u)) >> 8;
+    h = (uint16_t)(w >> 8);
+    l = (uint8_t)(w & 0xffu);
 }
 
-// E consecutive elements of a compensated plane pair: hi (E * 2 bytes) and
-// lo (E bytes) loads.
-template <int E>
-__device__ __forceinline__ void load_comp(const uint16_t* h, const int8_t* l, float (&x)[E]) {
-    if constexpr (E == 8) {
-        const uint4 hv = *reinterpret_cast<const uint4*>(h);
-        const uint2 lv = *reinterpret_cast<const uint2*>(l);
-        const uint32_t hw[4] = {hv.x, hv.y, hv.z, hv.w};
-        const uint32_t lw[2] = {lv.x, lv.y};
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            x[e] = comp_decode((uint16_t)(hw[e / 2] >> (16 * (e % 2))),
-                               (int8_t)(lw[e / 4] >> (8 * (e % 4))));
-    } else {
-        const uint2 hv = *reinterpret_cast<const uint2*>(h);
-        const uint32_t lv = *reinterpret_cast<const uint32_t*>(l);
-        const uint32_t hw[2] = {hv.x, hv.y};
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-            x[e] = comp_decode((uint16_t)(hw[e / 2] >> (16 * (e % 2))), (int8_t)(lv >> (8 * e)));
-    }
+// Iterates in compensated form are packed in groups of 4 consecutive
+// values (row-major): 12 bytes per group, [hi0 hi1 | hi2 hi3 | lo0..lo3], so
+// one 12-byte load brings 4 values, as one 8-byte load does for bf16.
+__device__ __forceinline__ void unpack4(const uint3 g, float (&x)[4]) {
+    x[0] = comp_decode((uint16_t)g.x, (uint8_t)g.z);
+    x[1] = comp_decode((uint16_t)(g.x >> 16), (uint8_t)(g.z >> 8));
+    x[2] = comp_decode((uint16_t)g.y, (uint8_t)(g.z >> 16));
+    x[3] = comp_decode((uint16_t)(g.y >> 16), (uint8_t)(g.z >> 24));
 }
 
-template <int E>
-__device__ __forceinline__ void store_comp(uint16_t* h, int8_t* l, const float (&a)[E]) {
-    uint16_t hs[E];
-    int8_t ls[E];
+__device__ __forceinline__ uint3 pack4(const float (&a)[4]) {
+    uint16_t h[4];
+    uint8_t l[4];
 #pragma unroll
-    for (int e = 0; e < E; ++e) comp_encode(a[e], hs[e], ls[e]);
-    uint32_t hw[E / 2], lw[E / 4];
-#pragma unroll
-    for (int q = 0; q < E / 2; ++q) hw[q] = (uint32_t)hs[2 * q] | ((uint32_t)hs[2 * q + 1] << 16);
-#pragma unroll
-    for (int q = 0; q < E / 4; ++q)
-        lw[q] = (uint32_t)(uint8_t)ls[4 * q] | ((uint32_t)(uint8_t)ls[4 * q + 1] << 8) |
-                ((uint32_t)(uint8_t)ls[4 * q + 2] << 16) | ((uint32_t)(uint8_t)ls[4 * q + 3] << 24);
-    if constexpr (E == 8) {
-        *reinterpret_cast<uint4*>(h) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-        *reinterpret_cast<uint2*>(l) = make_uint2(lw[0], lw[1]);
-    } else {
-        *reinterpret_cast<uint2*>(h) = make_uint2(hw[0], hw[1]);
-        *reinterpret_cast<uint32_t*>(l) = lw[0];
-    }
-}
-
-template <int E>
-__device__ __forceinline__ void load_f32(const float* p, float (&x)[E]) {
-#pragma unroll
-    for (int q = 0; q < E / 4; ++q) {
-        const f4 v = *reinterpret_cast<const f4*>(p + 4 * q);
-        x[4 * q + 0] = v.x;
-        x[4 * q + 1] = v.y;
-        x[4 * q + 2] = v.z;
-        x[4 * q + 3] = v.w;
-    }
+    for (int e = 0; e < 4; ++e) comp_encode(a[e], h[e], l[e]);
+    return make_uint3((uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                      (uint32_t)h[2] | ((uint32_t)h[3] << 16),
+                      (uint32_t)l[0] | ((uint32_t)l[1] << 8) | ((uint32_t)l[2] << 16) |
+                          ((uint32_t)l[3] << 24));
 }
 
 // One step with compensated input and/or output and/or values; the row
 // chain is the row-split one (fp32 FMAs in stored order) over the decoded
-// values.
-template <bool IN_C, bool OUT_C, bool VAL_C, int E, int G, int U>
+// values.  G lanes per row, 4 values per lane (G = 64: one row per wave,
+// the row's indices on the scalar unit), the (col, val) pairs of the next
+// chunk fetched while the current chunk's gathers are in flight.
+template <bool IN_C, bool OUT_C, bool VAL_C, int G, int U>
 __global__ __launch_bounds__(kBlock) void k_spmm_comp(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const uint16_t* __restrict__ vh,
-    const int8_t* __restrict__ vl, const int32_t* __restrict__ sched,
-    const float* __restrict__ X, const uint16_t* __restrict__ Xh,
-    const int8_t* __restrict__ Xl, float* __restrict__ Y, uint16_t* __restrict__ Yh,
-    int8_t* __restrict__ Yl, int32_t n, int32_t F, int32_t n_rowblocks) {
+    const uint8_t* __restrict__ vl, const int32_t* __restrict__ sched,
+    const float* __restrict__ X, const uint3* __restrict__ Xc, float* __restrict__ Y,
+    uint3* __restrict__ Yc, int32_t n, int32_t F, int32_t n_rowblocks) {
     constexpr int RPW = 64 / G;
     constexpr int RPB = (kBlock / 64) * RPW;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -333,95 +294,114 @@ __global__ __launch_bounds__(kBlock) void k_spmm_comp(
     if (rr >= n) return;
     int row = sched ? sched[rr] : rr;
     if constexpr (G == 64) row = __builtin_amdgcn_readfirstlane(row);
-    const int f = slab * (G * E) + li * E;
+    const int f = slab * (G * 4) + li * 4;
     const bool active = f < F;
     const int fo = active ? f : 0;  // lanes past F read feature 0, store nothing
     const int beg = row_ptr[row], end = row_ptr[row + 1];
+    const int F4 = F / 4;
     auto value = [&](int k) -> float {
         if constexpr (VAL_C) return comp_decode(vh[k], vl[k]);
         else return val[k];
     };
-    auto gather = [&](int j, float (&x)[E]) {
-        const size_t o = (size_t)j * F + fo;
-        if constexpr (IN_C) load_comp<E>(Xh + o, Xl + o, x);
-        else load_f32<E>(X + o, x);
+    auto gather = [&](int j, float (&x)[4]) {
+        if constexpr (IN_C) {
+            unpack4(Xc[(size_t)j * F4 + fo / 4], x);
+        } else {
+            const f4 v = *reinterpret_cast<const f4*>(X + (size_t)j * F + fo);
+            x[0] = v.x;
+            x[1] = v.y;
+            x[2] = v.z;
+            x[3] = v.w;
+        }
     };
-    float acc[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) acc[e] = 0.0f;
-    for (int k = beg; k < end; k += U) {
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (beg < end) {
         int j[U];
         float v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int kk = min(k + u, end - 1);
+            const int kk = min(beg + u, end - 1);
             j[u] = col[kk];
             v[u] = value(kk);
         }
-        float x[U][E];
+        for (int k = beg;; k += U) {
+            float x[U][4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) gather(j[u], x[u]);
+            for (int u = 0; u < U; ++u) gather(j[u], x[u]);
+            const int kn = k + U;
+            int jn[U];
+            float vn[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (k + u < end)
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(kn + u, end - 1);
+                jn[u] = col[kk];
+                vn[u] = value(kk);
+            }
 #pragma unroll
-                for (int e = 0; e < E; ++e) acc[e] = __builtin_fmaf(v[u], x[u][e], acc[e]);
+            for (int u = 0; u < U; ++u)
+                if (k + u < end)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) acc[e] = __builtin_fmaf(v[u], x[u][e], acc[e]);
+            if (kn >= end) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                j[u] = jn[u];
+                v[u] = vn[u];
+            }
+        }
     }
     if (!active) return;
-    const size_t o = (size_t)row * F + f;
-    if constexpr (OUT_C) {
-        store_comp<E>(Yh + o, Yl + o, acc);
-    } else {
-#pragma unroll
-        for (int q = 0; q < E / 4; ++q)
-            *reinterpret_cast<f4*>(Y + o + 4 * q) =
-                f4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
-    }
+    if constexpr (OUT_C) Yc[(size_t)row * F4 + f / 4] = pack4(acc);
+    else *reinterpret_cast<f4*>(Y + (size_t)row * F + f) = f4{acc[0], acc[1], acc[2], acc[3]};
 }
 
+// values: one thread each, two planes
 __global__ __launch_bounds__(kBlock) void k_encode_comp(const float* __restrict__ src,
                                                         uint16_t* __restrict__ h,
-                                                        int8_t* __restrict__ l, int64_t m) {
+                                                        uint8_t* __restrict__ l, int64_t m) {
     const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (p >= m) return;
     uint16_t hh;
-    int8_t ll;
+    uint8_t ll;
     comp_encode(src[p], hh, ll);
     h[p] = hh;
     l[p] = ll;
 }
 
-template <bool IN_C, bool OUT_C, bool VAL_C, int E>
-void launch_comp_e(const CsrView& P, const CompPlanes& vals, const void* X, const CompPlanes& Xc,
-                   void* Y, const CompPlanes& Yc, int64_t F, bool use_sched, hipStream_t s) {
+// iterates: one group of 4 per thread (m % 4 == 0, src 16-byte aligned)
+__global__ __launch_bounds__(kBlock) void k_pack_comp(const float* __restrict__ src,
+                                                      uint3* __restrict__ dst, int64_t groups) {
+    const int64_t g = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= groups) return;
+    const f4 v = *reinterpret_cast<const f4*>(src + 4 * g);
+    const float a[4] = {v.x, v.y, v.z, v.w};
+    dst[g] = pack4(a);
+}
+
+template <bool IN_C, bool OUT_C, bool VAL_C>
+void launch_comp_io(const CsrView& P, const CompPlanes& vals, const float* X,
+                    const uint32_t* Xc, float* Y, uint32_t* Yc, int64_t F, bool use_sched,
+                    hipStream_t s) {
     int G = 1;
-    while (G < (F + E - 1) / E && G < 64) G <<= 1;
+    while (G < (F + 3) / 4 && G < 64) G <<= 1;
     G = std::max(G, 16);
     const int rpb = (kBlock / 64) * (64 / G);
     const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
-    const int64_t n_slabs = (F + (int64_t)G * E - 1) / ((int64_t)G * E);
+    const int64_t n_slabs = (F + (int64_t)G * 4 - 1) / ((int64_t)G * 4);
     const int64_t nb = n_rowblocks * n_slabs;
     if (nb > INT32_MAX) throw std::runtime_error("compensated spmm grid too large");
     const int32_t* sched = use_sched ? P.sched : nullptr;
-#define SCL_COMP(GG)                                                                       \
-    hipLaunchKernelGGL((k_spmm_comp<IN_C, OUT_C, VAL_C, E, GG, 8>), dim3((unsigned)nb),    \
-                       dim3(kBlock), 0, s, P.row_ptr, P.col, P.val, vals.hi, vals.lo, sched, \
-                       static_cast<const float*>(X), Xc.hi, Xc.lo, static_cast<float*>(Y),  \
-                       Yc.hi, Yc.lo, P.n, (int32_t)F, (int32_t)n_rowblocks)
+#define SCL_COMP(GG)                                                                         \
+    hipLaunchKernelGGL((k_spmm_comp<IN_C, OUT_C, VAL_C, GG, 8>), dim3((unsigned)nb),         \
+                       dim3(kBlock), 0, s, P.row_ptr, P.col, P.val, vals.hi, vals.lo, sched,  \
+                       X, reinterpret_cast<const uint3*>(Xc), Y, reinterpret_cast<uint3*>(Yc), \
+                       P.n, (int32_t)F, (int32_t)n_rowblocks)
     switch (G) {
         case 16: SCL_COMP(16); break;
         case 32: SCL_COMP(32); break;
         default: SCL_COMP(64); break;
     }
 #undef SCL_COMP
-}
-
-template <bool IN_C, bool OUT_C, bool VAL_C>
-void launch_comp_io(const CsrView& P, const CompPlanes& vals, const void* X,
-                    const CompPlanes& Xc, void* Y, const CompPlanes& Yc, int64_t F,
-                    bool use_sched, hipStream_t s) {
-    if (F % 8 == 0) launch_comp_e<IN_C, OUT_C, VAL_C, 8>(P, vals, X, Xc, Y, Yc, F, use_sched, s);
-    else launch_comp_e<IN_C, OUT_C, VAL_C, 4>(P, vals, X, Xc, Y, Yc, F, use_sched, s);
 }
 
 constexpr int kLowpU = 8;
@@ -550,24 +530,33 @@ void launch_encode_comp(const float* src, const CompPlanes& dst, int64_t m, hipS
     check_launch("compensated encode launch");
 }
 
-void launch_spmm_comp(const CsrView& P, const CompPlanes* vals, const void* X, const CompPlanes* Xc,
-                      void* Y, const CompPlanes* Yc, int64_t F, bool use_sched, hipStream_t s) {
+void launch_pack_comp(const float* src, uint32_t* dst, int64_t m, hipStream_t s) {
+    if (m <= 0) return;
+    if (m % 4 != 0 || reinterpret_cast<uintptr_t>(src) % 16 != 0)
+        throw std::runtime_error("compensated packing needs groups of 4 aligned values");
+    const int64_t groups = m / 4;
+    hipLaunchKernelGGL(k_pack_comp, dim3((unsigned)((groups + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, src, reinterpret_cast<uint3*>(dst), groups);
+    check_launch("compensated pack launch");
+}
+
+void launch_spmm_comp(const CsrView& P, const CompPlanes* vals, const float* X,
+                      const uint32_t* Xc, float* Y, uint32_t* Yc, int64_t F, bool use_sched,
+                      hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
     if (F % 4 != 0) throw std::runtime_error("compensated storage needs n_features % 4 == 0");
     const CompPlanes none{};
     const CompPlanes& v = vals ? *vals : none;
-    const CompPlanes& xi = Xc ? *Xc : none;
-    const CompPlanes& yo = Yc ? *Yc : none;
     const int code = (Xc ? 4 : 0) | (Yc ? 2 : 0) | (vals ? 1 : 0);
     switch (code) {
-        case 0: launch_comp_io<false, false, false>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 1: launch_comp_io<false, false, true>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 2: launch_comp_io<false, true, false>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 3: launch_comp_io<false, true, true>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 4: launch_comp_io<true, false, false>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 5: launch_comp_io<true, false, true>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        case 6: launch_comp_io<true, true, false>(P, v, X, xi, Y, yo, F, use_sched, s); break;
-        default: launch_comp_io<true, true, true>(P, v, X, xi, Y, yo, F, use_sched, s); break;
+        case 0: launch_comp_io<false, false, false>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 1: launch_comp_io<false, false, true>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 2: launch_comp_io<false, true, false>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 3: launch_comp_io<false, true, true>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 4: launch_comp_io<true, false, false>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 5: launch_comp_io<true, false, true>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        case 6: launch_comp_io<true, true, false>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
+        default: launch_comp_io<true, true, true>(P, v, X, Xc, Y, Yc, F, use_sched, s); break;
     }
     check_launch("compensated spmm launch");
 }

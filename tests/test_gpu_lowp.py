@@ -67,11 +67,13 @@ def test_rounded_input_matches_restatement(gpu, fmt, F, t, vb):
 @pytest.mark.parametrize("F,t,vb,ri", [(4, 1, False, False), (8, 2, True, False),
                                        (12, 3, True, True), (64, 4, False, True),
                                        (256, 4, True, True), (256, 3, False, False),
-                                       (300, 2, True, True), (1000, 4, True, False)])
+                                       (300, 2, True, True), (1000, 4, True, False),
+                                       (272, 2, True, True), (512, 3, False, True)])
 def test_compensated_path_matches_restatement(gpu, F, t, vb, ri):
     """Compensated bf16 iterates (and X with round_input, P's values with
-    bf16_values): bit-identical to the restatement, for F a multiple of 8
-    (8 values per lane) and of 4 only (4 per lane), t = 1 included."""
+    bf16_values): bit-identical to the restatement, for 16 values per lane
+    (F a multiple of 16 from 256, 272 ending inside a slab), 8 (F a multiple
+    of 8) and 4 (F a multiple of 4 only), t = 1 included."""
     n = 20_000 if F <= 256 else 5_000
     g = synth.uniform_knn(n, 15, seed=F + t + 3).normalized()
     X = synth.normal(n, F, seed=F + 3) * np.float32(5.0)

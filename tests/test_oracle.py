@@ -357,37 +357,38 @@ def test_ulp_distance_counts_representable_steps():
 
 
 def test_compensated_bf16_rounding_properties():
-    """orc_comp_round (bf16 plus an 8-bit residual in 1/256 of the bf16's
-    last place).  Pinned by properties a wrong step or offset breaks: the
-    relative error is at most 2^-15 above 2^-96 and exceeds 2^-16 only where
-    the residual saturates at +-127 (under 1% of N(0,1) samples; a residual
-    step twice as large puts about a quarter of them there, half as large
-    clamps at a quarter bf16 place, 2^-10); values with 16 significant bits
-    at the residual's positions come back exactly; it is idempotent, odd and
-    monotone; below 2^-96 and for inf / nan it is plain bf16."""
+    """orc_comp_round: one rounding to 24 bits (15 mantissa bits, nearest
+    even).  Pinned by what a wrong bit position or direction breaks: the low
+    8 bits of the result are zero; it is the nearer of the two 24-bit
+    neighbours of x (so the error is at most 2^-16 relative, and reaches it);
+    exact halves go to the even neighbour; idempotent, odd, monotone; inf
+    stays inf, nan stays nan and the largest finite float rounds to inf."""
     rng = np.random.default_rng(7)
     x = np.concatenate([rng.standard_normal(20000).astype(np.float32),
                         (rng.standard_normal(5000) * 1e20).astype(np.float32),
-                        (rng.standard_normal(5000) * 1e-20).astype(np.float32)])
+                        (rng.standard_normal(5000) * 1e-30).astype(np.float32)])
     c = orc.comp_round(x)
-    rel = np.abs(c.astype(np.float64) - x) / np.abs(x.astype(np.float64))
-    assert rel.max() <= 2.0 ** -15
-    assert np.mean(rel > 2.0 ** -16) < 0.01
-    assert np.mean(rel > 2.0 ** -17) > 0.1  # the residual is 8 bits, not more
+    cb = c.view(np.uint32)
+    assert np.all(cb & 0xFF == 0)
+    ulp = 256.0 * np.abs(np.spacing(c.astype(np.float32))).astype(np.float64)
+    d = np.abs(c.astype(np.float64) - x)
+    assert np.all(d <= ulp / 2 * (1 + 1e-12))       # nearest 24-bit value
+    rel = d / np.abs(x.astype(np.float64))
+    normal = np.abs(x) > 1e-37
+    assert rel[normal].max() <= 2.0 ** -16 and rel[normal].max() > 2.0 ** -17
+    # exact halves between two 24-bit values go to the even one
+    lo = (rng.integers(0x3F800000, 0x40000000, 1000, dtype=np.int64) & ~0xFF).astype(np.uint32)
+    half = (lo | 0x80).view(np.float32)
+    got = orc.comp_round(half).view(np.uint32)
+    even = np.where((lo >> 8) & 1, lo + 0x100, lo).astype(np.uint32)
+    assert np.array_equal(got, even)
     assert np.array_equal(orc.comp_round(c), c)
     assert np.array_equal(orc.comp_round(-x), -c)
     xs = np.sort(x)
     assert np.all(np.diff(orc.comp_round(xs).astype(np.float64)) >= 0)
-    # exact cases: hi plus k 2^-8 of hi's last place, |k| <= 127
-    hi = orc.bf16_round(rng.standard_normal(500).astype(np.float32))
-    e = (hi.view(np.uint32) >> 23) & 0xFF
-    k = rng.integers(-127, 128, 500)
-    q = (hi.astype(np.float64) + k * np.exp2(e.astype(np.float64) - 142)).astype(np.float32)
-    assert np.array_equal(orc.comp_round(q), q)
-    tiny = np.array([1e-30, -3e-32, 2.0 ** -100], np.float32)
-    assert np.array_equal(orc.comp_round(tiny), orc.bf16_round(tiny))
-    special = np.array([np.inf, -np.inf], np.float32)
-    assert np.array_equal(orc.comp_round(special), special)
+    special = np.array([np.inf, -np.inf, np.finfo(np.float32).max], np.float32)
+    assert np.array_equal(orc.comp_round(special), np.array([np.inf, -np.inf, np.inf],
+                                                            np.float32))
     assert np.isnan(orc.comp_round(np.array([np.nan], np.float32)))[0]
 
 
