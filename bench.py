@@ -712,7 +712,8 @@ def main() -> int:
                             scalar_rows=bool(fl & 8), wg_per_cu=(fl >> 8) & 0xF,
                             merge=(fl >> 12) & 0xF,
                             rows_per_group=(torch_ops.TILED if fl & (1 << 18)
-                                            else (0, 4, 8, 64)[(fl >> 16) & 0x3]))
+                                            else (0, 4, 8, 64)[(fl >> 16) & 0x3]),
+                            xcd_block=(fl >> 20) & 0xF)
         log(f"shape set to {torch_ops.get_shape(A, F)}")
 
     def one():

@@ -140,6 +140,11 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
  *           sorted order (bit-identical).  Needs strictly increasing column
  *           indices in every row; setting it builds the staged pattern on
  *           the host (the values stay the operator's own);
+ *           SCL_GPU_SHAPE_XCD_BLOCKS(b), b in 1..8 (row-split and grouped
+ *           shapes): each XCD owns a contiguous range of rows and runs it in
+ *           blocks of 16 << b workgroup row blocks, every slab of a block
+ *           before the next block, so the X rows a block gathers stay in
+ *           that XCD's L2 while all its slabs run;
  *           SCL_GPU_SHAPE_TILED (vec 4, transpose 0): LDS tiles -- rows of
  *           the visiting order are cut into tiles whose column union fits
  *           the LDS of one workgroup; per slab of group * 4 features
@@ -167,6 +172,7 @@ SCL_EXPORT scl_error_t scl_sparse_locality_order(scl_sparse_t matrix,
 #define SCL_GPU_SHAPE_ROW_GROUPS(r) ((((r) / 4) & 0x3) << 16)
 #define SCL_GPU_SHAPE_STAGED_GROUPS (3 << 16)
 #define SCL_GPU_SHAPE_TILED (1 << 18)
+#define SCL_GPU_SHAPE_XCD_BLOCKS(b) (((b) & 0xF) << 20)
 
 SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
                                               scl_index_t n_features,

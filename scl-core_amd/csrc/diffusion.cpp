@@ -383,6 +383,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_set_shape(scl_sparse_t transition,
     sh.nontemporal = (flags & kShapeNontemporal) != 0;
     sh.schedule = (flags & kShapeNoSchedule) == 0;
     sh.xcd_rows = (flags & kShapeXcdRows) != 0;
+    sh.xcd_block = (flags >> kShapeXcdBlockShift) & kShapeXcdBlockMask;
     sh.scalar_rows = (flags & kShapeScalarRows) != 0;
     sh.wg_per_cu = (flags >> kShapeOccupancyShift) & kShapeOccupancyMask;
     sh.merge = (flags >> kShapeMergeShift) & kShapeMergeMask;
@@ -478,6 +479,7 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_get_shape(scl_sparse_t transition,
         *flags = (sh.nontemporal ? kShapeNontemporal : 0) |
                  (sh.schedule ? 0 : kShapeNoSchedule) |
                  (sh.xcd_rows ? kShapeXcdRows : 0) |
+                 (sh.xcd_block << kShapeXcdBlockShift) |
                  (sh.scalar_rows ? kShapeScalarRows : 0) |
                  (sh.wg_per_cu << kShapeOccupancyShift) |
                  (sh.merge << kShapeMergeShift) |

@@ -31,11 +31,25 @@ using namespace dev;
 namespace {
 
 // Workgroup -> (feature slab, row block).  rb_per_xcd > 0: XCD x owns row
-// blocks [x * rb_per_xcd, (x+1) * rb_per_xcd) and walks them slab by slab,
-// so all XCDs work on the same slab at a time; else the logical order is
-// slab-major and each XCD owns a contiguous run of it.
-__device__ __forceinline__ bool block_coords(int n_rowblocks, int rb_per_xcd, int& slab,
-                                             int& rb) {
+// blocks [x * rb_per_xcd, (x+1) * rb_per_xcd); with rb_per_block == 0 it
+// walks them slab by slab (all XCDs on the same slab at a time), with
+// rb_per_block > 0 it takes them rb_per_block at a time and runs every slab
+// of those before the next ones, so the X rows the block gathers stay in
+// the XCD's L2 across its rows; else the logical order is slab-major and
+// each XCD owns a contiguous run of it.
+__device__ __forceinline__ bool block_coords(int n_rowblocks, int rb_per_xcd, int rb_per_block,
+                                             int& slab, int& rb) {
+    if (rb_per_block > 0) {
+        const int x = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
+        const int n_blk = (rb_per_xcd + rb_per_block - 1) / rb_per_block;
+        const int n_slabs = (int)(gridDim.x / kNumXcd) / (n_blk * rb_per_block);
+        const int per = rb_per_block * n_slabs;
+        const int blk = idx / per, rem = idx - blk * per;
+        slab = rem / rb_per_block;
+        const int loc = blk * rb_per_block + (rem - slab * rb_per_block);
+        rb = x * rb_per_xcd + loc;
+        return loc < rb_per_xcd && rb < n_rowblocks;
+    }
     if (rb_per_xcd > 0) {
         const int x = blockIdx.x % kNumXcd, idx = blockIdx.x / kNumXcd;
         slab = idx / rb_per_xcd;
@@ -128,12 +142,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const int32_t* __restrict__ sched,
     const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
-    int32_t n_rowblocks, int32_t rb_per_xcd) {
+    int32_t n_rowblocks, int32_t rb_per_xcd, int32_t rb_per_block) {
     using VT = typename Vec<VEC>::T;
     constexpr int RPW = 64 / G;
     constexpr int RPB = (kBlock / 64) * RPW;
     int slab, rb;
-    if (!block_coords(n_rowblocks, rb_per_xcd, slab, rb)) return;
+    if (!block_coords(n_rowblocks, rb_per_xcd, rb_per_block, slab, rb)) return;
     const int lane = threadIdx.x & 63;
     // wave index is uniform across the wave: keep it in a scalar register
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -229,11 +243,11 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int32_t* __restrict__ gptr, const uint32_t* __restrict__ gword,
     const int32_t* __restrict__ gvptr, const float* __restrict__ gval,
     const int32_t* __restrict__ grow, const float* __restrict__ X, float* __restrict__ Y,
-    int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd) {
+    int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd, int32_t rb_per_block) {
     static_assert(U * R <= 128, "a chunk's values must fit two per lane");
     using VT = typename Vec<VEC>::T;
     int slab, gb;
-    if (!block_coords(n_gblocks, rb_per_xcd, slab, gb)) return;
+    if (!block_coords(n_gblocks, rb_per_xcd, rb_per_block, slab, gb)) return;
     const int lane = threadIdx.x & 63;
     const int g = gb * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= n_groups) return;
@@ -436,12 +450,12 @@ __global__ __launch_bounds__(kBlock) void k_spmm_multirow(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
     const float* __restrict__ val, const int32_t* __restrict__ sched,
     const float* __restrict__ X, float* __restrict__ Y, int32_t n, int32_t F,
-    int32_t n_rowblocks, int32_t rb_per_xcd) {
+    int32_t n_rowblocks, int32_t rb_per_xcd, int32_t rb_per_block) {
     using VT = typename Vec<VEC>::T;
     constexpr int G = 64 / RPW;
     constexpr int RPB = (kBlock / 64) * RPW;
     int slab, rb;
-    if (!block_coords(n_rowblocks, rb_per_xcd, slab, rb)) return;
+    if (!block_coords(n_rowblocks, rb_per_xcd, rb_per_block, slab, rb)) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int grp = lane / G, li = lane % G;
@@ -502,7 +516,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_multirow(
 
 using SpmmFn = void (*)(const int32_t*, const int32_t*, const float*,
                         const int32_t*, const float*, float*, int32_t, int32_t,
-                        int32_t, int32_t);
+                        int32_t, int32_t, int32_t);
 
 template <int VEC, int G, int U>
 SpmmFn pick_nt(bool nt) {
@@ -554,6 +568,27 @@ SpmmFn pick_spmm(const SpmmShape& s) {
     }
 }
 
+// Grid of a row-split or grouped launch over n_rb row blocks and n_slabs
+// slabs: rb_per_xcd (0: slab-major order), rbb (row blocks per XCD block,
+// 0: none) and the workgroup count.
+void xcd_grid(const SpmmShape& shape, int64_t n_rb, int64_t n_slabs, int64_t& rb_per_xcd,
+              int64_t& rbb, int64_t& nb) {
+    rbb = 0;
+    if (shape.xcd_block > 0) {
+        rb_per_xcd = (n_rb + kNumXcd - 1) / kNumXcd;
+        rbb = std::min<int64_t>((int64_t)16 << shape.xcd_block, rb_per_xcd);
+        const int64_t n_blk = (rb_per_xcd + rbb - 1) / rbb;
+        nb = kNumXcd * n_blk * rbb * n_slabs;
+    } else if (shape.xcd_rows) {
+        rb_per_xcd = (n_rb + kNumXcd - 1) / kNumXcd;
+        nb = kNumXcd * rb_per_xcd * n_slabs;
+    } else {
+        rb_per_xcd = 0;
+        nb = n_rb * n_slabs;
+    }
+    if (nb > INT32_MAX) throw std::runtime_error("spmm grid too large");
+}
+
 int pow2_ceil(int64_t x) {
     int p = 1;
     while (p < x && p < 64) p <<= 1;
@@ -569,6 +604,7 @@ bool spmm_shape_valid(const SpmmShape& s, int64_t F, const void* X, const void* 
     if (F % s.vec != 0 || a % (4 * (uintptr_t)s.vec) != 0) return false;
     if (s.group < 1 || s.group > 64 || (s.group & (s.group - 1)) != 0) return false;
     if (s.wg_per_cu < 0 || s.wg_per_cu > kShapeOccupancyMask) return false;
+    if (s.xcd_block < 0 || s.xcd_block > 8) return false;
     // merge path: lanes over features, one piece of the path per wave
     if (s.merge < 0 || s.merge > kMergeMax || (s.merge > 0 && s.group != 64)) return false;
     // grouped form: lanes over features, 4 or 8 rows per wave, not with merge
@@ -699,16 +735,15 @@ namespace {
 
 template <int VEC, int R, int U, bool NT>
 void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F,
-                        bool xcd_rows, int lds, hipStream_t s) {
+                        const SpmmShape& shape, int lds, hipStream_t s) {
     const int64_t n_gblocks = (G.n_groups + kBlock / 64 - 1) / (kBlock / 64);
     const int64_t n_slabs = (F + 64 * VEC - 1) / (64 * VEC);
-    const int64_t rb_per_xcd = xcd_rows ? (n_gblocks + kNumXcd - 1) / kNumXcd : 0;
-    const int64_t nb = xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : n_gblocks * n_slabs;
-    if (nb > INT32_MAX) throw std::runtime_error("grouped spmm grid too large");
+    int64_t rb_per_xcd, rbb, nb;
+    xcd_grid(shape, n_gblocks, n_slabs, rb_per_xcd, rbb, nb);
     hipLaunchKernelGGL((k_spmm_grouped<VEC, R, U, NT>), dim3((unsigned)nb), dim3(kBlock),
                        (unsigned)lds, s, G.gptr, G.gword, G.gvptr, G.gval, G.grow, X, Y,
                        (int32_t)G.n_groups, (int32_t)F, (int32_t)n_gblocks,
-                       (int32_t)rb_per_xcd);
+                       (int32_t)rb_per_xcd, (int32_t)rbb);
 }
 
 // ---------------------------------------------------- LDS-staged groups --
@@ -738,7 +773,7 @@ __global__ __launch_bounds__(64 * W) void k_spmm_staged(
     static_assert(CW <= 64 && RW * W == kStagedRows, "staged layout");
     __shared__ f4 stage[C][64];
     int slab, g;
-    if (!block_coords(n_groups, rb_per_xcd, slab, g)) return;
+    if (!block_coords(n_groups, rb_per_xcd, 0, slab, g)) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int f = slab * 256 + lane * 4;
@@ -1064,8 +1099,8 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
     // U entries per chunk with U * R <= 128 (a chunk's values fit two per lane)
 #define SCL_GROUPED(V, RR, UU)                                                          \
     (shape.nontemporal                                                                  \
-         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape.xcd_rows, lds, s)     \
-         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape.xcd_rows, lds, s))
+         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape, lds, s)              \
+         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape, lds, s))
 #define SCL_GROUPED_V(RR, UU)                          \
     do {                                               \
         if (shape.vec == 4) SCL_GROUPED(4, RR, UU);    \
@@ -1118,14 +1153,13 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     const int64_t n_rowblocks = (P.n + rpb - 1) / rpb;
     const int64_t n_slabs = (F + (int64_t)shape.group * shape.vec - 1) /
                             ((int64_t)shape.group * shape.vec);
-    const int64_t rb_per_xcd = shape.xcd_rows ? (n_rowblocks + kNumXcd - 1) / kNumXcd : 0;
-    const int64_t nb = shape.xcd_rows ? kNumXcd * rb_per_xcd * n_slabs : n_rowblocks * n_slabs;
-    if (nb > INT32_MAX) throw std::runtime_error("spmm grid too large");
+    int64_t rb_per_xcd, rbb, nb;
+    xcd_grid(shape, n_rowblocks, n_slabs, rb_per_xcd, rbb, nb);
     const int32_t* sched = shape.schedule ? P.sched : nullptr;
     hipLaunchKernelGGL(pick_spmm(shape), dim3((unsigned)nb), dim3(kBlock),
                        (unsigned)occupancy_lds_bytes(shape.wg_per_cu), s,
                        P.row_ptr, P.col, P.val, sched, X, Y, P.n, (int32_t)F,
-                       (int32_t)n_rowblocks, (int32_t)rb_per_xcd);
+                       (int32_t)n_rowblocks, (int32_t)rb_per_xcd, (int32_t)rbb);
     check_launch("spmm launch");
 }
 

@@ -120,6 +120,9 @@ struct SpmmShape {
     bool nontemporal = false;  // streaming (non-temporal) stores of Y
     bool schedule = true;      // visit rows in the operator's locality order
     bool xcd_rows = false;     // each XCD owns a row range (else a slab range)
+    int xcd_block = 0;         // b > 0: each XCD owns a row range and runs it
+                               // in blocks of 16 << b workgroup row blocks,
+                               // every slab of a block before the next block
     bool scalar_rows = false;  // group 16/32: rows' indices on the scalar unit
     int wg_per_cu = 0;  // cap on resident workgroups per CU (0: none), set by
                         // reserving LDS the kernel does not use
@@ -152,6 +155,9 @@ constexpr int kShapeGroupShift = 16;
 constexpr int kShapeGroupMask = 0x3;
 // bit 18: LDS-tiled form (see TiledCsr)
 constexpr int kShapeTiled = 1 << 18;
+// bits 20..23: XCD row blocks (0 = off, b = 1..8: 16 << b row blocks)
+constexpr int kShapeXcdBlockShift = 20;
+constexpr int kShapeXcdBlockMask = 0xF;
 // Waves a merge-path launch with code m uses for n rows and nnz nonzeros.
 int64_t merge_waves(int64_t n, int64_t nnz, int m);
 

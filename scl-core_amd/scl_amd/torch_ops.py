@@ -140,6 +140,7 @@ def get_shape(P: SparseMatrixBase, n_features: int, transpose: bool = False,
             "nontemporal": bool(f.value & 1), "schedule": not (f.value & 2),
             "xcd_rows": bool(f.value & 4), "scalar_rows": bool(f.value & 8),
             "wg_per_cu": (f.value >> 8) & 0xF, "merge": (f.value >> 12) & 0xF,
+            "xcd_block": (f.value >> 20) & 0xF,
             "rows_per_group": (TILED if f.value & (1 << 18)
                                else _ROWS_PER_GROUP[(f.value >> 16) & 0x3])}
 
@@ -148,7 +149,8 @@ def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
               unroll: int = 8, nontemporal: bool = False, schedule: bool = True,
               transpose: bool = False, xcd_rows: bool = False,
               scalar_rows: bool = False, adjoint: bool = False,
-              wg_per_cu: int = 0, merge: int = 0, rows_per_group: int = 0) -> None:
+              wg_per_cu: int = 0, merge: int = 0, rows_per_group: int = 0,
+              xcd_block: int = 0) -> None:
     """merge=m > 0 selects the merge-path SpMM with 256 << (m - 1) path items
     per wave (group 64 only); rows_per_group=4 or 8 the row-grouped form, 64
     the LDS-staged groups (vec 4, group 64), TILED (128) the LDS tiles
@@ -158,7 +160,7 @@ def set_shape(P: SparseMatrixBase, n_features: int, vec: int, group: int,
             else _ROWS_PER_GROUP.index(rows_per_group) << 16)
     flags = ((1 if nontemporal else 0) | (0 if schedule else 2) | (4 if xcd_rows else 0)
              | (8 if scalar_rows else 0) | ((wg_per_cu & 0xF) << 8) | ((merge & 0xF) << 12)
-             | code)
+             | code | ((xcd_block & 0xF) << 20))
     _lib.check(_lib.load().scl_gpu_spmm_set_shape(
         P.handle, n_features, _direction(transpose, adjoint), vec, group, unroll, flags))
 

@@ -101,3 +101,24 @@ def test_tiled_with_other_values_and_errors(gpu):
     torch_ops.set_shape(L, 8, 4, 4, 8, rows_per_group=torch_ops.TILED, wg_per_cu=1)
     with pytest.raises(Exception):
         torch_ops.set_shape(L, 8, 4, 8, 8, rows_per_group=torch_ops.TILED, wg_per_cu=2)
+
+
+@pytest.mark.parametrize("b", [1, 3, 6])
+def test_xcd_blocks_are_bit_identical(gpu, b):
+    """Row-split and row-grouped shapes with XCD row blocks (each XCD runs
+    its row range in blocks, every slab of a block before the next) give the
+    restatement's result bit for bit, with blocks larger than an XCD's range
+    included."""
+    import torch
+    g = synth.knn_graph(20_000, 12, seed=b)
+    A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
+    for F in (8, 520):
+        X = synth.normal(g.n, F, seed=F + b)
+        Xd = torch.from_numpy(X).cuda()
+        ref = orc.spmm_step_f32(g.indptr, g.indices, g.data, X)
+        for kw in (dict(rows_per_group=4), dict(rows_per_group=8), dict(), dict(group=16)):
+            group = kw.pop("group", 64)
+            torch_ops.set_shape(A, F, 4, group, 8, xcd_block=b, **kw)
+            assert torch_ops.get_shape(A, F)["xcd_block"] == b
+            Y = torch_ops.spmm(A, Xd).cpu().numpy()
+            assert np.array_equal(Y, ref), (F, b, group, kw)
