@@ -67,20 +67,91 @@ constexpr int kC = 128;   // candidates per tile
 constexpr int kK = 16;    // features per LDS stage
 constexpr int kPad = 4;   // LDS row padding (floats)
 
-// (d, j) before (d', j'): smaller distance, then smaller index
-__device__ __forceinline__ bool before(float d, int j, float d2, int j2) {
-    return d < d2 || (d == d2 && j < j2);
-}
+// The reference's k-nearest search for one query (neighbors.hpp:330-476),
+// restated for a thread that sees candidates in increasing index order:
+// Cauchy-Schwarz pruning against the current k-th distance, a max-heap of k
+// entries (sift-up while filling, replace the top only when strictly
+// closer), and at the end the heap sort (min-heap build, extraction to the
+// back, reversal).  h_d / h_j hold the heap (k entries, global memory).
+struct KnnHeap {
+    float* h_d;
+    int32_t* h_j;
+    int64_t k;
+    int64_t cnt;
+    float maxd;
+
+    __device__ void swap(int64_t a, int64_t b) {
+        const float td = h_d[a];
+        const int32_t tj = h_j[a];
+        h_d[a] = h_d[b];
+        h_j[a] = h_j[b];
+        h_d[b] = td;
+        h_j[b] = tj;
+    }
+    __device__ void offer(float d2, int32_t j) {
+        if (cnt < k) {
+            h_d[cnt] = d2;
+            h_j[cnt] = j;
+            int64_t p = cnt;
+            while (p > 0) {
+                const int64_t par = (p - 1) / 2;
+                if (h_d[p] <= h_d[par]) break;
+                swap(p, par);
+                p = par;
+            }
+            ++cnt;
+            if (cnt == k) maxd = h_d[0];
+        } else if (d2 < h_d[0]) {
+            h_d[0] = d2;
+            h_j[0] = j;
+            int64_t p = 0;
+            for (;;) {
+                int64_t big = p;
+                const int64_t l = 2 * p + 1, r = 2 * p + 2;
+                if (l < cnt && h_d[l] > h_d[big]) big = l;
+                if (r < cnt && h_d[r] > h_d[big]) big = r;
+                if (big == p) break;
+                swap(p, big);
+                p = big;
+            }
+            maxd = h_d[0];
+        }
+    }
+    __device__ void min_sift(int64_t p, int64_t size) {
+        for (;;) {
+            int64_t s = p;
+            const int64_t l = 2 * p + 1, r = 2 * p + 2;
+            if (l < size && h_d[l] < h_d[s]) s = l;
+            if (r < size && h_d[r] < h_d[s]) s = r;
+            if (s == p) break;
+            swap(p, s);
+            p = s;
+        }
+    }
+    // ascending order of the reference's heap sort
+    __device__ void sort() {
+        if (cnt >= 2)
+            for (int64_t m = cnt / 2; m > 0; --m) min_sift(m - 1, cnt);
+        for (int64_t m = cnt; m > 1; --m) {
+            swap(0, m - 1);
+            min_sift(0, m - 1);
+        }
+        for (int64_t m = 0; m < cnt / 2; ++m) swap(m, cnt - 1 - m);
+    }
+};
 
 // One workgroup owns kQ query rows of the dense, zero-padded n_pad x f_pad
 // matrix X (n_pad a multiple of kC) and scans every candidate tile.  Thread
 // (ty, tx) computes the 4 x 8 dot products of queries 4ty..4ty+3 with
 // candidates 4tx..4tx+3 and 64+4tx..64+4tx+3, each a sequential FMA chain
-// over the features.  Candidates that beat a query's current k-th entry (in
-// (distance, index) order) go to an LDS buffer; after each tile one thread
-// per query sorts its buffer and merges it into the query's sorted list in
-// ld / lj (k entries per query, global memory).  At the end the lists are
-// written out as sqrt(distance) and index, padded with inf / -1.
+// over the features.  A candidate can change a query's heap only while the
+// heap is filling or when it is strictly closer than the heap's top at the
+// start of the tile (the top never grows), so only those go to an LDS
+// buffer; after each tile one thread per query sorts its buffer by index
+// and offers the candidates to the query's heap in that order, with the
+// reference's pruning test against the heap's current top.  The heap then
+// evolves exactly as in the reference's scan over j = 0..n-1, and the final
+// heap sort gives its output order, ties included.
 __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
                                                 const float* __restrict__ norms, int64_t n,
                                                 int64_t f_pad, int64_t k,
@@ -91,17 +162,17 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
     __shared__ float Bs[kK][kC + kPad];
     __shared__ float bd[kQ][kC];
     __shared__ int bj[kQ][kC];
-    __shared__ int cnt[kQ], len[kQ], thr_j[kQ];
-    __shared__ float thr_d[kQ], qn[kQ];
+    __shared__ int cnt[kQ];
+    __shared__ long long hcnt[kQ];
+    __shared__ float top[kQ], qn[kQ];
 
     const int tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     const int64_t q0 = (int64_t)blockIdx.x * kQ;
     if (tid < kQ) {
         cnt[tid] = 0;
-        len[tid] = 0;
-        thr_d[tid] = __builtin_inff();
-        thr_j[tid] = 0x7fffffff;
+        hcnt[tid] = 0;
+        top[tid] = 3.40282347e38f;   // the reference's max_dist_sq before the heap fills
         qn[tid] = q0 + tid < n ? norms[q0 + tid] : 0.0f;
     }
     // loader mapping: thread tid moves 4 consecutive features of one query
@@ -146,19 +217,21 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
             }
             __syncthreads();
         }
-        // distances of this tile; candidates ahead of the k-th go to the buffer
+        // distances of this tile; candidates that can still enter the heap
+        // go to the buffer
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int ql = ty * 4 + r;
             const int64_t q = q0 + ql;
             if (q >= n) continue;
+            const bool filling = hcnt[ql] < k;
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int64_t j = c0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
                 if (j >= n || j == q) continue;
                 float d2 = __builtin_fmaf(-2.0f, acc[r][c], qn[ql] + norms[j]);
                 if (d2 < 0.0f) d2 = 0.0f;
-                if (before(d2, (int)j, thr_d[ql], thr_j[ql])) {
+                if (filling || d2 < top[ql]) {
                     const int slot = atomicAdd(&cnt[ql], 1);
                     bd[ql][slot] = d2;
                     bj[ql][slot] = (int)j;
@@ -169,12 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
         if (tid < kQ && cnt[tid] > 0) {
             const int ql = tid;
             const int m = cnt[ql];
-            // insertion sort of the buffer
+            // insertion sort of the buffer by candidate index
             for (int a = 1; a < m; ++a) {
                 const float kd = bd[ql][a];
                 const int kj = bj[ql][a];
                 int b = a - 1;
-                while (b >= 0 && before(kd, kj, bd[ql][b], bj[ql][b])) {
+                while (b >= 0 && bj[ql][b] > kj) {
                     bd[ql][b + 1] = bd[ql][b];
                     bj[ql][b + 1] = bj[ql][b];
                     --b;
@@ -182,43 +255,27 @@ __global__ __launch_bounds__(kBlock) void k_knn(const float* __restrict__ X,
                 bd[ql][b + 1] = kd;
                 bj[ql][b + 1] = kj;
             }
-            // merge from the back into the list, keeping the first k
-            float* Ld = ld + (q0 + ql) * k;
-            int32_t* Lj = lj + (q0 + ql) * k;
-            const int64_t L = len[ql];
-            const int64_t total = L + m, keep = total < k ? total : k;
-            int64_t i = L - 1;
-            int b = m - 1;
-            for (int64_t t = 0; t < total - keep; ++t) {
-                if (b < 0 || (i >= 0 && before(bd[ql][b], bj[ql][b], Ld[i], Lj[i]))) --i;
-                else --b;
+            KnnHeap h{ld + (q0 + ql) * k, lj + (q0 + ql) * k, k, (int64_t)hcnt[ql], top[ql]};
+            const float si = sqrtf(qn[ql]);
+            for (int a = 0; a < m; ++a) {
+                const float df = si - sqrtf(norms[bj[ql][a]]);
+                if (df * df >= h.maxd) continue;   // neighbors.hpp:420-425
+                h.offer(bd[ql][a], bj[ql][a]);
             }
-            for (int64_t pos = keep - 1; pos >= 0; --pos) {
-                if (b < 0 || (i >= 0 && before(bd[ql][b], bj[ql][b], Ld[i], Lj[i]))) {
-                    Ld[pos] = Ld[i];
-                    Lj[pos] = Lj[i];
-                    --i;
-                } else {
-                    Ld[pos] = bd[ql][b];
-                    Lj[pos] = bj[ql][b];
-                    --b;
-                }
-            }
-            len[ql] = (int)keep;
-            if (keep == k) {
-                thr_d[ql] = Ld[k - 1];
-                thr_j[ql] = Lj[k - 1];
-            }
+            hcnt[ql] = h.cnt;
+            top[ql] = h.maxd;
             cnt[ql] = 0;
         }
         __syncthreads();
     }
     if (tid < kQ && q0 + tid < n) {
         const int64_t q = q0 + tid;
+        KnnHeap h{ld + q * k, lj + q * k, k, (int64_t)hcnt[tid], top[tid]};
+        h.sort();
         for (int64_t m = 0; m < k; ++m) {
-            if (m < len[tid]) {
-                out_dist[q * k + m] = sqrtf(ld[q * k + m]);
-                out_idx[q * k + m] = lj[q * k + m];
+            if (m < h.cnt) {
+                out_dist[q * k + m] = sqrtf(h.h_d[m]);
+                out_idx[q * k + m] = h.h_j[m];
             } else {
                 out_dist[q * k + m] = __builtin_inff();
                 out_idx[q * k + m] = -1;

@@ -1,9 +1,10 @@
 """GPU parity of the exact k-nearest-neighbour search (scl_knn,
 scl_knn_simple, scl_neighbors_compute_norms) against the CPU restatement of
-the reference's heap search (oracle/neighbors.c).  Norms and distances are
-bit-identical; the GPU lists are sorted by (distance, index), so the
-restatement's lists are compared after the same sort (the reference's heap
-orders equal distances its own way)."""
+the reference's heap search (oracle/neighbors.c).  Norms, distances and
+index lists are bit-identical as returned: the GPU offers candidates to each
+query's heap in index order with the reference's pruning test, heap steps
+and final heap sort, so equal distances come out in the reference's order
+and ties at the k-th distance keep the reference's members."""
 from __future__ import annotations
 
 import numpy as np
@@ -27,14 +28,6 @@ def samples(n, f, density, seed):
     return np.array(ip, np.int64), np.array(ix, np.int64), v
 
 
-def by_distance_then_index(idx, dist):
-    out_i, out_d = idx.copy(), dist.copy()
-    for r in range(idx.shape[0]):
-        o = np.lexsort((idx[r], dist[r]))
-        out_i[r], out_d[r] = idx[r, o], dist[r, o]
-    return out_i, out_d
-
-
 @pytest.mark.parametrize("n,f,k", [(3000, 150, 15), (1000, 37, 1), (777, 64, 70),
                                    (130, 20, 200)])
 def test_knn_matches_restatement(gpu, n, f, k):
@@ -43,7 +36,6 @@ def test_knn_matches_restatement(gpu, n, f, k):
     norms = construct.neighbor_norms(M)
     assert np.array_equal(norms, orc.norms_f32(ip, v))
     ri, rd = orc.knn_f32(ip, ix, v, k)
-    ri, rd = by_distance_then_index(ri, rd)
     for gi, gd in (construct.knn(M, k), construct.knn(M, k, norms)):
         assert np.array_equal(gd, rd), (n, f, k)
         assert np.array_equal(gi, ri), (n, f, k)
@@ -56,16 +48,42 @@ def test_knn_samples_of_a_csc_handle_and_ties(gpu):
     ip, ix, v = samples(n, f, 0.2, seed=5)
     C = CscMatrix.from_arrays(ip, ix, v, (f, n))     # samples are the columns
     gi, gd = construct.knn(C, 12)
-    ri, rd = by_distance_then_index(*orc.knn_f32(ip, ix, v, 12))
+    ri, rd = orc.knn_f32(ip, ix, v, 12)
     assert np.array_equal(gd, rd) and np.array_equal(gi, ri)
-    # four identical samples: zero distances among them, nearest first
+    # four identical samples: zero distances among them, in the reference's
+    # heap order
     ip = np.array([0, 2, 4, 6, 8, 11], np.int64)
     ix = np.array([0, 2, 0, 2, 0, 2, 0, 2, 0, 1, 2], np.int64)
     v = np.array([1, 2, 1, 2, 1, 2, 1, 2, 9, 9, 9], np.float32)
     gi, gd = construct.knn(CsrMatrix.from_arrays(ip, ix, v, (5, 3)), 3)
-    assert list(gi[0]) == [1, 2, 3] and np.all(gd[:4] == 0)
-    assert list(gi[4]) == [0, 1, 2]
+    ri, rd = orc.knn_f32(ip, ix, v, 3)
+    assert np.array_equal(gi, ri) and np.array_equal(gd, rd)
+    assert sorted(gi[0]) == [1, 2, 3] and np.all(gd[:4] == 0)
     assert np.allclose(gd[4], np.sqrt(64 + 81 + 49), rtol=1e-6)
+
+
+@pytest.mark.parametrize("copies,k", [(10, 7), (10, 15), (37, 20), (3, 4), (200, 130)])
+def test_knn_ties_at_the_kth_distance_keep_reference_members(gpu, copies, k):
+    """Samples that are copies of a few distinct vectors (and integer-valued
+    ones whose distances repeat), so that many candidates share the k-th
+    distance: the members kept and their order are the reference heap's, as
+    returned, index for index, across several candidate tiles."""
+    rng = np.random.Generator(np.random.PCG64(copies + k))
+    base = (rng.integers(0, 3, (6, 12)) * (rng.random((6, 12)) < 0.5)).astype(np.float32)
+    dense = np.repeat(base, copies, axis=0)
+    dense = dense[rng.permutation(len(dense))]
+    n = len(dense)
+    ip = np.concatenate([[0], np.cumsum((dense != 0).sum(1))]).astype(np.int64)
+    ix = np.concatenate([np.flatnonzero(r) for r in dense]).astype(np.int64)
+    v = dense[dense != 0].astype(np.float32)
+    M = CsrMatrix.from_arrays(ip, ix, v, (n, 12))
+    ri, rd = orc.knn_f32(ip, ix, v, k)
+    gi, gd = construct.knn(M, k)
+    assert np.array_equal(gd, rd)
+    assert np.array_equal(gi, ri)
+    # the case is what it claims: ties straddle the k-th place
+    _, rd1 = orc.knn_f32(ip, ix, v, k + 1)
+    assert np.any(rd1[:, k - 1] == rd1[:, k])
 
 
 def test_knn_argument_errors(gpu):
@@ -97,12 +115,15 @@ def test_neighbors_reference_cases(gpu):
                               np.zeros(0, np.float32), (0, 10))
     assert e.value.code == 10
     # 166-205 and 348-385: the unit square with explicit zeros stored, k = 2:
-    # the two axis neighbours at distance 1, smaller index first
+    # the two axis neighbours at distance 1, in the reference heap's order
+    # (by hand for point 1: j = 0 enters, j = 2 (d^2 = 2) enters and rises to
+    # the top, j = 3 (d^2 = 1) replaces it; the heap sort of [(1, 3), (1, 0)]
+    # then returns 3 before 0)
     sq = CsrMatrix.from_arrays(np.array([0, 2, 4, 6, 8], np.int64),
                                np.array([0, 1, 0, 1, 0, 1, 0, 1], np.int64),
                                np.array([0, 0, 1, 0, 0, 1, 1, 1], np.float32), (4, 2))
     for gi, gd in (construct.knn(sq, 2), construct.knn(sq, 2, construct.neighbor_norms(sq))):
-        assert gi.tolist() == [[1, 2], [0, 3], [0, 3], [1, 2]]
+        assert gi.tolist() == [[1, 2], [3, 0], [3, 0], [2, 1]]
         assert np.all(gd == 1.0)
     # 208-244, 246-268 and 387-425 (seeds 42, 123, 456): 10-30 samples of 5-20
     # features, density 0.05-0.3, k = 1..5; knn with norms equals knn_simple
@@ -122,7 +143,7 @@ def test_neighbors_reference_cases(gpu):
             assert np.array_equal(si, wi) and np.array_equal(sd, wd)
             assert np.all(sd >= 0) and np.all((si >= 0) & (si < n))
             assert np.all(np.diff(sd, axis=1) >= 0)
-            ri, rd = by_distance_then_index(*orc.knn_f32(ip, ix, v, k))
+            ri, rd = orc.knn_f32(ip, ix, v, k)
             assert np.array_equal(si, ri) and np.array_equal(sd, rd), (seed, k)
     # 320-346: k larger than the other samples: the rest is -1 / inf
     gi, gd = construct.knn(A, n + 5)

@@ -77,3 +77,18 @@ def test_knn_ties_padding_and_duplicates():
     ip2 = np.array([0, 0, 2, 4], np.int64)
     idx, dist = orc.knn_f32(ip2, ix[:4], v[:4], 2)
     assert np.allclose(dist[0], np.sqrt(5.0), rtol=1e-6)
+
+
+def test_knn_heap_order_on_the_unit_square_by_hand():
+    """The reference's heap (neighbors.hpp:330-476) traced by hand on the
+    unit square (reference test_neighbors.cpp:166-205), k = 2.  Point 1 =
+    (1, 0): j = 0 enters (d^2 = 1); j = 2 (d^2 = 2) enters and sifts up to the
+    top; j = 3 (d^2 = 1 < 2) replaces the top and stays; the heap sort of
+    [(1, 3), (1, 0)] (min-heap build, extract, reverse) returns 3 before 0.
+    Point 0 fills with 1 and 2 (equal keys never swap) and rejects 3."""
+    ip = np.array([0, 2, 4, 6, 8], np.int64)
+    ix = np.array([0, 1, 0, 1, 0, 1, 0, 1], np.int64)
+    v = np.array([0, 0, 1, 0, 0, 1, 1, 1], np.float32)
+    gi, gd = orc.knn_f32(ip, ix, v, 2)
+    assert gi.tolist() == [[1, 2], [3, 0], [3, 0], [2, 1]]
+    assert np.all(gd == 1.0)
