@@ -253,6 +253,95 @@ int orc_pseudotime_f32(int64_t n, const int64_t* ip, const int64_t* ix, const fl
     return 0;
 }
 
+/* ------------------------------------------------------------ FP64 anchor --
+ * orc_pseudotime_f32 in FP64 (double orthonormalisation and SpMM) from the
+ * same fp32 FastRNG(42) start: the exact-arithmetic counterpart that FP32
+ * implementations are measured against. */
+static void pt_orthonormalise_f64(double* Q, int64_t n, int64_t K, double* w) {
+    for (int64_t c = 0; c < K; ++c) {
+        for (int64_t i = 0; i < n; ++i) w[i] = Q[i * K + c];
+        for (int64_t p = 0; p < c; ++p) {
+            double dot = 0.0, nrm = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                const double q = Q[i * K + p];
+                dot += w[i] * q;
+                nrm += q * q;
+            }
+            if (nrm > 1e-15) {
+                const double coeff = dot / nrm;
+                for (int64_t i = 0; i < n; ++i) w[i] -= coeff * Q[i * K + p];
+            }
+        }
+        double nrm = 0.0;
+        for (int64_t i = 0; i < n; ++i) nrm += w[i] * w[i];
+        if (nrm > 1e-15) {
+            const double inv = 1.0 / sqrt(nrm);
+            for (int64_t i = 0; i < n; ++i) Q[i * K + c] = w[i] * inv;
+        } else {
+            orc_rng r;
+            rng_seed(&r, (uint64_t)c * 12345u, NULL);
+            nrm = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                w[i] = (double)(rng_uniform(&r) - 0.5f);
+                nrm += w[i] * w[i];
+            }
+            const double inv = 1.0 / sqrt(nrm);
+            for (int64_t i = 0; i < n; ++i) Q[i * K + c] = w[i] * inv;
+        }
+    }
+}
+
+int orc_pseudotime_f64(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                       int64_t root, int64_t K, int64_t n_iter, double* pt) {
+    if (n == 0) return 0;
+    const size_t sz = (size_t)n * (size_t)(K > 0 ? K : 1);
+    double* Q = (double*)malloc(sz * sizeof(double));
+    double* Qn = (double*)malloc(sz * sizeof(double));
+    double* w = (double*)malloc((size_t)n * sizeof(double));
+    if (!Q || !Qn || !w) {
+        free(Q);
+        free(Qn);
+        free(w);
+        return -1;
+    }
+    orc_rng r;
+    rng_seed(&r, 42, NULL);
+    for (int64_t i = 0; i < n * K; ++i) Q[i] = (double)(rng_uniform(&r) - 0.5f);
+    pt_orthonormalise_f64(Q, n, K, w);
+    for (int64_t it = 0; it < n_iter; ++it) {
+#pragma omp parallel for schedule(static) if (n >= 256)
+        for (int64_t i = 0; i < n; ++i) {
+            double* y = Qn + i * K;
+            for (int64_t c = 0; c < K; ++c) y[c] = 0.0;
+            for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+                const double a = (double)v[k];
+                for (int64_t c = 0; c < K; ++c) y[c] += a * Q[ix[k] * K + c];
+            }
+        }
+        pt_orthonormalise_f64(Qn, n, K, w);
+        double* s = Q;
+        Q = Qn;
+        Qn = s;
+    }
+    const double* rq = Q + root * K;
+    for (int64_t i = 0; i < n; ++i) {
+        double d2 = 0.0;
+        for (int64_t c = 0; c < K; ++c) {
+            const double d = Q[i * K + c] - rq[c];
+            d2 += d * d;
+        }
+        pt[i] = sqrt(d2);
+    }
+    double mx = pt[0];
+    for (int64_t i = 1; i < n; ++i) mx = pt[i] > mx ? pt[i] : mx;
+    if (mx > 0.0)
+        for (int64_t i = 0; i < n; ++i) pt[i] /= mx;
+    free(Q);
+    free(Qn);
+    free(w);
+    return 0;
+}
+
 /* algebra::spmm (algebra.hpp:546-609): Y (primary x F) = beta Y first
  * (scale_output 59-: beta 0 zeroes, beta 1 keeps, else multiplies); nothing
  * more when alpha == 0; else for each row and each stored entry in order,

@@ -170,6 +170,9 @@ int orc_dpt_multi_f32(int64_t n, const int64_t* ip, const int64_t* ix, const flo
 void orc_dmap_init_f32(int64_t n, int64_t K, float* Q);
 int orc_dmap_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
                  float* emb, int64_t K, int64_t n_iter);
+/* The same in FP64 from the same fp32 start vectors (an error anchor). */
+int orc_dmap_f64(int64_t n, const int64_t* ip, const int64_t* ix, const float* v, double* emb,
+                 int64_t K, int64_t n_iter);
 int orc_mgs_f32(int64_t n, int64_t K, float* Q);
 int orc_diffusion_distance_f32(int64_t n, const int64_t* ip, const int64_t* ix,
                                const float* v, float* D, int64_t n_steps);
@@ -194,6 +197,9 @@ int orc_pt_orthonormalise_f32(int64_t n, int64_t K, float* Q);
 /* diffusion_pseudotime (pseudotime.hpp:646-712). */
 int orc_pseudotime_f32(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
                        int64_t root, int64_t K, int64_t n_iter, float* pt);
+/* The same in FP64 from the same fp32 start (an error anchor). */
+int orc_pseudotime_f64(int64_t n, const int64_t* ip, const int64_t* ix, const float* v,
+                       int64_t root, int64_t K, int64_t n_iter, double* pt);
 /* algebra::spmm, Y = alpha A X + beta Y (algebra.hpp:546-609); A has m rows. */
 void orc_algebra_spmm_f32(int64_t m, int64_t F, const int64_t* ip, const int64_t* ix,
                           const float* v, const float* X, float* Y, float alpha,

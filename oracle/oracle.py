@@ -77,6 +77,8 @@ def load():
         "orc_dpt_multi_f32": (ctypes.c_int, [I64, P, P, P, P, I64, P, I64]),
         "orc_dmap_init_f32": (None, [I64, I64, P]),
         "orc_dmap_f32": (ctypes.c_int, [I64, P, P, P, P, I64, I64]),
+        "orc_dmap_f64": (ctypes.c_int, [I64, P, P, P, P, I64, I64]),
+        "orc_pseudotime_f64": (ctypes.c_int, [I64, P, P, P, I64, I64, I64, P]),
         "orc_mgs_f32": (ctypes.c_int, [I64, I64, P]),
         "orc_diffusion_distance_f32": (ctypes.c_int, [I64, P, P, P, P, I64]),
         "orc_densify_f32": (None, [I64, I64, P, P, P, ctypes.c_int, P]),
@@ -379,6 +381,17 @@ def dmap_init_f32(n, K):
     return Q
 
 
+def dmap_f64(indptr, indices, vals, K, n_iter):
+    """dmap_f32's steps in FP64 from the same fp32 start vectors (an anchor)."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    out = np.empty((n, K), np.float64)
+    if load().orc_dmap_f64(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, out.ctypes.data,
+                           K, n_iter):
+        raise MemoryError
+    return out
+
+
 def dmap_f32(indptr, indices, vals, K, n_iter):
     ip, ix, v = _csr(indptr, indices, vals)
     n = len(ip) - 1
@@ -463,6 +476,17 @@ def pt_orthonormalise_f32(Q):
     if load().orc_pt_orthonormalise_f32(Q.shape[0], Q.shape[1], Q.ctypes.data):
         raise MemoryError
     return Q
+
+
+def pseudotime_f64(indptr, indices, vals, root, n_dcs, n_iter=100):
+    """pseudotime_f32's steps in FP64 from the same fp32 start (an anchor)."""
+    ip, ix, v = _csr(indptr, indices, vals)
+    n = len(ip) - 1
+    out = np.empty(n, np.float64)
+    if load().orc_pseudotime_f64(n, ip.ctypes.data, ix.ctypes.data, v.ctypes.data, root,
+                                 n_dcs, n_iter, out.ctypes.data):
+        raise MemoryError
+    return out
 
 
 def pseudotime_f32(indptr, indices, vals, root, n_dcs, n_iter=100):

@@ -309,3 +309,84 @@ int orc_diffusion_distance_f32(int64_t n, const int64_t* ip, const int64_t* ix,
     free(A); free(B);
     return 0;
 }
+
+/* ------------------------------------------------------------ FP64 anchors --
+ * The diffusion map of orc_dmap_f32 in FP64 (double accumulators, double
+ * SpMM), from the same fp32 start vectors: the exact-arithmetic counterpart
+ * that FP32 implementations are measured against. */
+static void spmm_step_f64(int64_t n, int64_t F, const int64_t* ip, const int64_t* ix,
+                          const float* v, const double* X, double* Y) {
+#pragma omp parallel for schedule(static) if (n >= 256)
+    for (int64_t i = 0; i < n; ++i) {
+        double* y = Y + i * F;
+        for (int64_t c = 0; c < F; ++c) y[c] = 0.0;
+        for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+            const double a = (double)v[k];
+            const double* x = X + ix[k] * F;
+            for (int64_t c = 0; c < F; ++c) y[c] += a * x[c];
+        }
+    }
+}
+
+static void mgs_f64(double* Q, int64_t n, int64_t K, double* w) {
+    for (int64_t c = 0; c < K; ++c) {
+        for (int64_t i = 0; i < n; ++i) w[i] = Q[i * K + c];
+        for (int reorth = 0; reorth <= 2; ++reorth) {
+            for (int64_t p = 0; p < c; ++p) {
+                double dot = 0.0, nrm = 0.0;
+                for (int64_t i = 0; i < n; ++i) {
+                    const double q = Q[i * K + p];
+                    dot += w[i] * q;
+                    nrm += q * q;
+                }
+                if (nrm > 1e-15) {
+                    const double coeff = dot / nrm;
+                    for (int64_t i = 0; i < n; ++i) w[i] -= coeff * Q[i * K + p];
+                }
+            }
+            double nv = 0.0;
+            for (int64_t i = 0; i < n; ++i) nv += w[i] * w[i];
+            if (reorth == 0 && c > 0) {
+                double max_cos = 0.0;
+                for (int64_t p = 0; p < c; ++p) {
+                    double dot = 0.0;
+                    for (int64_t i = 0; i < n; ++i) dot += w[i] * Q[i * K + p];
+                    const double cs = nv > 1e-15 ? fabs(dot) / sqrt(nv) : 0.0;
+                    max_cos = max_cos > cs ? max_cos : cs;
+                }
+                if (max_cos < 0.7) break;
+            } else {
+                break;
+            }
+        }
+        double nrm = 0.0;
+        for (int64_t i = 0; i < n; ++i) nrm += w[i] * w[i];
+        if (nrm > 1e-15) {
+            const double inv = 1.0 / sqrt(nrm);
+            for (int64_t i = 0; i < n; ++i) Q[i * K + c] = w[i] * inv;
+        } else {
+            for (int64_t i = 0; i < n; ++i) Q[i * K + c] = (i == c % n) ? 1.0 : 0.0;
+        }
+    }
+}
+
+int orc_dmap_f64(int64_t n, const int64_t* ip, const int64_t* ix, const float* v, double* emb,
+                 int64_t K, int64_t n_iter) {
+    if (n == 0 || K == 0) return 0;
+    float* q32 = (float*)malloc((size_t)(n * K) * sizeof(float));
+    double* Q = (double*)malloc((size_t)(n * K) * sizeof(double));
+    double* Qn = (double*)malloc((size_t)(n * K) * sizeof(double));
+    double* w = (double*)malloc((size_t)n * sizeof(double));
+    if (!q32 || !Q || !Qn || !w) { free(q32); free(Q); free(Qn); free(w); return -1; }
+    orc_dmap_init_f32(n, K, q32);
+    for (int64_t e = 0; e < n * K; ++e) Q[e] = (double)q32[e];
+    mgs_f64(Q, n, K, w);
+    for (int64_t it = 0; it < n_iter; ++it) {
+        spmm_step_f64(n, K, ip, ix, v, Q, Qn);
+        mgs_f64(Qn, n, K, w);
+        double* tmp = Q; Q = Qn; Qn = tmp;
+    }
+    memcpy(emb, Q, (size_t)(n * K) * sizeof(double));
+    free(q32); free(Q); free(Qn); free(w);
+    return 0;
+}

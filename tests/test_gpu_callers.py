@@ -123,14 +123,23 @@ def test_pseudotime_matches_restatement(gpu):
     for root, K in ((0, 3), (1234, 6)):
         got = callers.pseudotime_diffusion(A, root, K)
         ref = orc.pseudotime_f32(ip, ix, v, root, K, 100)
-        assert np.abs(got - ref).max() < 1e-4, (root, K)
+        ref64 = orc.pseudotime_f64(ip, ix, v, root, K, 100)
+        # anchored to FP64: the GPU (its sums fold in another order) is at
+        # most twice as far from the FP64 result as the restatement
+        eg, ec = np.abs(got - ref64).max(), np.abs(ref - ref64).max()
+        print(f"\npseudotime three blocks root {root} K {K}: GPU {eg:.2e}, "
+              f"restatement {ec:.2e} vs FP64")
+        assert eg <= 2 * ec + 1e-7, (root, K)
         assert got[root] == 0.0 and got.max() == 1.0
     # a kNN graph (raw weights, as the reference's tests pass adjacencies)
     g = synth.knn_graph(30_000, 10, seed=25)
     A = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
     got = callers.pseudotime_diffusion(A, 77, 4)
     ref = orc.pseudotime_f32(g.indptr, g.indices, g.data, 77, 4, 100)
-    assert np.abs(got - ref).max() < 2e-3
+    ref64 = orc.pseudotime_f64(g.indptr, g.indices, g.data, 77, 4, 100)
+    eg, ec = np.abs(got - ref64).max(), np.abs(ref - ref64).max()
+    print(f"\npseudotime knn 30k: GPU {eg:.2e}, restatement {ec:.2e} vs FP64")
+    assert eg <= 2 * ec + 1e-7
     assert np.all((got >= 0) & (got <= 1)) and got[77] == 0.0
 
 

@@ -174,6 +174,31 @@ def test_pseudotime_depends_only_on_the_leading_subspace():
     assert np.abs(orc.pseudotime_f32(ip, ix, v, 5, 4, 0) - d / d.max()).max() < 1e-6
 
 
+def test_pseudotime_fp64_anchor():
+    """The FP64 anchor: converged distances equal the leading eigenspace's
+    to FP64 accuracy (the fp32 restatement reaches 1e-4); with no iteration
+    they are those of the QR of the fp32 FastRNG(42) start in FP64."""
+    from test_oracle_walks import three_blocks
+    ip, ix, v, S = three_blocks(seed=11, m=80)
+    w, V = np.linalg.eigh(S.astype(np.float32).astype(np.float64))
+    order = np.argsort(-np.abs(w))
+    B = V[:, order[:3]]
+    for root in (0, 137):
+        got = orc.pseudotime_f64(ip, ix, v, root, 3, 300)
+        d = np.linalg.norm(B - B[root], axis=1)
+        assert np.abs(got - d / d.max()).max() < 1e-9
+        assert got[root] == 0.0 and got.max() == 1.0
+        e = np.abs(orc.pseudotime_f32(ip, ix, v, root, 3, 100)
+                   - orc.pseudotime_f64(ip, ix, v, root, 3, 100)).max()
+        assert 0 < e < 1e-5
+    n = len(ip) - 1
+    A = (orc.fastrng_uniform(42, n * 4) - np.float32(0.5)).reshape(n, 4).astype(np.float64)
+    q, r = np.linalg.qr(A)
+    Q0 = q * np.sign(np.diag(r))
+    d = np.linalg.norm(Q0 - Q0[5], axis=1)
+    assert np.abs(orc.pseudotime_f64(ip, ix, v, 5, 4, 0) - d / d.max()).max() < 1e-12
+
+
 def test_algebra_spmm_known_answers_and_skips():
     # test_algebra.cpp:281-325 with make_test_matrix_3x3 [[1,0,2],[0,3,0],[4,5,6]]
     ip, ix = [0, 2, 3, 6], [0, 2, 1, 0, 1, 2]

@@ -143,6 +143,36 @@ def test_diffusion_map_finds_the_leading_eigenvectors():
                       atol=1e-6)
 
 
+def test_diffusion_map_fp64_anchor():
+    """The FP64 anchor: no iteration is the QR of the fp32 start (R with a
+    positive diagonal); one step on a non-symmetric P is the QR of P Q0 (a
+    transposed or misindexed SpMM fails it); 300 steps span the top-3
+    eigenspace to FP64 accuracy; the fp32 restatement stays near it."""
+    g = small(200, 5, seed=4)
+    P = dense(g)
+    init = orc.dmap_init_f32(200, 3).astype(np.float64)
+
+    def qr_pos(A):
+        q, r = np.linalg.qr(A)
+        return q * np.sign(np.diag(r))
+
+    Q0 = qr_pos(init)
+    assert np.abs(orc.dmap_f64(g.indptr, g.indices, g.data, 3, 0) - Q0).max() < 1e-12
+    Q1 = qr_pos(P @ Q0)
+    assert np.abs(orc.dmap_f64(g.indptr, g.indices, g.data, 3, 1) - Q1).max() < 1e-12
+    assert np.abs(qr_pos(P.T @ Q0) - Q1).max() > 1e-2
+    ip, ix, v, S = three_blocks()
+    Q = orc.dmap_f64(ip, ix, v, 3, 300)
+    w, V = np.linalg.eigh(S.astype(np.float32).astype(np.float64))
+    order = np.argsort(-np.abs(w))
+    B, _ = np.linalg.qr(V[:, order[:3]])
+    assert np.abs(Q - B @ (B.T @ Q)).max() < 1e-9
+    assert np.abs(Q.T @ Q - np.eye(3)).max() < 1e-13
+    Q32 = orc.dmap_f32(ip, ix, v, 3, 60)
+    e = orc.rel_err(Q32, orc.dmap_f64(ip, ix, v, 3, 60))
+    assert 0 < e < 1e-5
+
+
 def test_diffusion_distance_both_branches():
     for n in (120, 1001):                     # rows of P^t; rows of (P^T)^t
         g = small(n, 5, seed=n)
