@@ -137,6 +137,93 @@ __global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict_
     y[row] = acc;
 }
 
+// F = 1: y = P x with one lane per row and 64 consecutive rows per wave.
+// The wave stages its rows' entries in windows of C: coalesced loads of
+// indices and values, the x values gathered for them, and (value, x) pairs
+// written to its own LDS region.  Each lane then adds its row's products in
+// stored order from 0, one FMA each (the row-split kernel's chain), reading
+// the pairs eight at a time.  Waves never share LDS, so a wave-level fence
+// orders the writes before the reads.
+template <int C>
+__global__ __launch_bounds__(kBlock) void k_spmv_staged(const int32_t* __restrict__ row_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val,
+                                                        const float* __restrict__ x,
+                                                        float* __restrict__ y, int32_t n) {
+    constexpr int kWaves = kBlock / 64;
+    constexpr int kBatch = C / 64 < 16 ? C / 64 : 16;  // entries per lane per batch
+    constexpr int kSpan = 64 * kBatch;                  // entries per batch
+    __shared__ float2 pairs[kWaves][C];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r0 = (blockIdx.x * kWaves + wid) * 64;
+    if (r0 >= n) return;
+    const int row = r0 + lane;
+    const bool valid = row < n;
+    const int rs = valid ? row_ptr[row] : 0;
+    const int re = valid ? row_ptr[row + 1] : 0;
+    const int S = row_ptr[r0];
+    const int E = row_ptr[min(r0 + 64, n)];
+    float2* buf = pairs[wid];
+    int k = rs;
+    float acc = 0.0f;
+    // batch b covers entries [b, b + kSpan); positions past E are clamped to
+    // E - 1, so every load stays inside the operator's arrays and nothing is
+    // branched on.  Offsets are taken from the batch start (uniform), 32-bit.
+    int c[kBatch];
+    float v[kBatch];
+    auto load = [&](int b) {
+        const char* cb = reinterpret_cast<const char*>(col + b);
+        const char* vb = reinterpret_cast<const char*>(val + b);
+        const unsigned last = (unsigned)(E - 1 - b);
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const unsigned o = min((unsigned)(lane + 64 * j), last) << 2;
+            c[j] = *reinterpret_cast<const int32_t*>(cb + o);
+            v[j] = *reinterpret_cast<const float*>(vb + o);
+        }
+    };
+    if (S < E) load(S);
+    for (int ws = S; ws < E; ws += C) {
+        const int we = min(ws + C, E);
+        for (int b = ws; b < we; b += kSpan) {
+            float xg[kBatch], vc[kBatch];
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j) {
+                // byte offsets fit 32 bits: the launcher requires n_sec < 2^30
+                xg[j] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(x) +
+                                                        ((unsigned)c[j] << 2));
+                vc[j] = v[j];
+            }
+            // the next batch's indices and values load under these gathers
+            // and under the window's chains
+            if (b + kSpan < E) load(b + kSpan);
+            // batches tile the window, so every place written is inside the
+            // wave's region; places past the window are never read
+#pragma unroll
+            for (int j = 0; j < kBatch; ++j)
+                buf[b - ws + lane + 64 * j] = make_float2(vc[j], xg[j]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int hi = min(re, we);
+        const int k_next = max(k, hi);        // where the chain resumes
+        for (; k < hi; k += 8) {
+            float2 p[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) p[u] = buf[min(k + u, we - 1) - ws];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < hi) acc = __builtin_fmaf(p[u].x, p[u].y, acc);
+        }
+        k = k_next;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (valid) y[row] = acc;
+}
+
 template <int VEC, int G, int U, bool NT>
 __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
     const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
@@ -1139,6 +1226,22 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
     }
     if (shape.rows_per_group > 0) {
         launch_spmm_grouped(P, X, Y, F, shape, s);
+        return;
+    }
+    // F = 1: windows of 256 * unroll entries per wave (4: 1,024, 8: 2,048,
+    // 16: 4,096)
+    if (F == 1 && shape.group == 1 && shape.vec == 1 && P.n_sec < (1 << 30)) {
+        const unsigned nb = (unsigned)((P.n + kBlock - 1) / kBlock);
+        if (shape.unroll == 4)
+            hipLaunchKernelGGL(k_spmv_staged<1024>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
+                               P.col, P.val, X, Y, P.n);
+        else if (shape.unroll == 16)
+            hipLaunchKernelGGL(k_spmv_staged<4096>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
+                               P.col, P.val, X, Y, P.n);
+        else
+            hipLaunchKernelGGL(k_spmv_staged<2048>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
+                               P.col, P.val, X, Y, P.n);
+        check_launch("spmv launch");
         return;
     }
     if (F == 1 && shape.group == 1 && shape.vec == 1 &&

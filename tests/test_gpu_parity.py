@@ -112,10 +112,10 @@ def test_ragged_rows_empty_long_unsorted(gpu):
 
 @pytest.mark.parametrize("n", [5000, 20_000])
 def test_single_feature_rows_of_every_length(gpu, n):
-    """F = 1 runs one lane per row, reading indices and values 16 bytes at a
-    time once aligned, 16, 8 then 4 at a time, then the tail: rows of 0 to
-    40 nonzeros starting at every offset modulo 4 cover each path, with and
-    without the locality schedule (from 16,384 rows)."""
+    """F = 1 runs one lane per row, 64 rows per wave, the rows' entries
+    staged through LDS in windows: rows of 0 to 40 nonzeros at every offset
+    within a wave's window, with and without the locality schedule (from
+    16,384 rows)."""
     rng = np.random.Generator(np.random.PCG64(n))
     lens = np.arange(n) % 41
     rng.shuffle(lens)
@@ -127,6 +127,38 @@ def test_single_feature_rows_of_every_length(gpu, n):
     P = CsrMatrix.from_arrays(ip, ix, d, (n, n))
     got = diffusion.diffuse_vector(P, x[:, 0].copy(), 4)
     assert np.array_equal(got, orc.diffuse_vector_f32(ip, ix, d, x[:, 0], 4))
+
+
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+def test_single_feature_windows(gpu, unroll):
+    """Windows of 1,024, 2,048 and 4,096 entries (unroll 4, 8, 16): rows
+    longer than a window (up to 5,000 entries) continue their chain across
+    windows; a wave of 64 rows holds several windows; a run of empty rows;
+    n not a multiple of 64 leaves the last wave partial; the transposed
+    operator runs the same kernel."""
+    from scl_amd import torch_ops
+    import torch
+    n = 6_001
+    rng = np.random.Generator(np.random.PCG64(unroll))
+    lens = rng.integers(0, 60, n)
+    lens[[3, 64, 700, 701, 6000]] = [5000, 1500, 2049, 4097, 1025]
+    lens[100:200] = 0
+    ip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    ix = np.concatenate([rng.integers(0, n, l) for l in lens]).astype(np.int64)
+    d = rng.random(ip[-1]).astype(np.float32)
+    x = synth.normal(n, 1, seed=unroll)
+    P = CsrMatrix.from_arrays(ip, ix, d, (n, n))
+    P.prepare_device()
+    for tr in (False, True):
+        torch_ops.set_shape(P, 1, vec=1, group=1, unroll=unroll, transpose=tr)
+        got = torch_ops.spmm(P, torch.from_numpy(x).cuda(), transpose=tr).cpu().numpy()
+        if tr:
+            tp, ti, td = P.transpose().export()
+            ref = orc.spmm_step_f32(tp, ti, td, x)
+        else:
+            ref = orc.spmm_step_f32(ip, ix, d, x)
+            assert np.all(got[100:200] == 0)
+        assert np.array_equal(got, ref), (unroll, tr)
 
 
 def test_no_nonzeros_and_single_node(gpu):
