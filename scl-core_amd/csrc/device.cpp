@@ -247,7 +247,18 @@ DeviceCSR::~DeviceCSR() {
     dev_free(merge.carry_val);
     release_grouped();
     release_transpose();
+    release_renumbered();
     if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+}
+
+void DeviceCSR::release_renumbered() {
+    dev_free(ren.row_ptr);
+    dev_free(ren.col);
+    dev_free(ren.val);
+    dev_free(ren.src);
+    ren.row_ptr = ren.col = ren.src = nullptr;
+    ren.val = nullptr;
+    ren.fingerprint = 0;
 }
 
 void DeviceCSR::release_grouped() {
@@ -391,6 +402,15 @@ DeviceCSR& device_csr(scl_sparse_matrix* h, int device) {
             slot->release_grouped();
         }
     }
+    if (slot->ren.row_ptr) {
+        // the renumbered copy follows the schedule's pattern
+        if (same_shape && slot->sched && slot->ren.fingerprint == slot->fingerprint) {
+            launch_gather_f32(slot->ren.src, slot->val, slot->ren.val, nnz, nullptr);
+            hip_check(hipStreamSynchronize(nullptr), "refresh renumbered values");
+        } else {
+            slot->release_renumbered();
+        }
+    }
     if (slot->pins > 0 && slot->t_row_ptr) {
         // a plan holds the transposed arrays: refresh their values in place
         launch_gather_f32(slot->t_src, slot->val, slot->t_val, nnz, nullptr);
@@ -516,7 +536,44 @@ bool ensure_staged(scl_sparse_matrix* h, DeviceCSR& d) {
     return true;
 }
 
+bool ensure_renumbered(scl_sparse_matrix* h, DeviceCSR& d) {
+    if (!d.sched || d.n != d.n_sec) return false;
+    if (d.ren.row_ptr && d.ren.fingerprint == d.fingerprint) return true;
+    d.release_renumbered();
+    const int64_t n = d.n, nnz = d.nnz;
+    std::vector<int32_t> order((size_t)n), inv((size_t)n);
+    hip_check(hipMemcpy(order.data(), d.sched, (size_t)n * 4, hipMemcpyDeviceToHost),
+              "download schedule");
+    for (int64_t i = 0; i < n; ++i) inv[order[i]] = (int32_t)i;
+    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)std::max<int64_t>(nnz, 1)),
+        src((size_t)std::max<int64_t>(nnz, 1));
+    rp[0] = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t r = order[i], b = h->indptr[r], e = h->indptr[r + 1];
+        int32_t k2 = rp[i];
+        for (int64_t k = b; k < e; ++k, ++k2) {
+            ci[k2] = inv[h->indices[k]];
+            src[k2] = (int32_t)k;
+        }
+        rp[i + 1] = k2;
+    }
+    d.ren.row_ptr = dev_alloc<int32_t>((size_t)n + 1, "renumbered row_ptr");
+    d.ren.col = dev_alloc<int32_t>(ci.size(), "renumbered col");
+    d.ren.val = dev_alloc<float>(ci.size(), "renumbered val");
+    d.ren.src = dev_alloc<int32_t>(src.size(), "renumbered entry map");
+    hip_check(hipMemcpy(d.ren.row_ptr, rp.data(), rp.size() * 4, hipMemcpyHostToDevice),
+              "upload");
+    hip_check(hipMemcpy(d.ren.col, ci.data(), ci.size() * 4, hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(d.ren.src, src.data(), src.size() * 4, hipMemcpyHostToDevice),
+              "upload");
+    launch_gather_f32(d.ren.src, d.val, d.ren.val, nnz, nullptr);
+    hip_check(hipStreamSynchronize(nullptr), "renumbered values");
+    d.ren.fingerprint = d.fingerprint;
+    return true;
+}
+
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s) {
+    if (d.ren.row_ptr) launch_gather_f32(d.ren.src, d.val, d.ren.val, d.nnz, s);
     for (GroupedCsr& g : d.grouped)
         if (g.R) launch_gather_f32(g.gperm, d.val, g.gval, d.nnz, s);
     for (TiledCsr& t : d.tiled)

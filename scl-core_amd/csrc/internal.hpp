@@ -113,10 +113,23 @@ struct DeviceCSR {
     // LDS-tiled forms, [tiled_index(lpr, budget)] (values refreshed with the
     // operator's)
     TiledCsr tiled[4];
+    // the operator renumbered in the locality order, for single-vector
+    // products: row r' is row sched[r'] with its entries in stored order,
+    // column c becomes inv(sched)[c]; src[k'] = the entry's place in (col,
+    // val).  Built on first use (square operators with a schedule), values
+    // re-gathered with the operator's.
+    struct {
+        int32_t* row_ptr = nullptr;
+        int32_t* col = nullptr;
+        float* val = nullptr;
+        int32_t* src = nullptr;
+        uint64_t fingerprint = 0;
+    } ren;
 
     ~DeviceCSR();
     void release_grouped();
     void release_transpose();
+    void release_renumbered();
 };
 
 }  // namespace scl_amd
@@ -180,6 +193,9 @@ bool ensure_tiled(scl_sparse_matrix* h, DeviceCSR& d, int lpr, int budget);
 // Re-gathers the grouped forms' values from d.val (after the device values
 // changed in place), stream-ordered on s.
 void refresh_grouped_values(DeviceCSR& d, hipStream_t s);
+// Builds d.ren for the handle's current pattern when missing or stale; false
+// when the operator has no schedule or is not square.
+bool ensure_renumbered(scl_sparse_matrix* h, DeviceCSR& d);
 
 // Operators with at least this many rows get a locality schedule.
 constexpr int64_t kScheduleMinRows = 16384;

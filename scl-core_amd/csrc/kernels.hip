@@ -143,13 +143,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv_lane(const int32_t* __restrict_
 // written to its own LDS region.  Each lane then adds its row's products in
 // stored order from 0, one FMA each (the row-split kernel's chain), reading
 // the pairs eight at a time.  Waves never share LDS, so a wave-level fence
-// orders the writes before the reads.
+// orders the writes before the reads.  With out_row set, row r is stored at
+// y[out_row[r]] (a renumbered operator writing the original numbering).
 template <int C>
 __global__ __launch_bounds__(kBlock) void k_spmv_staged(const int32_t* __restrict__ row_ptr,
                                                         const int32_t* __restrict__ col,
                                                         const float* __restrict__ val,
                                                         const float* __restrict__ x,
-                                                        float* __restrict__ y, int32_t n) {
+                                                        float* __restrict__ y, int32_t n,
+                                                        const int32_t* __restrict__ out_row) {
     constexpr int kWaves = kBlock / 64;
     constexpr int kBatch = C / 64 < 16 ? C / 64 : 16;  // entries per lane per batch
     constexpr int kSpan = 64 * kBatch;                  // entries per batch
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_staged(const int32_t* __restric
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (valid) y[row] = acc;
+    if (valid) y[out_row ? out_row[row] : row] = acc;
 }
 
 template <int VEC, int G, int U, bool NT>
@@ -1209,6 +1211,23 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
 
 }  // namespace
 
+void launch_spmv_staged(const CsrView& P, const int32_t* out_row, const float* x, float* y,
+                        int unroll, hipStream_t s) {
+    if (P.n <= 0) return;
+    // windows of 256 * unroll entries per wave (4: 1,024, 8: 2,048, 16: 4,096)
+    const unsigned nb = (unsigned)((P.n + kBlock - 1) / kBlock);
+    if (unroll == 4)
+        hipLaunchKernelGGL(k_spmv_staged<1024>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr, P.col,
+                           P.val, x, y, P.n, out_row);
+    else if (unroll == 16)
+        hipLaunchKernelGGL(k_spmv_staged<4096>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr, P.col,
+                           P.val, x, y, P.n, out_row);
+    else
+        hipLaunchKernelGGL(k_spmv_staged<2048>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr, P.col,
+                           P.val, x, y, P.n, out_row);
+    check_launch("spmv launch");
+}
+
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
                      const SpmmShape& shape, hipStream_t s) {
     if (P.n <= 0 || F <= 0) return;
@@ -1228,20 +1247,8 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
         launch_spmm_grouped(P, X, Y, F, shape, s);
         return;
     }
-    // F = 1: windows of 256 * unroll entries per wave (4: 1,024, 8: 2,048,
-    // 16: 4,096)
     if (F == 1 && shape.group == 1 && shape.vec == 1 && P.n_sec < (1 << 30)) {
-        const unsigned nb = (unsigned)((P.n + kBlock - 1) / kBlock);
-        if (shape.unroll == 4)
-            hipLaunchKernelGGL(k_spmv_staged<1024>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
-                               P.col, P.val, X, Y, P.n);
-        else if (shape.unroll == 16)
-            hipLaunchKernelGGL(k_spmv_staged<4096>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
-                               P.col, P.val, X, Y, P.n);
-        else
-            hipLaunchKernelGGL(k_spmv_staged<2048>, dim3(nb), dim3(kBlock), 0, s, P.row_ptr,
-                               P.col, P.val, X, Y, P.n);
-        check_launch("spmv launch");
+        launch_spmv_staged(P, nullptr, X, Y, shape.unroll, s);
         return;
     }
     if (F == 1 && shape.group == 1 && shape.vec == 1 &&

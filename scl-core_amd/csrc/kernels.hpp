@@ -208,6 +208,11 @@ void launch_sub_f32(const float* a, const float* b, float* c, int64_t m, hipStre
 // dst[idx[p]] = src[p] and dst[p] = src[idx[p]] for p < m.
 void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
                         hipStream_t s);
+// F = 1 product through LDS-staged windows of 256 * unroll entries (unroll
+// 4, 8 or 16); with out_row set, row r of P is stored at y[out_row[r]].
+// Requires P.n_sec < 2^30.
+void launch_spmv_staged(const CsrView& P, const int32_t* out_row, const float* x, float* y,
+                        int unroll, hipStream_t s);
 void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
                        hipStream_t s);
 // dst[p] = idx[p] >= 0 ? src[idx[p]] : -0.0 for p < m.

@@ -7,6 +7,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -44,9 +45,17 @@ struct Session {
     CsrView v;
     hipStream_t s;
     int64_t n;
+    scl_sparse_matrix* h;
+    std::unique_ptr<DevBuf> xs;  // x in the locality order (renumbered products)
+    int ren_state = -1;          // -1 unknown, 0 no renumbered copy, 1 ready
+
+    bool renumbered() {
+        if (ren_state < 0) ren_state = ensure_renumbered(h, *d) ? 1 : 0;
+        return ren_state == 1;
+    }
 
     Session(scl_sparse_matrix* P, int64_t n_nodes, bool square = true)
-        : dev(current_device()), call_lk(device_mutex(dev)), lk(P->mu) {
+        : dev(current_device()), call_lk(device_mutex(dev)), lk(P->mu), h(P) {
         if (square && P->rows != P->cols)
             fail(SCL_ERROR_DIMENSION_MISMATCH, "the transition matrix must be square");
         n = P->primary();
@@ -67,8 +76,25 @@ struct Session {
         if (it != d->shapes.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
         return spmm_default_shape(F, X, Y);
     }
-    // y = P x
-    void spmv(const float* x, float* y) { launch_spmm_f32(v, x, y, 1, shape(1, x, y), s); }
+    // y = P x.  With the default single-vector shape, square operators with
+    // a locality schedule run their renumbered copy (built on the first such
+    // call): x is gathered into the locality order, and each row's result
+    // is stored at its original place; every row keeps its chain.
+    void spmv(const float* x, float* y) {
+        const SpmmShape sh = shape(1, x, y);
+        if (sh.group == 1 && sh.vec == 1 && !sh.merge && !sh.rows_per_group &&
+            v.n_sec < (1 << 30) && renumbered()) {
+            if (!xs) xs.reset(new DevBuf((size_t)v.n));
+            launch_gather_f32(d->sched, x, xs->get(), v.n, s);
+            CsrView r = v;
+            r.row_ptr = d->ren.row_ptr;
+            r.col = d->ren.col;
+            r.val = d->ren.val;
+            launch_spmv_staged(r, d->sched, xs->get(), y, sh.unroll, s);
+            return;
+        }
+        launch_spmm_f32(v, x, y, 1, sh, s);
+    }
     void spmm(const float* X, float* Y, int64_t F) {
         launch_spmm_f32(v, X, Y, F, shape(F, X, Y), s);
     }

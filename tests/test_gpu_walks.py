@@ -38,6 +38,35 @@ def test_lazy_walk_and_heat_kernel(gpu):
         assert np.array_equal(got, orc.heat_kernel_f32(ip, ix, v, t, used))
 
 
+def test_renumbered_products_follow_value_changes(gpu):
+    """From 16,384 rows an operator has a locality schedule, and the walks'
+    single-vector products run a copy renumbered in that order (x gathered
+    into it, each row stored back at its place): bit-identical, also after
+    device values change (set_values) and after host values change (a
+    re-normalisation), on a symmetric and on a non-symmetric pattern."""
+    import torch
+    from scl_amd import torch_ops
+    for sym in (True, False):
+        g, P = op(20_000, 10, seed=5, sym=sym)
+        x = synth.normal(g.n, 1, seed=6)[:, 0]
+        got = diffusion.lazy_random_walk(P, x.copy(), 5, 0.3)
+        assert np.array_equal(got, orc.lazy_walk_f32(g.indptr, g.indices, g.data, x, 5, 0.3))
+        v2 = (g.data * 0.5).astype(np.float32)
+        torch_ops.set_values(P, torch.from_numpy(v2).cuda())
+        got = diffusion.lazy_random_walk(P, x.copy(), 5, 0.3)
+        assert np.array_equal(got, orc.lazy_walk_f32(g.indptr, g.indices, v2, x, 5, 0.3))
+    raw = synth.knn_graph(20_000, 10, seed=7)
+    A = CsrMatrix.from_arrays(raw.indptr, raw.indices, raw.data, (raw.n, raw.n))
+    x = synth.normal(raw.n, 1, seed=8)[:, 0]
+    got = diffusion.lazy_random_walk(A, x.copy(), 3, 0.3)
+    assert np.array_equal(got, orc.lazy_walk_f32(raw.indptr, raw.indices, raw.data, x, 3, 0.3))
+    diffusion.compute_transition_matrix(A)
+    ip, ix, v = A.export()
+    assert not np.array_equal(v, raw.data)
+    got = diffusion.lazy_random_walk(A, x.copy(), 3, 0.3)
+    assert np.array_equal(got, orc.lazy_walk_f32(ip, ix, v, x, 3, 0.3))
+
+
 def test_random_walk_with_restart(gpu):
     g, P = op(sym=False)
     seeds = [5, 17, 17, -4, 10 ** 9]
