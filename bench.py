@@ -395,6 +395,12 @@ def leg_wk(steps: int, with_cpu: bool = True) -> dict:
 
     for name, (gpu_fn, cpu_fn) in cases.items():
         record(name, gpu_fn, cpu_fn, max(3, steps // 2))
+    # one lazy-walk step (F = 1 SpMV on the renumbered copy, x gathered into
+    # its order, blend) from the difference of a 101-step and a 1-step walk
+    reps = max(3, steps // 2)
+    t1 = host_median_ms(lambda: diffusion.lazy_random_walk(P, x.copy(), 1, 0.5), reps)
+    t101 = host_median_ms(lambda: diffusion.lazy_random_walk(P, x.copy(), 101, 0.5), reps)
+    out["lazy_walk_step_us"] = round((t101 - t1) * 10.0, 2)
     # dense output: diffusion distance on a 4000-node graph
     gs = synth.knn_graph(4000, 15, seed=7).normalized()
     Ps = CsrMatrix.from_arrays(gs.indptr, gs.indices, gs.data, (4000, 4000))
@@ -699,9 +705,16 @@ def main() -> int:
     stream = torch.cuda.current_stream(dev)
     tune_ms = None
     if args.calibration_copy:
+        # a runtime device-to-device copy (the __amd_rocclr_copyBuffer
+        # kernel) of known traffic, for scaling the byte counters
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
         a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.cuda.synchronize()
+        if hip.hipMemcpy(ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(a.data_ptr()),
+                         ctypes.c_size_t(a.numel() * 4), 3) != 0:
+            raise RuntimeError("calibration copy failed")
         torch.cuda.synchronize()
         log("calibration copy: 1 GiB read, 1 GiB written")
         del a, b
