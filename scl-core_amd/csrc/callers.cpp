@@ -177,23 +177,52 @@ SCL_EXPORT scl_error_t scl_pseudotime_diffusion(scl_sparse_t adjacency,
             std::vector<float> start((size_t)(N * K));
             FastRng rng(kPseudotimeSeed);
             for (auto& x : start) x = rng.uniform() - 0.5f;
+            const bool chol = K <= kCholQrMaxK;
             DevBuf Q(N * K), Qn(N * K), C(N * K), w(N), gpart(gram_schmidt_part_floats(N, K)),
-                sc(8);
-            S.upload(Q.get(), start.data(), N * K);
-            // Gram-Schmidt over contiguous columns (C is K x N), across the GPU
-            auto orthonormalise = [&](float* M) {
+                sc(8), cw(chol ? cholqr_workspace_doubles(N, K) : 0, sizeof(double)),
+                flag(1, sizeof(int));
+            // a Cholesky QR (four launches); with more than kCholQrMaxK
+            // columns or a degenerate basis, Gram-Schmidt over contiguous
+            // columns (C is K x N) across the GPU, which keeps the reference's
+            // FastRNG directions for degenerate columns
+            auto orthonormalise = [&](float* M, bool use_chol) {
+                if (use_chol) {
+                    launch_cholqr(M, N, K, cw.as<double>(), flag.as<int>(), S.s);
+                    return;
+                }
                 launch_transpose_rect(M, C.get(), N, K, S.s);
                 launch_gram_schmidt(C.get(), w.get(), N, K, gpart.get(), sc.get(), false,
                                     kDegenerateSeedStep, S.s);
                 launch_transpose_rect(C.get(), M, K, N, S.s);
             };
-            orthonormalise(Q.get());
-            float* cur = Q.get();
-            float* nxt = Qn.get();
-            for (int64_t it = 0; it < kPseudotimeIterations; ++it) {
-                S.spmm(cur, nxt, K);
-                orthonormalise(nxt);
-                std::swap(cur, nxt);
+            // the Cholesky path iterates in the operator's locality numbering
+            // (its renumbered copy: the products gather neighbouring rows);
+            // the orthonormalisation does not depend on the row order
+            auto components = [&](bool use_chol) {
+                const bool ren = use_chol && S.renumbered();
+                hip_check(hipMemsetAsync(flag.get(), 0, sizeof(int), S.s), "clear flag");
+                S.upload(ren ? Qn.get() : Q.get(), start.data(), N * K);
+                if (ren) S.renumber_rows(Qn.get(), Q.get(), K, false);
+                orthonormalise(Q.get(), use_chol);
+                float* cur = Q.get();
+                float* nxt = Qn.get();
+                for (int64_t it = 0; it < kPseudotimeIterations; ++it) {
+                    if (ren) S.spmm_renumbered(cur, nxt, K);
+                    else S.spmm(cur, nxt, K);
+                    orthonormalise(nxt, use_chol);
+                    std::swap(cur, nxt);
+                }
+                if (ren) {
+                    S.renumber_rows(cur, nxt, K, true);
+                    std::swap(cur, nxt);
+                }
+                return cur;
+            };
+            float* cur = components(chol);
+            if (chol) {
+                int degenerate = 0;
+                S.download(reinterpret_cast<float*>(&degenerate), flag.get(), 1);
+                if (degenerate) cur = components(false);
             }
             launch_pt_distance(cur, N, K, root_cell, pt.get(), part.get(), S.s);
         }

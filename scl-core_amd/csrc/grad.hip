@@ -377,6 +377,19 @@ __global__ __launch_bounds__(kBlock) void k_gather(const int32_t* __restrict__ i
     if (p < m) dst[p] = src[idx[p]];
 }
 
+// Rows of an n x K row-major matrix: dst[i, :] = src[idx[i], :] (gather) or
+// dst[idx[i], :] = src[i, :] (scatter), one element per thread.
+__global__ __launch_bounds__(kBlock) void k_rows(const int32_t* __restrict__ idx,
+                                                 const float* __restrict__ src,
+                                                 float* __restrict__ dst, int64_t n, int64_t K,
+                                                 int scatter) {
+    const int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n * K) return;
+    const int64_t i = p / K, c = p - i * K, j = idx[i];
+    if (scatter) dst[j * K + c] = src[p];
+    else dst[p] = src[j * K + c];
+}
+
 __global__ __launch_bounds__(kBlock) void k_gather_pad(const int32_t* __restrict__ idx,
                                                        const float* __restrict__ src,
                                                        float* __restrict__ dst, int64_t m) {
@@ -522,6 +535,14 @@ void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((m + kBlock - 1) / kBlock)),
                        dim3(kBlock), 0, s, idx, src, dst, m);
     check_launch("gather launch");
+}
+
+void launch_rows_f32(const int32_t* idx, const float* src, float* dst, int64_t n, int64_t K,
+                     bool scatter, hipStream_t s) {
+    if (n <= 0 || K <= 0) return;
+    hipLaunchKernelGGL(k_rows, dim3((unsigned)((n * K + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       s, idx, src, dst, n, K, scatter ? 1 : 0);
+    check_launch("row permutation launch");
 }
 
 void launch_gather_pad_f32(const int32_t* idx, const float* src, float* dst, int64_t m,

@@ -213,6 +213,10 @@ void launch_scatter_f32(const int32_t* idx, const float* src, float* dst, int64_
 // Requires P.n_sec < 2^30.
 void launch_spmv_staged(const CsrView& P, const int32_t* out_row, const float* x, float* y,
                         int unroll, hipStream_t s);
+// dst[i, :] = src[idx[i], :] for the n rows of a row-major n x K matrix, or
+// with scatter set dst[idx[i], :] = src[i, :] (idx a permutation).
+void launch_rows_f32(const int32_t* idx, const float* src, float* dst, int64_t n, int64_t K,
+                     bool scatter, hipStream_t s);
 void launch_gather_f32(const int32_t* idx, const float* src, float* dst, int64_t m,
                        hipStream_t s);
 // dst[p] = idx[p] >= 0 ? src[idx[p]] : -0.0 for p < m.
@@ -311,6 +315,17 @@ void launch_mgs(float* Q, float* w, int64_t n, int64_t K, hipStream_t s);
 void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, float* S,
                          bool reorth, uint64_t rng_mult, hipStream_t s);
 int64_t gram_schmidt_part_floats(int64_t n, int64_t K);
+// Cholesky QR of a row-major n x K matrix in place (K <= kCholQrMaxK): the
+// Gram matrix and the factor in FP64, M <- M R^-1 rounded once.  A pivot
+// (squared norm of a column after projection) at or below kCholQrMinPivot
+// sets *degenerate and leaves M unchanged; callers then redo the work with
+// modified Gram-Schmidt, which keeps the reference's degenerate-column rules.
+// work: cholqr_workspace_doubles(n, K) doubles.
+constexpr int kCholQrMaxK = 22;
+constexpr double kCholQrMinPivot = 1e-12;
+void launch_cholqr(float* M, int64_t n, int64_t K, double* work, int* degenerate,
+                   hipStream_t s);
+int64_t cholqr_workspace_doubles(int64_t n, int64_t K);
 // B (cols x rows) = A (rows x cols)^T, both row-major.
 void launch_transpose_rect(const float* A, float* B, int64_t rows, int64_t cols,
                            hipStream_t s);

@@ -98,6 +98,24 @@ struct Session {
     void spmm(const float* X, float* Y, int64_t F) {
         launch_spmm_f32(v, X, Y, F, shape(F, X, Y), s);
     }
+    // Y = P X in the locality numbering (X and Y hold row r' for row
+    // sched[r'] of the operator); needs renumbered().  Every row keeps its
+    // stored-order chain.
+    void spmm_renumbered(const float* X, float* Y, int64_t F) {
+        CsrView r = v;
+        r.row_ptr = d->ren.row_ptr;
+        r.col = d->ren.col;
+        r.val = d->ren.val;
+        r.sched = nullptr;
+        r.grouped = nullptr;
+        r.staged = nullptr;
+        r.tiled = nullptr;
+        launch_spmm_f32(r, X, Y, F, spmm_default_shape(F, X, Y), s);
+    }
+    // rows into (scatter = false) or out of (true) the locality numbering
+    void renumber_rows(const float* src, float* dst, int64_t F, bool back) {
+        launch_rows_f32(d->sched, src, dst, v.n, F, back, s);
+    }
     // Y = P' X for the operator's pattern with other values (same order as
     // the stored ones).
     void spmm_values(const float* val, const float* X, float* Y, int64_t F) {

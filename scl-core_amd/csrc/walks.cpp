@@ -290,30 +290,57 @@ SCL_EXPORT scl_error_t scl_diffusion_diffusion_map_embedding(scl_sparse_t transi
             const int64_t iters = n_iter == 0 ? kDefaultNIter : n_iter;
             DevBuf Q(n * K), Qn(n * K), w(n);
             // one workgroup orthogonalises small bases; from kWideGramSchmidtRows
-            // rows the columns are transposed to contiguous vectors and every
-            // dot product spreads over the GPU
+            // rows a Cholesky QR (four launches) replaces it, or, with more
+            // than kCholQrMaxK columns or a degenerate basis, the columns are
+            // transposed to contiguous vectors and every dot product of the
+            // Gram-Schmidt spreads over the GPU
             const bool wide = n >= kWideGramSchmidtRows;
+            const bool chol = wide && K <= kCholQrMaxK;
             DevBuf C(wide ? n * K : 0), part(wide ? gram_schmidt_part_floats(n, K) : 0),
-                sc(wide ? 8 : 0);
-            auto orthogonalise = [&](float* M) {
+                sc(wide ? 8 : 0), cw(chol ? cholqr_workspace_doubles(n, K) : 0, sizeof(double)),
+                flag(1, sizeof(int));
+            auto orthogonalise = [&](float* M, bool use_chol) {
                 if (!wide) {
                     launch_mgs(M, w.get(), n, K, S.s);
-                    return;
+                } else if (use_chol) {
+                    launch_cholqr(M, n, K, cw.as<double>(), flag.as<int>(), S.s);
+                } else {
+                    launch_transpose_rect(M, C.get(), n, K, S.s);
+                    launch_gram_schmidt(C.get(), w.get(), n, K, part.get(), sc.get(), true, 0,
+                                        S.s);
+                    launch_transpose_rect(C.get(), M, K, n, S.s);
                 }
-                launch_transpose_rect(M, C.get(), n, K, S.s);
-                launch_gram_schmidt(C.get(), w.get(), n, K, part.get(), sc.get(), true, 0, S.s);
-                launch_transpose_rect(C.get(), M, K, n, S.s);
             };
-            launch_dmap_init(Q.get(), n, K, S.s);
-            orthogonalise(Q.get());
-            float* cur = Q.get();
-            float* nxt = Qn.get();
-            for (int64_t it = 0; it < iters; ++it) {
-                S.spmm(cur, nxt, K);
-                orthogonalise(nxt);
-                std::swap(cur, nxt);
+            // the Cholesky path iterates in the operator's locality numbering
+            // (its renumbered copy: the products gather neighbouring rows);
+            // the orthonormalisation does not depend on the row order
+            auto embed = [&](bool use_chol) {
+                const bool ren = use_chol && S.renumbered();
+                hip_check(hipMemsetAsync(flag.get(), 0, sizeof(int), S.s), "clear flag");
+                launch_dmap_init(ren ? Qn.get() : Q.get(), n, K, S.s);
+                if (ren) S.renumber_rows(Qn.get(), Q.get(), K, false);
+                orthogonalise(Q.get(), use_chol);
+                float* cur = Q.get();
+                float* nxt = Qn.get();
+                for (int64_t it = 0; it < iters; ++it) {
+                    if (ren) S.spmm_renumbered(cur, nxt, K);
+                    else S.spmm(cur, nxt, K);
+                    orthogonalise(nxt, use_chol);
+                    std::swap(cur, nxt);
+                }
+                if (ren) {
+                    S.renumber_rows(cur, nxt, K, true);
+                    std::swap(cur, nxt);
+                }
+                return cur;
+            };
+            float* out = embed(chol);
+            if (chol) {
+                int degenerate = 0;
+                S.download(reinterpret_cast<float*>(&degenerate), flag.get(), 1);
+                if (degenerate) out = embed(false);
             }
-            S.download(embedding, cur, n * K);
+            S.download(embedding, out, n * K);
         }
         SCL_API_RETURN_OK;
     SCL_API_CATCH

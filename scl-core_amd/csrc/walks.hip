@@ -516,6 +516,141 @@ __global__ __launch_bounds__(kBlock) void k_pairwise(const float* __restrict__ A
         }
 }
 
+// ---------------------------------------------------- Cholesky QR ------
+// Q = M R^-1 with R^T R = M^T M (FP64), the Q factor of M with a positive
+// diagonal of R, the basis modified Gram-Schmidt produces in exact
+// arithmetic.  Four launches per orthonormalisation: Gram partials, their
+// sum per entry, the Cholesky factor on one workgroup, the row update.
+
+// Gram entry t (a <= b, row-major upper triangle) of a K x K matrix.
+__device__ __forceinline__ void pair_of(int t, int K, int& a, int& b) {
+    a = 0;
+    int rem = t;
+    while (rem >= K - a) {
+        rem -= K - a;
+        ++a;
+    }
+    b = a + rem;
+}
+
+// Per-workgroup partials of M^T M over tiles of kBlock rows (row-major n x K
+// M), one pair per thread, summed in row order with FP64 FMAs.
+__global__ __launch_bounds__(kBlock) void k_gram_partial(const float* __restrict__ M, int64_t n,
+                                                         int K, double* __restrict__ part) {
+    __shared__ float tile[kBlock][kCholQrMaxK + 1];
+    const int t = threadIdx.x, pairs = K * (K + 1) / 2;
+    int a = 0, b = 0;
+    if (t < pairs) pair_of(t, K, a, b);
+    double acc = 0.0;
+    for (int64_t r0 = (int64_t)blockIdx.x * kBlock; r0 < n; r0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t rows = n - r0 < kBlock ? n - r0 : kBlock;
+        __syncthreads();
+        for (int i = t; i < kBlock * K; i += kBlock) {
+            const int r = i / K, c = i - r * K;
+            tile[r][c] = r < rows ? M[(r0 + r) * K + c] : 0.0f;
+        }
+        __syncthreads();
+        if (t < pairs)
+            for (int r = 0; r < kBlock; ++r)
+                acc = __builtin_fma((double)tile[r][a], (double)tile[r][b], acc);
+    }
+    if (t < pairs) part[(int64_t)blockIdx.x * pairs + t] = acc;
+}
+
+// Gram entry blockIdx.x summed over the partials: thread j adds partials
+// j, j + kBlock, ... in order, then a fixed pairwise tree over the threads.
+__global__ __launch_bounds__(kBlock) void k_gram_reduce(const double* __restrict__ part,
+                                                        int n_parts, int pairs,
+                                                        double* __restrict__ gram) {
+    __shared__ double sh[kBlock];
+    const int p = blockIdx.x, j = threadIdx.x;
+    double s = 0.0;
+    for (int w = j; w < n_parts; w += kBlock) s += part[(int64_t)w * pairs + p];
+    sh[j] = s;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (j < h) sh[j] += sh[j + h];
+        __syncthreads();
+    }
+    if (j == 0) gram[p] = sh[0];
+}
+
+// Factors G = R^T R and writes R^-1 (K x K, upper triangular, FP64).  A
+// pivot at or below kCholQrMinPivot, or not finite, sets *degenerate (never
+// cleared here) and writes the identity.  One thread factors in LDS; each
+// column of R^-1 is then solved by its own thread.
+__global__ __launch_bounds__(kBlock) void k_cholqr_factor(const double* __restrict__ gram, int K,
+                                                          double* __restrict__ rinv,
+                                                          int* __restrict__ degenerate) {
+    __shared__ double G[kCholQrMaxK][kCholQrMaxK];
+    __shared__ double R[kCholQrMaxK][kCholQrMaxK];
+    __shared__ double Ri[kCholQrMaxK][kCholQrMaxK];
+    __shared__ int bad;
+    const int t = threadIdx.x, pairs = K * (K + 1) / 2;
+    if (t < pairs) {
+        int a, b;
+        pair_of(t, K, a, b);
+        G[a][b] = gram[t];
+    }
+    if (t == 0) bad = 0;
+    __syncthreads();
+    if (t == 0) {
+        for (int j = 0; j < K; ++j) {
+            double d = G[j][j];
+            for (int k = 0; k < j; ++k) d -= R[k][j] * R[k][j];
+            if (!(d > kCholQrMinPivot) || !(d < 1e300)) {
+                bad = 1;
+                break;
+            }
+            const double rjj = sqrt(d);
+            R[j][j] = rjj;
+            for (int i = j + 1; i < K; ++i) {
+                double v = G[j][i];
+                for (int k = 0; k < j; ++k) v -= R[k][j] * R[k][i];
+                R[j][i] = v / rjj;
+            }
+        }
+    }
+    __syncthreads();
+    if (bad) {
+        if (t == 0) *degenerate = 1;
+        for (int i = t; i < K * K; i += kBlock) rinv[i] = (i / K == i % K) ? 1.0 : 0.0;
+        return;
+    }
+    // column c of R^-1: back substitution of R x = e_c
+    if (t < K) {
+        const int c = t;
+        for (int i = K - 1; i >= 0; --i) {
+            if (i > c) {
+                Ri[i][c] = 0.0;
+                continue;
+            }
+            double v = i == c ? 1.0 : 0.0;
+            for (int k = i + 1; k <= c; ++k) v -= R[i][k] * Ri[k][c];
+            Ri[i][c] = v / R[i][i];
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < K * K; i += kBlock) rinv[i] = Ri[i / K][i % K];
+}
+
+// M <- M R^-1, row by row in FP64, rounded once to fp32.
+__global__ __launch_bounds__(kBlock) void k_cholqr_apply(float* __restrict__ M, int64_t n, int K,
+                                                         const double* __restrict__ rinv) {
+    __shared__ double Ri[kCholQrMaxK * kCholQrMaxK];
+    for (int i = threadIdx.x; i < K * K; i += kBlock) Ri[i] = rinv[i];
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    float q[kCholQrMaxK];
+    for (int c = 0; c < K; ++c) q[c] = M[r * K + c];
+    for (int c = 0; c < K; ++c) {
+        double v = 0.0;
+        for (int p = 0; p <= c; ++p) v = __builtin_fma((double)q[p], Ri[p * K + c], v);
+        M[r * K + c] = (float)v;
+    }
+}
+
 unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 unsigned parts_for(int64_t n) {
     const int64_t b = (n + kBlock - 1) / kBlock;
@@ -662,6 +797,33 @@ void launch_gram_schmidt(float* C, float* w, int64_t n, int64_t K, float* part, 
                            (int)p, rng_mult);
     }
     check_launch("gram-schmidt launch");
+}
+
+int64_t cholqr_parts(int64_t n) {
+    return std::max<int64_t>(1, std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
+}
+
+int64_t cholqr_workspace_doubles(int64_t n, int64_t K) {
+    return (cholqr_parts(n) + 1) * (K * (K + 1) / 2) + K * K;
+}
+
+void launch_cholqr(float* M, int64_t n, int64_t K, double* work, int* degenerate,
+                   hipStream_t s) {
+    if (n <= 0 || K <= 0) return;
+    if (K > kCholQrMaxK) throw std::runtime_error("cholesky QR: too many columns");
+    const int64_t parts = cholqr_parts(n), pairs = K * (K + 1) / 2;
+    double* part = work;
+    double* gram = work + parts * pairs;
+    double* rinv = gram + pairs;
+    hipLaunchKernelGGL(k_gram_partial, dim3((unsigned)parts), dim3(kBlock), 0, s, M, n, (int)K,
+                       part);
+    hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)pairs), dim3(kBlock), 0, s, part,
+                       (int)parts, (int)pairs, gram);
+    hipLaunchKernelGGL(k_cholqr_factor, dim3(1), dim3(kBlock), 0, s, gram, (int)K, rinv,
+                       degenerate);
+    hipLaunchKernelGGL(k_cholqr_apply, dim3(blocks_for(n)), dim3(kBlock), 0, s, M, n, (int)K,
+                       rinv);
+    check_launch("cholesky QR launch");
 }
 
 int64_t gram_schmidt_part_floats(int64_t n, int64_t K) {

@@ -143,6 +143,21 @@ def test_pseudotime_matches_restatement(gpu):
     assert np.all((got >= 0) & (got <= 1)) and got[77] == 0.0
 
 
+def test_pseudotime_degenerate_basis_falls_back(gpu):
+    """The components are orthonormalised by a Cholesky QR; a degenerate basis
+    (an adjacency with no nonzeros: every product is zero) is redone with
+    Gram-Schmidt, whose zero columns become the normalised FastRNG(c * 12345)
+    directions, as in the reference."""
+    n = 20_000
+    ip, ix, v = np.zeros(n + 1, np.int64), np.zeros(0, np.int64), np.zeros(0, np.float32)
+    A = CsrMatrix.from_arrays(ip, ix, v, (n, n))
+    for root, K in ((0, 3), (777, 5)):
+        got = callers.pseudotime_diffusion(A, root, K)
+        ref = orc.pseudotime_f32(ip, ix, v, root, K, 100)
+        assert np.abs(got - ref).max() < 1e-5, (root, K)
+        assert got[root] == 0.0 and got.max() == 1.0
+
+
 def test_pseudotime_reference_cases(gpu):
     # test_pseudotime.cpp:213-248: path and ring of 5 cells with 10
     # components (more than the cells: the extra columns are rounding noise)

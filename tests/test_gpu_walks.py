@@ -163,6 +163,29 @@ def test_diffusion_map_embedding_wide(gpu):
         assert np.abs(G - np.eye(K)).max() < 1e-5
 
 
+def test_diffusion_map_cholesky_qr_and_its_fallbacks(gpu):
+    """From 16,384 rows the basis is orthonormalised by a Cholesky QR (FP64
+    Gram matrix and factor); more than 22 components take the Gram-Schmidt
+    path, and a degenerate basis (here an operator with no nonzeros, whose
+    products are exactly zero) is redone with Gram-Schmidt, which turns its
+    zero columns into the unit vectors e_c as the reference does."""
+    g, P = op(30_000, 10, seed=9)
+    for K, iters in ((23, 2), (22, 3)):
+        got = diffusion.diffusion_map_embedding(P, K, iters)
+        ref = orc.dmap_f32(g.indptr, g.indices, g.data, K, iters)
+        ref64 = orc.dmap_f64(g.indptr, g.indices, g.data, K, iters)
+        eg, ec = orc.rel_err(got, ref64), orc.rel_err(ref, ref64)
+        print(f"\ndiffusion map 30k K={K}: GPU {eg:.2e}, restatement {ec:.2e} vs FP64")
+        assert eg <= 2 * ec + 1e-7, K
+    n = 20_000
+    E = CsrMatrix.from_arrays(np.zeros(n + 1, np.int64), np.zeros(0, np.int64),
+                              np.zeros(0, np.float32), (n, n))
+    got = diffusion.diffusion_map_embedding(E, 4, 3)
+    assert np.array_equal(got, np.eye(n, 4, dtype=np.float32))
+    assert np.array_equal(got, orc.dmap_f32(np.zeros(n + 1, np.int64), np.zeros(0, np.int64),
+                                            np.zeros(0, np.float32), 4, 3))
+
+
 @pytest.mark.parametrize("n", [300, 1100])
 def test_diffusion_distance(gpu, n):
     g, P = op(n, 6, seed=n, sym=False)
