@@ -336,8 +336,19 @@ def leg_mp(steps: int) -> dict:
     row_shape = torch_ops.get_shape(P, F)
     Yr = torch_ops.spmm(P, X)
     row_ms = timed(lambda: torch_ops.spmm(P, X), steps, stream)
+    # FP64 oracle on 4 columns (outside timing), in the max-norm convention
+    from oracle import oracle as orc
+    ecols = [0, 1, 31, 63]
+    Xh = X[:, ecols].cpu().numpy()
+    y64 = orc.diffuse_f64(ip, cols, vals, Xh, 1)
+
+    def err64(Y):
+        Yh = Y[:, ecols].cpu().numpy().astype(np.float64)
+        return float(np.abs(Yh - y64).max() / np.abs(y64).max())
+
     out = {"workload": "pareto_1m_f64", "nnz": int(ip[-1]), "max_row": int(deg.max()),
-           "rowsplit_ms": round(row_ms, 3), "rowsplit_shape": row_shape}
+           "rowsplit_ms": round(row_ms, 3), "rowsplit_shape": row_shape,
+           "rowsplit_err_vs_fp64": err64(Yr), "error_columns": ecols}
     best = None
     for m in (1, 3, 5):
         torch_ops.set_shape(P, F, 4, 64, 8, merge=m)
@@ -346,6 +357,7 @@ def leg_mp(steps: int) -> dict:
         err = float((Ym - Yr).abs().max()) / float(Yr.abs().max())
         out[f"merge{m}_ms"] = round(ms, 3)
         out[f"merge{m}_max_rel_diff"] = err
+        out[f"merge{m}_err_vs_fp64"] = err64(Ym)
         if best is None or ms < best[0]:
             best = (ms, m)
     out["speedup_merge_vs_rowsplit"] = round(row_ms / best[0], 2)
