@@ -86,13 +86,30 @@ SpmmShape choose_shape(scl_sparse_matrix* h, DeviceCSR& d, bool transposed, int6
     return spmm_default_shape(F, X, Y);
 }
 
+// Whether the iterates between the first and the last of t steps are kept
+// slab-planar: when F floats per row leave rows off the 128-byte lines and
+// the shape reads planar operands, a full slab's row is then line-aligned
+// (one gather touches 8 lines of 128 bytes instead of 9).  The first step
+// reads X row-major and the last writes Y row-major; results are the same
+// bit for bit.
+bool planar_iterates(const CsrView& P, const SpmmShape& shape, int64_t F, int64_t t) {
+    return t >= 2 && (F * 4) % 128 != 0 && F > 64 * shape.vec &&
+           spmm_layout_supported(shape, P);
+}
+
+int step_layout(bool planar, int64_t step, int64_t t) {
+    if (!planar) return 0;
+    return (step > 1 ? 1 : 0) | (step < t ? 2 : 0);
+}
+
 // t steps between two device buffers, starting from buf[start]; returns the
 // index of the buffer holding the result.
 int pingpong(const CsrView& P, const SpmmShape& shape, float* buf[2], int start,
              int64_t F, int64_t t, hipStream_t s) {
+    const bool planar = planar_iterates(P, shape, F, t);
     int cur = start;
-    for (int64_t step = 0; step < t; ++step) {
-        launch_spmm_f32(P, buf[cur], buf[cur ^ 1], F, shape, s);
+    for (int64_t step = 1; step <= t; ++step) {
+        launch_spmm_f32(P, buf[cur], buf[cur ^ 1], F, shape, s, step_layout(planar, step, t));
         cur ^= 1;
     }
     return cur;
@@ -322,10 +339,11 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
         hipStream_t s = static_cast<hipStream_t>(stream);
         // step s (1-based) writes Y when t - s is even, else work, so the
         // last step lands in Y and consecutive steps alternate buffers
+        const bool planar = planar_iterates(v, sh, n_features, n_steps);
         const float* src = X_dev;
         for (int64_t step = 1; step <= n_steps; ++step) {
             float* dst = ((n_steps - step) % 2 == 0) ? Y_dev : work_dev;
-            launch_spmm_f32(v, src, dst, n_features, sh, s);
+            launch_spmm_f32(v, src, dst, n_features, sh, s, step_layout(planar, step, n_steps));
             src = dst;
         }
         SCL_API_RETURN_OK;

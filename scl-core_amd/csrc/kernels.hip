@@ -332,7 +332,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int32_t* __restrict__ gptr, const uint32_t* __restrict__ gword,
     const int32_t* __restrict__ gvptr, const float* __restrict__ gval,
     const int32_t* __restrict__ grow, const float* __restrict__ X, float* __restrict__ Y,
-    int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd, int32_t rb_per_block) {
+    int32_t n_groups, int32_t F, int32_t n_gblocks, int32_t rb_per_xcd, int32_t rb_per_block,
+    int32_t layout, int32_t n_rows) {
     static_assert(U * R <= 128, "a chunk's values must fit two per lane");
     using VT = typename Vec<VEC>::T;
     int slab, gb;
@@ -340,9 +341,15 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int lane = threadIdx.x & 63;
     const int g = gb * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (g >= n_groups) return;
-    const int f = slab * (64 * VEC) + lane * VEC;
-    const bool active = f < F;
-    const float* Xf = X + (active ? f : 0);
+    // slab-planar operands (layout bit 0: X, bit 1: Y) keep slab s of all
+    // rows contiguous, n_rows x ws floats from s * 64 * VEC * n_rows, so a
+    // full slab's row starts on a 64 * VEC * 4-byte boundary
+    const int f0 = slab * (64 * VEC), fl = lane * VEC;
+    const int ws = min(64 * VEC, F - f0);
+    const bool active = fl < ws;
+    const size_t plane = (size_t)f0 * (size_t)n_rows;
+    const float* Xf = (layout & 1) ? X + plane + (active ? fl : 0) : X + f0 + (active ? fl : 0);
+    const size_t xs = (layout & 1) ? (size_t)ws : (size_t)F;
     const int e1 = __builtin_amdgcn_readfirstlane(gptr[g + 1]);
     int e = __builtin_amdgcn_readfirstlane(gptr[g]);
     int vo = __builtin_amdgcn_readfirstlane(gvptr[g]);
@@ -366,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
         VT x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * F);
+            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * xs);
         int c = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -389,7 +396,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int row = __builtin_amdgcn_readfirstlane(grow[(size_t)g * R + r]);
-        if (row >= 0 && active) store_vec<VEC, NT>(Y + (size_t)row * F + f, acc[r]);
+        float* y = (layout & 2) ? Y + plane + (size_t)row * ws + fl : Y + (size_t)row * F + f0 + fl;
+        if (row >= 0 && active) store_vec<VEC, NT>(y, acc[r]);
     }
 }
 
@@ -824,7 +832,8 @@ namespace {
 
 template <int VEC, int R, int U, bool NT>
 void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F,
-                        const SpmmShape& shape, int lds, hipStream_t s) {
+                        const SpmmShape& shape, int lds, int layout, int32_t n_rows,
+                        hipStream_t s) {
     const int64_t n_gblocks = (G.n_groups + kBlock / 64 - 1) / (kBlock / 64);
     const int64_t n_slabs = (F + 64 * VEC - 1) / (64 * VEC);
     int64_t rb_per_xcd, rbb, nb;
@@ -832,7 +841,7 @@ void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F
     hipLaunchKernelGGL((k_spmm_grouped<VEC, R, U, NT>), dim3((unsigned)nb), dim3(kBlock),
                        (unsigned)lds, s, G.gptr, G.gword, G.gvptr, G.gval, G.grow, X, Y,
                        (int32_t)G.n_groups, (int32_t)F, (int32_t)n_gblocks,
-                       (int32_t)rb_per_xcd, (int32_t)rbb);
+                       (int32_t)rb_per_xcd, (int32_t)rbb, (int32_t)layout, n_rows);
 }
 
 // ---------------------------------------------------- LDS-staged groups --
@@ -1179,7 +1188,7 @@ void launch_spmm_tiled(const CsrView& P, const float* X, float* Y, int64_t F,
 }
 
 void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
-                         const SpmmShape& shape, hipStream_t s) {
+                         const SpmmShape& shape, int layout, hipStream_t s) {
     const int R = shape.rows_per_group;
     const GroupedCsr* G = P.grouped ? &P.grouped[R == 8 ? 1 : 0] : nullptr;
     if (!G || G->R != R || G->n_groups * R < P.n)
@@ -1188,8 +1197,8 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
     // U entries per chunk with U * R <= 128 (a chunk's values fit two per lane)
 #define SCL_GROUPED(V, RR, UU)                                                          \
     (shape.nontemporal                                                                  \
-         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape, lds, s)              \
-         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape, lds, s))
+         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape, lds, layout, P.n, s) \
+         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape, lds, layout, P.n, s))
 #define SCL_GROUPED_V(RR, UU)                          \
     do {                                               \
         if (shape.vec == 4) SCL_GROUPED(4, RR, UU);    \
@@ -1228,9 +1237,20 @@ void launch_spmv_staged(const CsrView& P, const int32_t* out_row, const float* x
     check_launch("spmv launch");
 }
 
+bool spmm_layout_supported(const SpmmShape& shape, const CsrView& P) {
+    return shape.rows_per_group > 0 && shape.rows_per_group != kStagedRows &&
+           shape.rows_per_group != kTiledRows && !shape.merge && P.n == P.n_sec;
+}
+
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
-                     const SpmmShape& shape, hipStream_t s) {
+                     const SpmmShape& shape, hipStream_t s, int layout) {
     if (P.n <= 0 || F <= 0) return;
+    if (layout != 0) {
+        if (!spmm_layout_supported(shape, P))
+            throw std::runtime_error("slab-planar operands need the row-grouped shapes");
+        launch_spmm_grouped(P, X, Y, F, shape, layout, s);
+        return;
+    }
     if (shape.merge > 0) {
         launch_spmm_merge(P, X, Y, F, shape, s);
         return;
@@ -1244,7 +1264,7 @@ void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
         return;
     }
     if (shape.rows_per_group > 0) {
-        launch_spmm_grouped(P, X, Y, F, shape, s);
+        launch_spmm_grouped(P, X, Y, F, shape, 0, s);
         return;
     }
     if (F == 1 && shape.group == 1 && shape.vec == 1 && P.n_sec < (1 << 30)) {

@@ -172,8 +172,15 @@ std::vector<SpmmShape> spmm_candidate_shapes(int64_t F, const void* X, const voi
                                              bool has_schedule);
 
 // One step Y = P X (Y[i,:] = sum over row i, stored order, FMA in fp32).
+// layout bit 0 / bit 1: X / Y are slab-planar: slab s (features s * 64 * vec
+// onward, ws = min(64 * vec, F - s * 64 * vec) of them) of all n rows is an
+// n x ws row-major block starting at float s * 64 * vec * n; the same n * F
+// floats as row-major, with every full slab's rows aligned to its width.
+// Only the row-grouped shapes on square operators take a non-zero layout
+// (spmm_layout_supported).
 void launch_spmm_f32(const CsrView& P, const float* X, float* Y, int64_t F,
-                     const SpmmShape& shape, hipStream_t s);
+                     const SpmmShape& shape, hipStream_t s, int layout = 0);
+bool spmm_layout_supported(const SpmmShape& shape, const CsrView& P);
 
 // One fused backward step over the transposed pattern PT (rows = columns j
 // of P, entries = source rows i in increasing order, values p_ij):

@@ -161,6 +161,32 @@ def test_single_feature_windows(gpu, unroll):
         assert np.array_equal(got, ref), (unroll, tr)
 
 
+@pytest.mark.parametrize("F", [260, 300, 1000])
+def test_slab_planar_iterates(gpu, F):
+    """Wide operators keep the iterates between the first and the last step
+    slab-planar when F floats per row leave rows off the 128-byte lines (the
+    row-grouped default shape): every step's output is still the stored-order
+    chain, so Y equals the restatement for t = 1..4, through the device entry
+    point and the host-array drop-in, with a partial last slab of 4, 44 and
+    232 features."""
+    import torch
+    from scl_amd import torch_ops
+    g = synth.knn_graph(20_000, 10, seed=F).normalized()
+    n = g.n
+    P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
+    X = synth.normal(n, F, seed=F)
+    Xd = torch.from_numpy(X).cuda()
+    cols = np.unique(np.r_[0, 1, 127, 128, 255, 256, F - 1, F // 2])
+    for t in (1, 2, 3, 4):
+        Y = torch_ops.diffuse(P, Xd, t).cpu().numpy()
+        assert torch_ops.get_shape(P, F)["rows_per_group"] == 4
+        ref = orc.diffuse_matrix_f32(g.indptr, g.indices, g.data, X[:, cols], t)
+        assert np.array_equal(Y[:, cols], ref), t
+    Yh = X.copy()
+    diffusion.diffuse_matrix(P, Yh, 3)
+    assert np.array_equal(Yh, torch_ops.diffuse(P, Xd, 3).cpu().numpy())
+
+
 def test_no_nonzeros_and_single_node(gpu):
     Y = gpu_diffuse(np.zeros(6, np.int64), np.zeros(0, np.int64),
                     np.zeros(0, np.float32), np.ones((5, 3), np.float32), 2)
