@@ -50,15 +50,16 @@ constexpr int64_t kGroupedMinRows = 4096;
 
 // The measured-best forward shape of wide operators (n >= kGroupedMinRows,
 // F >= 128): row groups of 4 (vec 4, 16 union entries per chunk,
-// non-temporal stores, at most 4 workgroups per CU).  At n = 200,000, nnz
-// 8.9 M, F = 2000 it is the shape the autotuner picks.
+// non-temporal stores, no occupancy cap).  At n = 200,000, nnz 8.9 M,
+// F = 2000 with slab-planar iterates it runs 2.98 ms per step (a cap of 4
+// workgroups per CU: 3.08 ms; caps of 6 and 7: 2.98 ms).
 SpmmShape wide_default_shape() {
     SpmmShape g;
     g.vec = 4;
     g.group = 64;
     g.unroll = 16;
     g.rows_per_group = 4;
-    g.wg_per_cu = 4;
+    g.wg_per_cu = 0;
     g.nontemporal = true;
     return g;
 }
@@ -595,12 +596,16 @@ SCL_EXPORT scl_error_t scl_gpu_spmm_autotune(scl_sparse_t transition,
         }
         SCL_API_CHECK(!cands.empty(), SCL_ERROR_INVALID_ARGUMENT,
                       "no launch shape fits these buffers");
+        // forward shapes are timed as the inner steps of a diffusion run
+        // them: slab-planar in and out where the diffusion keeps its
+        // iterates so (Y is scratch here; every address stays in the buffers)
         auto run = [&](const SpmmShape& c) {
             if (transpose == 2)
                 launch_adjoint_edge(v, X_dev, dpt ? X_dev : nullptr, Y_dev, dpt, false,
                                     n_features, s, &c);
             else
-                launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s);
+                launch_spmm_f32(v, X_dev, Y_dev, n_features, c, s,
+                                transpose == 0 && planar_iterates(v, c, n_features, 3) ? 3 : 0);
         };
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
