@@ -67,6 +67,18 @@ def test_renumbered_products_follow_value_changes(gpu):
     assert np.array_equal(got, orc.lazy_walk_f32(ip, ix, v, x, 3, 0.3))
 
 
+def test_renumbered_products_of_a_csc_handle(gpu):
+    """A CSC handle walks its primary slots (columns) as rows, as the
+    reference's visit() does; its renumbered copy is built from the same
+    slots, so a walk on it equals the restatement on those arrays."""
+    from scl_amd import CscMatrix
+    g = synth.knn_graph(20_000, 10, seed=12).normalized()
+    C = CscMatrix.from_arrays(g.indptr, g.indices, g.data, (g.n, g.n))
+    x = synth.normal(g.n, 1, seed=13)[:, 0]
+    got = diffusion.lazy_random_walk(C, x.copy(), 4, 0.25)
+    assert np.array_equal(got, orc.lazy_walk_f32(g.indptr, g.indices, g.data, x, 4, 0.25))
+
+
 def test_random_walk_with_restart(gpu):
     g, P = op(sym=False)
     seeds = [5, 17, 17, -4, 10 ** 9]
