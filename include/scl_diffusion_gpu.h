@@ -39,7 +39,9 @@ SCL_EXPORT scl_error_t scl_gpu_sparse_prepare(scl_sparse_t matrix);
 
 /* Y_dev = P^t X_dev.  X_dev is not modified; Y_dev must not overlap X_dev.
  * work_dev (n_nodes x n_features floats, overlapping neither) is required
- * when n_steps >= 2 and ignored otherwise.  n_steps >= 1. */
+ * when n_steps >= 2 and ignored otherwise; on return it holds an
+ * intermediate iterate in an unspecified layout (wide operators keep the
+ * iterates between the first and last step slab-planar).  n_steps >= 1. */
 SCL_EXPORT scl_error_t scl_gpu_diffuse_matrix_device(
     scl_sparse_t transition, const scl_real_t* X_dev, scl_real_t* Y_dev,
     scl_real_t* work_dev, scl_index_t n_nodes, scl_index_t n_features,
