@@ -568,9 +568,28 @@ __global__ __launch_bounds__(kBlock) void k_absmax_partial(const float* __restri
                                                            float* __restrict__ partial) {
     __shared__ float sh[kBlock / 64];
     float v = 0.0f;
-    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < m;
-         p += (int64_t)gridDim.x * kBlock)
-        v = fmaxf(v, fabsf(x[p]));
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        // four 16-byte loads in flight per thread, then the tail
+        const f4* x4 = reinterpret_cast<const f4*>(x);
+        const int64_t m4 = m / 4;
+        for (; p + 3 * stride < m4; p += 4 * stride) {
+            f4 a[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = x4[p + u * stride];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                v = fmaxf(v, fmaxf(fmaxf(fabsf(a[u].x), fabsf(a[u].y)),
+                                   fmaxf(fabsf(a[u].z), fabsf(a[u].w))));
+        }
+        for (; p < m4; p += stride) {
+            const f4 a = x4[p];
+            v = fmaxf(v, fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w))));
+        }
+        p = m4 * 4 + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    }
+    for (; p < m; p += stride) v = fmaxf(v, fabsf(x[p]));
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
