@@ -116,6 +116,27 @@ struct Session {
     void renumber_rows(const float* src, float* dst, int64_t F, bool back) {
         launch_rows_f32(d->sched, src, dst, v.n, F, back, s);
     }
+    // Whether a walk keeps its vectors in the locality numbering: the
+    // default single-vector shape on an operator with a renumbered copy.
+    // Its products are then spmv_renumbered(); the element-wise steps and
+    // the maxima of a walk do not depend on the order.
+    bool walk_renumbered() {
+        const auto it = d->shapes.find(1);
+        if (it != d->shapes.end() &&
+            (it->second.group != 1 || it->second.vec != 1 || it->second.merge ||
+             it->second.rows_per_group))
+            return false;
+        return v.n_sec < (1 << 30) && renumbered();
+    }
+    // y = P x with x and y in the locality numbering
+    void spmv_renumbered(const float* x, float* y) {
+        CsrView r = v;
+        r.row_ptr = d->ren.row_ptr;
+        r.col = d->ren.col;
+        r.val = d->ren.val;
+        const auto it = d->shapes.find(1);
+        launch_spmv_staged(r, nullptr, x, y, it != d->shapes.end() ? it->second.unroll : 8, s);
+    }
     // Y = P' X for the operator's pattern with other values (same order as
     // the stored ones).
     void spmm_values(const float* val, const float* X, float* Y, int64_t F) {

@@ -49,12 +49,22 @@ constexpr int64_t kWideGramSchmidtRows = 16384;
 constexpr int64_t kStepsPerCheck = 8;
 
 // compute_dpt (diffusion.hpp:670-768) on device buffers; pt is device.
+// With a renumbered operator the walk runs in its locality numbering (the
+// maxima it reduces do not depend on the order) and pt is put back at the end.
 void dpt_device(Session& S, int64_t root, float* pt, int64_t max_iter, float tol) {
     const int64_t n = S.n;
+    const bool ren = S.walk_renumbered();
     DevBuf p(n), pn(n), hit(n), vis(n), part(1024), scal(3);
     float* done = scal.get() + 2;
-    launch_set_unit(p.get(), n, root, 1.0f, S.s);
-    launch_set_unit(vis.get(), n, root, 1.0f, S.s);
+    launch_set_unit(ren ? pn.get() : p.get(), n, root, 1.0f, S.s);
+    if (ren) {
+        S.renumber_rows(pn.get(), p.get(), 1, false);
+        hip_check(hipMemcpyAsync(vis.get(), p.get(), (size_t)n * 4, hipMemcpyDeviceToDevice,
+                                 S.s),
+                  "copy");
+    } else {
+        launch_set_unit(vis.get(), n, root, 1.0f, S.s);
+    }
     launch_set_unit(hit.get(), n, -1, 0.0f, S.s);
     launch_fill(done, 1, 0.0f, S.s);
     float* cur = p.get();
@@ -63,14 +73,17 @@ void dpt_device(Session& S, int64_t root, float* pt, int64_t max_iter, float tol
         // the step whose max(1 - visited) falls below tol sets the flag; the
         // later steps of the batch leave visited and hit as they are
         for (const int64_t end = std::min(max_iter, it + kStepsPerCheck - 1); it <= end; ++it) {
-            S.spmv(cur, nxt);
+            if (ren) S.spmv_renumbered(cur, nxt);
+            else S.spmv(cur, nxt);
             launch_dpt_update(nxt, vis.get(), hit.get(), n, (float)it, part.get(), scal.get(),
                               tol, done, S.s);
             std::swap(cur, nxt);
         }
         if (S.scalar(done) != 0.0f) break;
     }
-    launch_dpt_finish(vis.get(), hit.get(), pt, n, part.get(), scal.get() + 1, S.s);
+    launch_dpt_finish(vis.get(), hit.get(), ren ? nxt : pt, n, part.get(), scal.get() + 1,
+                      S.s);
+    if (ren) S.renumber_rows(nxt, pt, 1, true);
 }
 
 }  // namespace
@@ -93,13 +106,19 @@ SCL_EXPORT scl_error_t scl_diffusion_lazy_random_walk(scl_sparse_t transition, s
         if (n > 0) {
             const float lazy = laziness == 0 ? kDefaultLaziness : laziness;
             const float move = 1.0f - lazy;
+            // in the locality numbering when the operator has a renumbered
+            // copy (the blend is element-wise)
+            const bool ren = S.walk_renumbered();
             DevBuf xd(n), t(n);
-            S.upload(xd.get(), x, n);
+            S.upload(ren ? t.get() : xd.get(), x, n);
+            if (ren) S.renumber_rows(t.get(), xd.get(), 1, false);
             for (int64_t st = 0; st < n_steps; ++st) {
-                S.spmv(xd.get(), t.get());
+                if (ren) S.spmv_renumbered(xd.get(), t.get());
+                else S.spmv(xd.get(), t.get());
                 launch_blend(xd.get(), t.get(), n, lazy, move, S.s);
             }
-            S.download(x, xd.get(), n);
+            if (ren) S.renumber_rows(xd.get(), t.get(), 1, true);
+            S.download(x, ren ? t.get() : xd.get(), n);
         }
         SCL_API_RETURN_OK;
     SCL_API_CATCH
@@ -119,14 +138,19 @@ SCL_EXPORT scl_error_t scl_diffusion_heat_kernel_signature(scl_sparse_t transiti
         if (n > 0) {
             const int64_t steps = n_steps == 0 ? kDefaultHksSteps : n_steps;
             const float dt = t / (float)steps, keep = 1.0f - dt;
+            // in the locality numbering when the operator has a renumbered
+            // copy (the start is all ones, the blend element-wise)
+            const bool ren = S.walk_renumbered();
             DevBuf sig(n), tmp(n);
             std::vector<float> ones((size_t)n, 1.0f);
             S.upload(sig.get(), ones.data(), n);
             for (int64_t st = 0; st < steps; ++st) {
-                S.spmv(sig.get(), tmp.get());
+                if (ren) S.spmv_renumbered(sig.get(), tmp.get());
+                else S.spmv(sig.get(), tmp.get());
                 launch_blend(sig.get(), tmp.get(), n, keep, dt, S.s);
             }
-            S.download(signature, sig.get(), n);
+            if (ren) S.renumber_rows(sig.get(), tmp.get(), 1, true);
+            S.download(signature, ren ? tmp.get() : sig.get(), n);
         }
         SCL_API_RETURN_OK;
     SCL_API_CATCH
@@ -165,10 +189,17 @@ SCL_EXPORT scl_error_t scl_diffusion_random_walk_with_restart(
             }
             const float inv = 1.0f / (float)cnt;
             for (float& r : restart) r *= inv;
+            // in the locality numbering when the operator has a renumbered
+            // copy: the update is element-wise; its convergence total sums
+            // in that order
+            const bool ren = S.walk_renumbered();
             DevBuf out(n), other(n), rs(n), part(1024), total(2);
             float* done = total.get() + 1;
-            S.upload(rs.get(), restart.data(), n);
-            S.upload(out.get(), restart.data(), n);
+            S.upload(ren ? other.get() : rs.get(), restart.data(), n);
+            if (ren) S.renumber_rows(other.get(), rs.get(), 1, false);
+            hip_check(hipMemcpyAsync(out.get(), rs.get(), (size_t)n * 4,
+                                     hipMemcpyDeviceToDevice, S.s),
+                      "copy");
             launch_fill(done, 1, 0.0f, S.s);
             const float beta = 1.0f - a;
             // out is the caller's buffer in the reference; cur/nxt swap as its
@@ -177,7 +208,8 @@ SCL_EXPORT scl_error_t scl_diffusion_random_walk_with_restart(
             float* nxt = other.get();
             for (int64_t it = 0; it < iters;) {
                 for (const int64_t end = std::min(iters, it + kStepsPerCheck); it < end; ++it) {
-                    S.spmv(cur, nxt);
+                    if (ren) S.spmv_renumbered(cur, nxt);
+                    else S.spmv(cur, nxt);
                     launch_rwr_update(nxt, cur, rs.get(), n, a, beta, part.get(), total.get(),
                                       tl, done, S.s);
                     std::swap(cur, nxt);
@@ -192,7 +224,8 @@ SCL_EXPORT scl_error_t scl_diffusion_random_walk_with_restart(
                     break;
                 }
             }
-            S.download(scores, out.get(), n);
+            if (ren) S.renumber_rows(out.get(), rs.get(), 1, true);
+            S.download(scores, ren ? rs.get() : out.get(), n);
         }
         SCL_API_RETURN_OK;
     SCL_API_CATCH

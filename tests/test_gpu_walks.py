@@ -79,6 +79,19 @@ def test_renumbered_products_of_a_csc_handle(gpu):
     assert np.array_equal(got, orc.lazy_walk_f32(g.indptr, g.indices, g.data, x, 4, 0.25))
 
 
+def test_renumbered_dpt(gpu):
+    """From 16,384 rows DPT runs in the operator's locality numbering (its
+    reductions are maxima, which do not depend on the order): bit-identical
+    to the restatement, one root and several."""
+    g, P = op(20_000, 10, seed=14, sym=False)
+    for root, iters, tol in ((3, 40, 1e-6), (19_999, 12, 1e-2)):
+        got = diffusion.compute_dpt(P, root, iters, tol)
+        assert np.array_equal(got, orc.dpt_f32(g.indptr, g.indices, g.data, root, iters, tol))
+    multi = diffusion.compute_dpt_multi_root(P, [5, 70, -2], 20)
+    assert np.array_equal(multi, orc.dpt_multi_f32(g.indptr, g.indices, g.data, [5, 70, -2],
+                                                   20))
+
+
 def test_random_walk_with_restart(gpu):
     g, P = op(sym=False)
     seeds = [5, 17, 17, -4, 10 ** 9]
