@@ -346,6 +346,32 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion(
 SCL_EXPORT scl_error_t scl_gpu_plan_launch(scl_gpu_plan_t plan, void* stream);
 SCL_EXPORT scl_error_t scl_gpu_plan_destroy(scl_gpu_plan_t* plan);
 
+/* Batched plan (no reference counterpart): the same sequence for every graph
+ * of a block-diagonal stack (scl_sparse_block_diag) in ONE persistent kernel
+ * launch, recorded into a hipGraph: t forward steps, dY = Y - T, t fused
+ * adjoint + edge-gradient steps and the permutation of dP to CSR order, each
+ * graph's steps ordered by the graph's own progress rather than by launch
+ * boundaries.  block_rows[g] are the rows of block g (they add up to the
+ * operator's rows, and no stored entry leaves its block: refused otherwise).
+ * Each graph runs on one XCD (bin-packed by rows + nonzeros); rows_per_item
+ * rows form one work item (0: 32) and the queue of an XCD walks its graphs
+ * graphs_per_chunk at a time through all their steps (0: all of them, step
+ * by step).  n_features % 32 == 0, 32..512; X, Y, T, dY, dX 128-byte
+ * aligned.  dX_dev null: forward only (T, dY unused); dP_dev null: no edge
+ * gradient.  Results are bit-identical to scl_gpu_plan_diffusion with every
+ * = 1 and the default adjoint shape.  The plan allocates its own
+ * intermediates ((t - 1) planes of iterates, (t - 1) adjoint planes, t dP
+ * partials).  Launch with scl_gpu_plan_launch. */
+SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
+    scl_sparse_t transition, const scl_index_t* block_rows, scl_size_t n_blocks,
+    const scl_real_t* X_dev, scl_real_t* Y_dev, const scl_real_t* T_dev, scl_real_t* dY_dev,
+    scl_real_t* dX_dev, scl_real_t* dP_dev, scl_index_t n_features, scl_index_t n_steps,
+    scl_index_t rows_per_item, scl_index_t graphs_per_chunk, scl_gpu_plan_t* out);
+/* Waits for the stream, then reports whether the last launch of a batched
+ * plan completed every work item (SCL_ERROR_INTERNAL with the reason if a
+ * wait gave up or an XCD's queue was never served); SCL_OK for other plans. */
+SCL_EXPORT scl_error_t scl_gpu_plan_check(scl_gpu_plan_t plan, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,229 @@
+// adjoint_row.hpp -- the fused adjoint + edge-gradient step of one row of
+// the transposed pattern, shared by the per-step kernel (grad.hip) and the
+// persistent batched kernel (persist.hip), so both produce the same bits.
+//
+// For transposed row j (a column of P; entries = source rows i in
+// increasing order, values p_ij) and a feature range [f0, f0 + Fp):
+//   Gp[j, :]  = sum_i p_ij Gs[i, :]                 (fp32 FMA, stored order)
+//   ddst[p]   = (accumulate ? dsrc[p] : 0) + <Gs[i, :], Xprev[j, :]>
+// One gather of Gs[i, slab] feeds both; Xprev[j, :] is loaded once.  The U
+// partial dot products of a chunk are reduce-scattered across the row's G
+// lanes, and each writer lane stores one entry.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace scl_amd {
+namespace dev {
+
+__device__ __forceinline__ float dot4(f4 a, f4 b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+__device__ __forceinline__ float dotv(f4 a, f4 b) { return dot4(a, b); }
+__device__ __forceinline__ float dotv(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ float dotv(float a, float b) { return a * b; }
+
+// Cross-lane sums.  Inside a 16-lane row, stage b pairs lane li with
+// li ^ (2^(b+1) - 1) through DPP (quad_perm, row_half_mirror, row_mirror),
+// which costs no LDS traffic; lanes 16 and 32 apart are paired with
+// __shfl_xor.  Successive stages use linearly independent masks, so a chain
+// over all of a group's bits sums every lane of the group exactly once.
+template <int B>
+__device__ __forceinline__ float dpp_partner(float v) {
+    static_assert(B >= 0 && B < 4, "DPP stages cover bits 0..3");
+    constexpr int ctrl = B == 0 ? 0xB1 : B == 1 ? 0x1B : B == 2 ? 0x141 : 0x140;
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF,
+                                           false));
+}
+
+// Lanes with bit B clear end with a(self) + a(partner), lanes with bit B set
+// with b(self) + b(partner).
+template <int B>
+__device__ __forceinline__ float pair_reduce(float a, float b, bool hi) {
+    const float send = hi ? a : b;
+    const float keep = hi ? b : a;
+    return keep + dpp_partner<B>(send);
+}
+
+// Sum over bits B-1 .. 0 of the lane index (B <= 4).
+template <int B>
+__device__ __forceinline__ float sum_bits_below(float v) {
+    if constexpr (B > 0) return sum_bits_below<B - 1>(v + dpp_partner<B - 1>(v));
+    else return v;
+}
+
+template <int G>
+__device__ __forceinline__ constexpr int log2i() {
+    return G <= 1 ? 0 : 1 + log2i<G / 2>();
+}
+
+// Sum over lanes 16 and 32 apart, for groups wider than one row.
+template <int G>
+__device__ __forceinline__ float sum_across_rows(float v) {
+    if constexpr (G >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Sum over the G lanes of a group (G a power of two, groups aligned).
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+    constexpr int rb = log2i<G>() < 4 ? log2i<G>() : 4;
+    return sum_across_rows<G>(sum_bits_below<rb>(v));
+}
+
+template <int B, int C, int U>
+__device__ __forceinline__ void scatter_stages(float (&v)[U], int li) {
+    if constexpr (C > 1) {
+        const bool hi = ((li >> B) & 1) != 0;
+#pragma unroll
+        for (int q = 0; q < C / 2; ++q) v[q] = pair_reduce<B>(v[q], v[q + C / 2], hi);
+        scatter_stages<B - 1, C / 2, U>(v, li);
+    }
+}
+
+// Reduce-scatter of U partial dot products across a group of G lanes
+// (G >= U, both powers of two).  Within a row, the group's top log2(U) row
+// bits each halve the set of entries a lane carries; the remaining low bits
+// and the cross-row bits are then summed outright.  Lane li ends with the
+// total of entry scatter_entry(li).
+template <int G, int U>
+struct Scatter {
+    static constexpr int kRowBits = log2i<G>() < 4 ? log2i<G>() : 4;
+    static constexpr int kLow = kRowBits - log2i<U>();  // bits summed outright
+    static_assert(kLow >= 0, "reduce-scatter needs U <= min(G, 16)");
+    __device__ static int entry(int li) { return (li >> kLow) & (U - 1); }
+    __device__ static bool writer(int li) {
+        return (li & ((1 << kLow) - 1)) == 0 && (G <= 16 || (li >> 4) == 0);
+    }
+    __device__ static float reduce(float (&d)[U], int li) {
+        float v[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) v[q] = d[q];
+        scatter_stages<kRowBits - 1, U, U>(v, li);
+        return sum_across_rows<G>(sum_bits_below<kLow>(v[0]));
+    }
+};
+
+// One transposed row.  NC chunks of G*VEC features per lane group cover the
+// range; li is the lane's index in its group.  The source rows and weights
+// of the next U entries, and the running values of the current ones, are
+// fetched while the current gathers are in flight.  dsrc may equal ddst
+// (in-place accumulation).
+template <int VEC, int G, int U, int NC, bool EDGE>
+__device__ __forceinline__ void adjoint_edge_row(
+    const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
+    const float* __restrict__ t_val, const float* __restrict__ Gs,
+    const float* __restrict__ Xprev, float* __restrict__ Gp, const float* dsrc, float* ddst,
+    int accumulate, int32_t F, int32_t f0, int32_t Fp, int row, int li) {
+    using VT = typename Vec<VEC>::T;
+    const int beg = t_row_ptr[row], end = t_row_ptr[row + 1];
+
+    bool act[NC];
+    int off[NC];  // feature offset of chunk c for this lane (0 when inactive)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int f = c * (G * VEC) + li * VEC;
+        act[c] = f < Fp;
+        off[c] = act[c] ? f0 + f : 0;
+    }
+    VT xj[NC];
+    if constexpr (EDGE) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            xj[c] = *reinterpret_cast<const VT*>(Xprev + (size_t)row * F + off[c]);
+            if (!act[c]) xj[c] = VT{};
+        }
+    }
+    float acc[NC][VEC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) acc[c][e] = 0.0f;
+
+    // entry of the U-chunk this lane writes, and whether it writes at all
+    constexpr bool kScatter = G >= U;
+    using SC = Scatter<kScatter ? G : U, U>;
+    const int my_e = kScatter ? SC::entry(li) : 0;
+    const bool writer = kScatter ? SC::writer(li) : li == 0;
+
+    int i[U];
+    float w[U];
+    if (beg < end) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(beg + u, end - 1);
+            i[u] = t_col[kk];
+            w[u] = t_val[kk];
+        }
+    }
+    for (int k = beg; k < end; k += U) {
+        VT g[U][NC];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                g[u][c] = *reinterpret_cast<const VT*>(Gs + (size_t)i[u] * F + off[c]);
+        // running dP values of this chunk (kScatter: one per writer lane)
+        float old[kScatter ? 1 : U] = {};
+        if constexpr (EDGE) {
+            if (accumulate && writer) {
+                if constexpr (kScatter) {
+                    old[0] = dsrc[min(k + my_e, end - 1)];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) old[u] = dsrc[min(k + u, end - 1)];
+                }
+            }
+        }
+        float wc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) wc[u] = w[u];
+        // next chunk's indices (clamped into this row; unused past its end)
+        const int kn = k + U;
+        if (kn < end) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = min(kn + u, end - 1);
+                i[u] = t_col[kk];
+                w[u] = t_val[kk];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k + u < end) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) fma_vec(wc[u], g[u][c], acc[c]);
+            }
+        }
+        if constexpr (EDGE) {
+            float d[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float sdot = dotv(g[u][0], xj[0]);
+#pragma unroll
+                for (int c = 1; c < NC; ++c) sdot += dotv(g[u][c], xj[c]);
+                d[u] = sdot;
+            }
+            if constexpr (kScatter) {
+                const float r = SC::reduce(d, li);
+                if (writer && k + my_e < end) ddst[k + my_e] = accumulate ? old[0] + r : r;
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) d[u] = group_sum<G>(d[u]);
+                if (writer) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                        if (k + u < end) ddst[k + u] = accumulate ? old[u] + d[u] : d[u];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        if (act[c]) store_vec<VEC, false>(Gp + (size_t)row * F + off[c], acc[c]);
+}
+
+}  // namespace dev
+}  // namespace scl_amd

@@ -131,6 +131,9 @@ SIGNATURES = {
                 c_index, PP]),
     "scl_gpu_plan_launch": (c_err, [P, P]),
     "scl_gpu_plan_destroy": (c_err, [PP]),
+    "scl_gpu_plan_diffusion_batched": (
+        c_err, [P, P, c_size, P, P, P, P, P, P, c_index, c_index, c_index, c_index, PP]),
+    "scl_gpu_plan_check": (c_err, [P, P]),
     "scl_gpu_diffuse_backward_device": (
         c_err, [P, P, P, P, P, P, P, c_size, c_index, c_index, c_index, c_index, P]),
     "scl_gpu_diffuse_forward_ckpt_values_device": (

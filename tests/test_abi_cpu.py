@@ -320,6 +320,36 @@ def test_block_diag_stacks_handles():
     assert lib.scl_sparse_block_diag(arr, 0, ctypes.byref(out)) == 10
 
 
+def test_batched_plan_argument_checks_come_first():
+    """The persistent batched plan refuses a bad batch before any device work
+    (fake, never-dereferenced device pointers stand in for the buffers)."""
+    from scl_amd import batch
+    ip, ix, d = small_csr()
+    A = CsrMatrix.from_arrays(ip, ix, d, (3, 3))
+    S = batch.block_diag([A, A])
+    lib = _lib.load()
+    out = ctypes.c_void_p()
+    buf = 1 << 20  # 128-byte aligned stand-ins
+    rows = (ctypes.c_int64 * 2)(3, 3)
+
+    def plan(rows, n_blocks, X=buf, F=64, t=2, dX=buf):
+        return lib.scl_gpu_plan_diffusion_batched(
+            S.handle, rows, n_blocks, X, buf, buf, buf, dX, buf, F, t, 0, 0, ctypes.byref(out))
+
+    assert plan(rows, 2, F=50) == 10                    # not a multiple of 32
+    assert plan(rows, 2, F=1024) == 10                  # above 512
+    assert plan(rows, 2, X=buf + 4) == 10               # misaligned buffer
+    assert plan(rows, 2, t=0) == 10
+    assert plan(None, 2) == 4
+    assert plan((ctypes.c_int64 * 2)(3, 2), 2) == 11    # blocks do not cover the rows
+    assert plan((ctypes.c_int64 * 2)(2, 4), 2) == 10    # an entry leaves its block
+    assert plan((ctypes.c_int64 * 2)(3, 0), 2) == 10    # empty block
+    assert lib.scl_gpu_plan_diffusion_batched(
+        S.handle, rows, 2, buf, buf, None, None, buf, buf, 64, 2, 0, 0,
+        ctypes.byref(out)) == 4                         # reverse pass without T and dY
+    assert plan(rows, 2) == 41                          # valid batch, no device here
+
+
 def test_python_layer_surface():
     """The torch-facing helpers exist (they are exercised by the GPU tests)."""
     from scl_amd import autograd, batch, torch_ops
@@ -327,7 +357,7 @@ def test_python_layer_surface():
                  "lowp_scale_exp", "autotune", "get_shape", "set_shape"):
         assert callable(getattr(torch_ops, name)), name
     assert callable(autograd.diffuse) and callable(batch.block_diag)
-    assert hasattr(batch, "DiffusionPlan")
+    assert hasattr(batch, "DiffusionPlan") and hasattr(batch, "BatchedPlan")
 
 
 C_CALLER = r"""
