@@ -548,7 +548,8 @@ def leg_kn(steps: int, with_cpu: bool = True) -> dict:
 
 
 def leg_c5(steps: int) -> dict:
-    """256 uneven graphs, F=64, t=5, fwd+bwd: one hipGraph vs 256 sequential."""
+    """256 uneven graphs, F=64, t=5, fwd+bwd: one replayed hipGraph and one
+    persistent launch, each against 256 sequential calls."""
     import numpy as np
     import torch
     from scl_amd import CsrMatrix, batch, synth, torch_ops, _lib
@@ -573,6 +574,20 @@ def leg_c5(steps: int) -> dict:
     plan.launch()
     torch.cuda.synchronize()
     plan_ms = timed(plan.launch, steps, stream)
+    # the same work as ONE persistent launch (every graph pinned to an XCD,
+    # its steps ordered by its own progress)
+    t0 = time.perf_counter()
+    bp = batch.BatchedPlan(B, [int(x) for x in sizes], F, t)
+    bp_build_s = time.perf_counter() - t0
+    bp.X.copy_(X)
+    bp.T.copy_(T)
+    bp.launch()
+    bp.check()
+    persist_ms = timed(bp.launch, steps, stream)
+    bp.check()
+    persist_same = {"Y": bool(torch.equal(bp.Y, plan.Y)), "dX": bool(torch.equal(bp.dX, plan.dX)),
+                    "dP": bool(torch.equal(bp.dP[:B.nnz], plan.dP[:B.nnz]))}
+    bp.close()
     # sequential: per graph, the same fwd + dY + reverse with plain launches
     lib = _lib.load()
     import ctypes
@@ -631,6 +646,10 @@ def leg_c5(steps: int) -> dict:
             "plan_ms_per_fwd_bwd": round(plan_ms, 3),
             "sequential_ms_per_fwd_bwd": round(seq_ms, 3),
             "speedup_vs_sequential": round(seq_ms / plan_ms, 2),
+            "persistent_ms_per_fwd_bwd": round(persist_ms, 3),
+            "persistent_speedup_vs_sequential": round(seq_ms / persist_ms, 2),
+            "persistent_equals_plan": persist_same,
+            "persistent_plan_build_s": round(bp_build_s, 3),
             "forward_GBps_equiv": round(fwd_bytes / plan_ms / 1e6, 1),
             "autotuned_shape": shape, "results_identical": same,
             "err_vs_fp64": {"Y_max": ey, "dX_max": ex, "dP_max": ep,

@@ -354,14 +354,14 @@ SCL_EXPORT scl_error_t scl_gpu_plan_destroy(scl_gpu_plan_t* plan);
  * boundaries.  block_rows[g] are the rows of block g (they add up to the
  * operator's rows, and no stored entry leaves its block: refused otherwise).
  * Each graph runs on one XCD (bin-packed by rows + nonzeros); rows_per_item
- * rows form one work item (0: 32) and the queue of an XCD walks its graphs
- * graphs_per_chunk at a time through all their steps (0: all of them, step
- * by step).  n_features % 32 == 0, 32..512; X, Y, T, dY, dX 128-byte
- * aligned.  dX_dev null: forward only (T, dY unused); dP_dev null: no edge
- * gradient.  Results are bit-identical to scl_gpu_plan_diffusion with every
- * = 1 and the default adjoint shape.  The plan allocates its own
- * intermediates ((t - 1) planes of iterates, (t - 1) adjoint planes, t dP
- * partials).  Launch with scl_gpu_plan_launch. */
+ * rows form one work item (0: 16) and the queue of an XCD walks its graphs
+ * graphs_per_chunk at a time through all their steps (0: 4).
+ * n_features % 32 == 0, 32..512; X, Y, T, dY, dX 128-byte aligned.  dX_dev
+ * null: forward only (T, dY unused); dP_dev null: no edge gradient.  Results
+ * are bit-identical to scl_gpu_plan_diffusion with every = 1 and the default
+ * adjoint group (vec 4, one lane group per row covering F).  The plan
+ * allocates its own intermediates ((t - 1) planes of iterates, (t - 1)
+ * adjoint planes, t dP partials).  Launch with scl_gpu_plan_launch. */
 SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
     scl_sparse_t transition, const scl_index_t* block_rows, scl_size_t n_blocks,
     const scl_real_t* X_dev, scl_real_t* Y_dev, const scl_real_t* T_dev, scl_real_t* dY_dev,

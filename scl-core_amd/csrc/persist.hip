@@ -86,7 +86,9 @@ __device__ __forceinline__ void forward_row(const int32_t* __restrict__ row_ptr,
         }
         if (active) {
             const size_t o = (size_t)row * F + f;
-            *reinterpret_cast<f4*>(Y + o) = f4{acc[0], acc[1], acc[2], acc[3]};
+            // streaming store: the outputs leave L2 to the gathered rows
+            __builtin_nontemporal_store(f4{acc[0], acc[1], acc[2], acc[3]},
+                                        reinterpret_cast<f4*>(Y + o));
             if (dY) {
                 const f4 tt = *reinterpret_cast<const f4*>(T + o);
                 *reinterpret_cast<f4*>(dY + o) =
@@ -177,15 +179,12 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
         return __hip_atomic_fetch_add(head, lane == 0 ? 1 : 0, __ATOMIC_RELAXED,
                                       __HIP_MEMORY_SCOPE_AGENT);
     };
-    // Two items are claimed ahead: while item i runs, the descriptor of item
-    // i + 1 is already loading and the claim of item i + 2 is in flight (a
-    // wave's items stay in queue order, so the drain argument holds).
+    // The next item is claimed as this one starts and read after its work,
+    // so the dequeue's round trip overlaps the work (a wave's items stay in
+    // queue order, so the drain argument of the queues holds).
     const int it = __builtin_amdgcn_readlane(claim(), 0);
     if (it >= qn) return;
     PersistItem item = a.items[q0 + it];
-    int nxt = __builtin_amdgcn_readlane(claim(), 0);
-    PersistItem nitem{};
-    if (nxt < qn) nitem = a.items[q0 + nxt];
     // Completions are published per run of items of one (graph, phase): the
     // stores of the run drain once, then the run's count is added.  A wave
     // publishes everything pending before it waits, so it never waits on an
@@ -212,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
         item.end = __builtin_amdgcn_readfirstlane(item.end);
         item.need = __builtin_amdgcn_readfirstlane(item.need);
         item.slab = __builtin_amdgcn_readfirstlane(item.slab);
-        const int far_claim = claim();
+        const int next_claim = claim();
         if (item.g != pend_g || item.phase != pend_phase) publish();
         if (item.need > 0 && !(item.g == ok_g && item.need <= ok_need)) {
             publish();
@@ -264,10 +263,9 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
         pend_g = item.g;
         pend_phase = item.phase;
         ++pend_n;
-        if (nxt >= qn) break;
-        item = nitem;
-        nxt = __builtin_amdgcn_readlane(far_claim, 0);
-        if (nxt < qn) nitem = a.items[q0 + nxt];
+        const int next = __builtin_amdgcn_readlane(next_claim, 0);
+        if (next >= qn) break;
+        item = a.items[q0 + next];
     }
     publish();
 }

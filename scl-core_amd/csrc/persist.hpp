@@ -96,7 +96,7 @@ struct PersistArgs {
 // and nonzero offsets nz[0..G] (nz[g] = first stored entry of graph g, in
 // CSR and transposed order alike): graphs bin-packed over the queues by
 // rows + nonzeros, largest first; a queue takes its graphs
-// graphs_per_chunk at a time (0: all) and, for each chunk, phase by phase,
+// graphs_per_chunk at a time (0: 4) and, for each chunk, phase by phase,
 // graph by graph, lists the items (rows_per_item rows, 0: 16; a forward
 // phase slab by slab over fwd_slabs feature slabs, so that the graph's X
 // rows a queue gathers at a time are one slab wide; dP permutation items of

@@ -42,8 +42,7 @@ PersistQueues build_persist_queues(const std::vector<int64_t>& off,
         std::vector<PersistItem> list;
         auto& bin = bins[q];
         std::sort(bin.begin(), bin.end());
-        const size_t K = graphs_per_chunk > 0 ? (size_t)graphs_per_chunk
-                                              : std::max<size_t>(bin.size(), 1);
+        const size_t K = (size_t)(graphs_per_chunk > 0 ? graphs_per_chunk : 4);
         for (size_t c0 = 0; c0 < bin.size(); c0 += K) {
             const size_t c1 = std::min(bin.size(), c0 + K);
             for (int64_t p = 0; p < n_phase; ++p)
