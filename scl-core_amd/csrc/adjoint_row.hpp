@@ -6,6 +6,7 @@
 // increasing order, values p_ij) and a feature range [f0, f0 + Fp):
 //   Gp[j, :]  = sum_i p_ij Gs[i, :]                 (fp32 FMA, stored order)
 //   ddst[p]   = (accumulate ? dsrc[p] : 0) + <Gs[i, :], Xprev[j, :]>
+// (with SCAT, ddst[scat[p]]: the value lands at the entry's CSR position)
 // One gather of Gs[i, slab] feeds both; Xprev[j, :] is loaded once.  The U
 // partial dot products of a chunk are reduce-scattered across the row's G
 // lanes, and each writer lane stores one entry.
@@ -111,12 +112,13 @@ struct Scatter {
 // of the next U entries, and the running values of the current ones, are
 // fetched while the current gathers are in flight.  dsrc may equal ddst
 // (in-place accumulation).
-template <int VEC, int G, int U, int NC, bool EDGE>
+template <int VEC, int G, int U, int NC, bool EDGE, bool SCAT = false>
 __device__ __forceinline__ void adjoint_edge_row(
     const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
     const float* __restrict__ t_val, const float* __restrict__ Gs,
     const float* __restrict__ Xprev, float* __restrict__ Gp, const float* dsrc, float* ddst,
-    int accumulate, int32_t F, int32_t f0, int32_t Fp, int row, int li) {
+    int accumulate, int32_t F, int32_t f0, int32_t Fp, int row, int li,
+    const int32_t* __restrict__ scat = nullptr) {
     using VT = typename Vec<VEC>::T;
     const int beg = t_row_ptr[row], end = t_row_ptr[row + 1];
 
@@ -208,14 +210,20 @@ __device__ __forceinline__ void adjoint_edge_row(
             }
             if constexpr (kScatter) {
                 const float r = SC::reduce(d, li);
-                if (writer && k + my_e < end) ddst[k + my_e] = accumulate ? old[0] + r : r;
+                if (writer && k + my_e < end) {
+                    const int at = SCAT ? scat[k + my_e] : k + my_e;
+                    ddst[at] = accumulate ? old[0] + r : r;
+                }
             } else {
 #pragma unroll
                 for (int u = 0; u < U; ++u) d[u] = group_sum<G>(d[u]);
                 if (writer) {
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        if (k + u < end) ddst[k + u] = accumulate ? old[u] + d[u] : d[u];
+                        if (k + u < end) {
+                            const int at = SCAT ? scat[k + u] : k + u;
+                            ddst[at] = accumulate ? old[u] + d[u] : d[u];
+                        }
                 }
             }
         }

@@ -98,8 +98,9 @@ __device__ __forceinline__ void forward_row(const int32_t* __restrict__ row_ptr,
     }
 }
 
-// The items of graph g completed so far: the sum of its per-sub-queue
-// counters, read by the whole wave (coalesced loads) and broadcast.
+// The items of graph g completed so far, times 64: the sum of its
+// per-sub-queue counters, read by the whole wave (coalesced loads) and
+// broadcast.
 __device__ __forceinline__ int done_of(const int32_t* done_g) {
     int sum = 0;
 #pragma unroll
@@ -107,7 +108,7 @@ __device__ __forceinline__ int done_of(const int32_t* done_g) {
     return __builtin_amdgcn_readfirstlane(sum);
 }
 
-// Polls graph g's done counters until they reach `need`, sleeping longer
+// Polls graph g's done counters until they reach `need` (times 64), sleeping longer
 // after every empty poll (4 .. 64 units of 64 cycles); false when the XCD's
 // abort flag is set or the wait outlasts wait_ticks of the wall clock (which
 // sets the flag, so every other wait of the XCD gives up at its next poll).
@@ -115,6 +116,7 @@ __device__ __forceinline__ int done_of(const int32_t* done_g) {
 // wave-uniform.
 __device__ __forceinline__ bool wait_done(const int32_t* done_g, int32_t* abort_flag, int need,
                                           int64_t wait_ticks) {
+    need *= 64;
     if (done_of(done_g) >= need) return true;
     const int64_t t0 = wall_clock64();
     int nap = 1;
@@ -151,8 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
     constexpr int GF = G >= 16 ? G / 2 : G;
     constexpr int RPWF = 64 / GF;
     // The item loop has no lane-dependent branch around its control values:
-    // every lane issues the dequeue and completion atomics (lane 0 adds 1,
-    // the others 0) and everything the loop branches on is broadcast.  With
+    // every lane issues the dequeue and completion atomics (with the same
+    // operand) and everything the loop branches on is broadcast.  With
     // an `if (lane == 0)` dequeue the compiler may split the loop so that
     // lane 0 alone goes back to dequeue while the other lanes iterate on a
     // stale item index, never leaving it.
@@ -173,16 +175,25 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
     const int qn = __builtin_amdgcn_readfirstlane(a.q_off[l + 1]) - q0;
     const int t = a.t;
     const int F = a.F;
-    // dequeue: every lane issues the atomic (lane 0 adds 1), lane 0's result
-    // is the item index
+    // dequeue: every lane adds 1 (one atomic of 64 once folded), lane 0's
+    // result / 64 is the item index.  A result that is not a multiple of 64
+    // would mean the adds were not folded (the index could repeat): the
+    // wave then stops and flags the XCD, which the plan check reports.
     auto claim = [&]() {
-        return __hip_atomic_fetch_add(head, lane == 0 ? 1 : 0, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
+        return __hip_atomic_fetch_add(head, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto index_of = [&](int raw) -> int {
+        const int first = __builtin_amdgcn_readlane(raw, 0);
+        return (first & 63) == 0 ? first >> 6 : -1;
     };
     // The next item is claimed as this one starts and read after its work,
     // so the dequeue's round trip overlaps the work (a wave's items stay in
     // queue order, so the drain argument of the queues holds).
-    const int it = __builtin_amdgcn_readlane(claim(), 0);
+    const int it = index_of(claim());
+    if (it < 0) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     if (it >= qn) return;
     PersistItem item = a.items[q0 + it];
     // Completions are published per run of items of one (graph, phase): the
@@ -196,8 +207,8 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_s_waitcnt(0);
             __hip_atomic_fetch_add(
-                done + ((size_t)pend_g * kPersistSubQueues + sub) * kCtlStride,
-                lane == 0 ? pend_n : 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                done + ((size_t)pend_g * kPersistSubQueues + sub) * kCtlStride, pend_n,
+                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             pend_n = 0;
         }
     };
@@ -234,7 +245,8 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
                                         grad ? a.dY : nullptr, F, item.slab, r, lif);
             }
         } else if (BWD && item.phase < 2 * t) {
-            // adjoint step s: G_{s-1} = P^T G_s, D_s = D_{s+1} + <G_s[i], X_{s-1}[j]>
+            // adjoint step s: G_{s-1} = P^T G_s, D_s = D_{s+1} + <G_s[i], X_{s-1}[j]>;
+            // step 1 writes D_1 straight to dP at the entries' CSR positions
             const int s = 2 * t - item.phase;
             const float* Gs = s == t ? a.dY : a.Gw + (size_t)(s - 1) * a.plane;
             float* Gp = s == 1 ? a.dX : a.Gw + (size_t)(s - 2) * a.plane;
@@ -244,26 +256,32 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
                 if (r >= item.end) continue;
                 if (a.want_dp) {
                     const int64_t dp = a.dpad[item.g];
-                    float* dd = a.D + (size_t)(s - 1) * a.dstride + dp;
-                    const float* ds = s == t ? dd : a.D + (size_t)s * a.dstride + dp;
-                    adjoint_edge_row<4, G, U, NC, true>(a.t_row_ptr, a.t_col, a.t_val, Gs,
-                                                        Xprev, Gp, ds, dd, s < t ? 1 : 0, F, 0,
-                                                        F, r, li);
+                    const float* ds = s == t ? nullptr : a.D + (size_t)(s - 1) * a.dstride + dp;
+                    if (s > 1) {
+                        float* dd = a.D + (size_t)(s - 2) * a.dstride + dp;
+                        adjoint_edge_row<4, G, U, NC, true>(a.t_row_ptr, a.t_col, a.t_val, Gs,
+                                                            Xprev, Gp, ds, dd, s < t ? 1 : 0, F,
+                                                            0, F, r, li);
+                    } else {
+                        adjoint_edge_row<4, G, U, NC, true, true>(
+                            a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, a.dP,
+                            s < t ? 1 : 0, F, 0, F, r, li, a.t_src);
+                    }
                 } else {
                     adjoint_edge_row<4, G, U, NC, false>(a.t_row_ptr, a.t_col, a.t_val, Gs,
                                                          nullptr, Gp, nullptr, nullptr, 0, F, 0,
                                                          F, r, li);
                 }
             }
-        } else if (BWD) {
-            // dP in CSR order from the last partials D_1
-            const float* src = a.D + a.dpad[item.g];
-            for (int p = item.begin + lane; p < item.end; p += 64) a.dP[a.t_src[p]] = src[p];
         }
         pend_g = item.g;
         pend_phase = item.phase;
         ++pend_n;
-        const int next = __builtin_amdgcn_readlane(next_claim, 0);
+        const int next = index_of(next_claim);
+        if (next < 0) {
+            __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
         if (next >= qn) break;
         item = a.items[q0 + next];
     }

@@ -32,10 +32,12 @@ constexpr int kPersistQueues = 8;          // XCDs
 // same-address atomics serialise at a few tens of ns each.
 constexpr int kPersistSubQueues = 8;
 constexpr int kPersistLanes = kPersistQueues * kPersistSubQueues;
-constexpr int kPersistScatterItem = 8192;  // entries per dP-permutation item
 constexpr int kPersistWaitSeconds = 2;     // a wait gives up after this long
 // Control words, each alone in a 128-byte line (32 ints) so that polls of
-// one counter never queue behind another's traffic: head[l] of sub-queue l
+// one counter never queue behind another's traffic.  Every lane of a wave
+// adds to them (a wave-uniform operand, which the compiler folds into one
+// atomic of 64 times it), so heads and done counts run in units of 64.
+// head[l] of sub-queue l
 // (l = XCD * kPersistSubQueues + sub) at word l, abort[q] at word
 // kPersistLanes + q, done[g][sub] at word kPersistLanes + kPersistQueues +
 // g * kPersistSubQueues + sub.
@@ -48,9 +50,9 @@ constexpr int64_t persist_ctl_words(int64_t n_graphs) {
 
 // One work item: rows [begin, end) of graph g in phase `phase` (0 .. t-1:
 // forward step phase + 1, feature slab `slab` only; t .. 2t-1: adjoint
-// step 2t - phase, whole rows; 2t: dP permutation of transposed positions
-// [begin, end)).  It may start once graph g's done counter reaches `need`
-// (the items of its earlier phases).
+// step 2t - phase, whole rows; the last adjoint step writes dP at its CSR
+// positions).  It may start once graph g's done counters reach `need` (the
+// items of its earlier phases).
 struct PersistItem {
     int32_t g, phase, begin, end, need, slab, pad1, pad2;
 };
@@ -72,8 +74,8 @@ struct PersistArgs {
     float* dX;
     float* dP;
     // plan buffers: X_1 .. X_{t-1}, G_1 .. G_{t-1} (planes of n x F) and the
-    // dP partials D_1 .. D_t (dstride floats each, graph g at dpad[g] plus
-    // its transposed positions)
+    // dP partials D_2 .. D_t (dstride floats each, graph g at dpad[g] plus
+    // its transposed positions; step 1 writes dP itself)
     float* Xw;
     float* Gw;
     float* D;
@@ -99,11 +101,11 @@ struct PersistArgs {
 // graphs_per_chunk at a time (0: 4) and, for each chunk, phase by phase,
 // graph by graph, lists the items (rows_per_item rows, 0: 16; a forward
 // phase slab by slab over fwd_slabs feature slabs, so that the graph's X
-// rows a queue gathers at a time are one slab wide; dP permutation items of
-// kPersistScatterItem entries), then deals that list round-robin into the
-// XCD's sub-queues.  Every item's `need` counts its graph's items of earlier
-// phases; each sub-queue keeps the list's (chunk, phase) order, so the XCD's
-// sub-queues drained in order never wait on an item not yet taken.
+// rows a queue gathers at a time are one slab wide), then deals that list
+// round-robin into the XCD's sub-queues.  Every item's `need` counts its
+// graph's items of earlier phases; each sub-queue keeps the list's (chunk,
+// phase) order, so the XCD's sub-queues drained in order never wait on an
+// item not yet taken.
 struct PersistQueues {
     std::vector<PersistItem> items;
     std::vector<int32_t> q_off;   // kPersistLanes + 1
@@ -115,7 +117,7 @@ struct PersistQueues {
 };
 PersistQueues build_persist_queues(const std::vector<int64_t>& off,
                                    const std::vector<int64_t>& nz, int64_t t, bool backward,
-                                   bool want_dp, int64_t rows_per_item,
+                                   int64_t rows_per_item,
                                    int64_t graphs_per_chunk, int fwd_slabs);
 
 // Lanes per row of the persistent kernel for F features (F % 32 == 0,

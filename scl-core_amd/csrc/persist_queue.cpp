@@ -11,17 +11,16 @@ namespace scl_amd {
 
 PersistQueues build_persist_queues(const std::vector<int64_t>& off,
                                    const std::vector<int64_t>& nz, int64_t t, bool backward,
-                                   bool want_dp, int64_t rows_per_item,
+                                   int64_t rows_per_item,
                                    int64_t graphs_per_chunk, int fwd_slabs) {
     const int64_t G = (int64_t)off.size() - 1;
     const int64_t R = rows_per_item > 0 ? rows_per_item : 16;
     const int64_t row_phases = backward ? 2 * t : t;
-    const int64_t n_phase = row_phases + (want_dp ? 1 : 0);
+    const int64_t n_phase = row_phases;
     const int64_t S = std::max(fwd_slabs, 1);
     auto slabs_of = [&](int64_t p) -> int64_t { return p < t ? S : 1; };
     auto items_of = [&](int64_t g, int64_t p) -> int64_t {
-        if (p < row_phases) return slabs_of(p) * ((off[g + 1] - off[g] + R - 1) / R);
-        return (nz[g + 1] - nz[g] + kPersistScatterItem - 1) / kPersistScatterItem;
+        return slabs_of(p) * ((off[g + 1] - off[g] + R - 1) / R);
     };
     auto work = [&](int64_t g) { return (off[g + 1] - off[g]) + (nz[g + 1] - nz[g]); };
     std::vector<int64_t> order(G);
@@ -49,11 +48,9 @@ PersistQueues build_persist_queues(const std::vector<int64_t>& off,
                 for (size_t ci = c0; ci < c1; ++ci) {
                     const int64_t g = bin[ci];
                     const int64_t m = items_of(g, p);
-                    const bool perm = p >= row_phases;
-                    const int64_t lo = perm ? nz[g] : off[g];
-                    const int64_t hi = perm ? nz[g + 1] : off[g + 1];
-                    const int64_t step = perm ? kPersistScatterItem : R;
-                    const int64_t ns = perm ? 1 : slabs_of(p);
+                    const int64_t lo = off[g], hi = off[g + 1];
+                    const int64_t step = R;
+                    const int64_t ns = slabs_of(p);
                     const int64_t per = m / ns;
                     for (int64_t sl = 0; sl < ns; ++sl)
                         for (int64_t i = 0; i < per; ++i) {

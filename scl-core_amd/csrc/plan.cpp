@@ -170,8 +170,12 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
         nz[G] = h->nnz;
         const bool backward = dX_dev != nullptr;
         const bool want_dp = backward && dP_dev != nullptr;
-        PersistQueues Q = build_persist_queues(off, nz, t, backward, want_dp, rows_per_item,
+        PersistQueues Q = build_persist_queues(off, nz, t, backward, rows_per_item,
                                                graphs_per_chunk, persist_forward_slabs(F));
+        // heads and done counts run in units of 64 (int32)
+        for (int l = 0; l < kPersistLanes; ++l)
+            if ((int64_t)(Q.q_off[l + 1] - Q.q_off[l]) + 2 > std::numeric_limits<int32_t>::max() / 64)
+                fail(SCL_ERROR_INVALID_ARGUMENT, "Too many work items in one queue");
         const int dev = current_device();
         DeviceCSR* dp = nullptr;
         {
@@ -207,8 +211,10 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
             a.Gw = backward && t >= 2
                        ? plan_alloc<float>(plan, plane * (size_t)(t - 1), "plan adjoint iterates")
                        : nullptr;
-            a.D = want_dp ? plan_alloc<float>(plan, (size_t)Q.dstride * (size_t)t, "plan dP partials")
-                          : nullptr;
+            // partials D_2 .. D_t (step 1 writes dP itself)
+            a.D = want_dp && t >= 2
+                      ? plan_alloc<float>(plan, (size_t)Q.dstride * (size_t)(t - 1), "plan dP partials")
+                      : nullptr;
             int32_t* d_dpad = plan_alloc<int32_t>(plan, (size_t)G, "plan offsets");
             PersistItem* d_items = plan_alloc<PersistItem>(plan, Q.items.size(), "plan items");
             int32_t* d_qoff = plan_alloc<int32_t>(plan, Q.q_off.size(), "plan queues");
@@ -280,14 +286,14 @@ SCL_EXPORT scl_error_t scl_gpu_plan_check(scl_gpu_plan_t plan, void* stream) {
                 fail(SCL_ERROR_INTERNAL, "batched plan: a wait on XCD " + std::to_string(q) +
                                              " gave up");
         for (int l = 0; l < kPersistLanes; ++l)
-            if (word(l) < plan->queue_len[l])
+            if ((int64_t)word(l) < (int64_t)plan->queue_len[l] * 64)
                 fail(SCL_ERROR_INTERNAL, "batched plan: sub-queue " + std::to_string(l) +
                                              " was not drained (no wave ran there)");
         for (size_t g = 0; g < G; ++g) {
             int64_t sum = 0;
             for (int sq = 0; sq < kPersistSubQueues; ++sq)
                 sum += word(kCtlDone + g * kPersistSubQueues + sq);
-            if (sum != plan->expect_done[g])
+            if (sum != (int64_t)plan->expect_done[g] * 64)
                 fail(SCL_ERROR_INTERNAL, "batched plan: graph " + std::to_string(g) +
                                              " did not complete");
         }

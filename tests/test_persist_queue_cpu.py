@@ -3,7 +3,7 @@
 The queue builder (scl-core_amd/csrc/persist_queue.cpp) is compiled here
 with a small C++ driver.  For random batches and every option it checks that
 each graph lives in one XCD's queue (dealt round-robin into sub-queues), that every phase's items tile the graph's rows
-(or nonzeros) exactly once, that each item's `need` counts the graph's items
+exactly once (forward phases per feature slab), that each item's `need` counts the graph's items
 of its earlier phases (all listed before it in the same queue), and that draining an XCD's sub-queues in order with any number of
 workers on each, under any interleaving, never deadlocks and completes every
 item (randomised schedules).  The dP partial offsets are padded to 128 bytes and
@@ -39,13 +39,12 @@ int main() {
         }
         const int64_t t = 1 + (int64_t)(rng() % 6);
         const bool backward = rng() % 3 != 0;
-        const bool want_dp = backward && rng() % 2;
         const int64_t R = (int64_t)(rng() % 3 == 0 ? 0 : 1 + rng() % 300);
         const int64_t K = (int64_t)(rng() % 3 == 0 ? 0 : 1 + rng() % 6);
         const int S = 1 + (int)(rng() % 4);
-        PersistQueues Q = build_persist_queues(off, nz, t, backward, want_dp, R, K, S);
+        PersistQueues Q = build_persist_queues(off, nz, t, backward, R, K, S);
         const int64_t row_phases = backward ? 2 * t : t;
-        const int64_t n_phase = row_phases + (want_dp ? 1 : 0);
+        const int64_t n_phase = row_phases;
         REQUIRE((int)Q.q_off.size() == kPersistLanes + 1 && Q.q_off[0] == 0);
         REQUIRE(Q.q_off[kPersistLanes] == (int)Q.items.size());
         for (int l = 0; l < kPersistLanes; ++l) REQUIRE(Q.q_off[l] <= Q.q_off[l + 1]);
