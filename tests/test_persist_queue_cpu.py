@@ -118,8 +118,8 @@ int main() {
         }
         REQUIRE(prev_end + 32 <= Q.dstride);
         // drain each XCD's sub-queues together with the kernel's policy:
-        // W_l waves on sub-queue l, each holding its current item plus up to
-        // two claimed ahead (one more claimed as each item starts); a wave
+        // W_l waves on sub-queue l, each holding its current item and the
+        // next one, claimed as the current one starts; a wave
         // publishes its completed items lazily, per run of one (graph,
         // phase), and always before it waits and when it leaves; it skips the
         // check when its last satisfied dependency covers the item.  Random
@@ -153,8 +153,7 @@ int main() {
                 }
                 for (auto& w : ws) {
                     claim(w);
-                    if (w.held[0] < 0) { w.stage = 3; continue; }
-                    claim(w);
+                    if (w.held[0] < 0) w.stage = 3;
                 }
                 while (true) {
                     std::vector<int> movable;
