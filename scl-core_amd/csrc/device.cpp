@@ -742,7 +742,9 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
     if (d.pins > 0)
         fail(SCL_ERROR_INVALID_ARGUMENT,
              "a diffusion plan holds this operator: its grouped form cannot be rebuilt");
-    // groups follow the device's visiting order (the locality schedule)
+    // groups follow the device's visiting order (the locality schedule);
+    // on square operators each group gathers the neighbours that share the
+    // most columns (fewer distinct columns per group, so fewer gathers)
     std::vector<int32_t> order((size_t)p);
     if (d.sched) {
         hip_check(hipMemcpy(order.data(), d.sched, (size_t)p * 4, hipMemcpyDeviceToHost),
@@ -750,6 +752,7 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
     } else {
         for (int64_t i = 0; i < p; ++i) order[i] = (int32_t)i;
     }
+    if (p == q) order = neighbour_groups(p, h->indptr, h->indices, order, R);
     const int64_t n_groups = (p + R - 1) / R;
     std::vector<int32_t> gptr((size_t)n_groups + 1, 0), gvptr((size_t)n_groups + 1, 0);
     std::vector<int32_t> grow((size_t)n_groups * R, -1), gperm;

@@ -206,6 +206,16 @@ constexpr int64_t kScheduleMinRows = 16384;
 template <typename I>
 std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 
+// Rows of a square pattern in groups of R for the row-grouped SpMM form:
+// walks `order`; each row not yet placed opens a group, which is filled
+// with the unplaced neighbour of its rows sharing the most columns with the
+// group's union so far (the next unplaced row of `order` when no neighbour
+// is left).  Returns every row once, each group's rows consecutive, groups
+// in the order of their first row; or `order` itself when its consecutive
+// groups have fewer distinct columns in all.
+std::vector<int32_t> neighbour_groups(int64_t n, const int64_t* ptr, const int64_t* idx,
+                                      const std::vector<int32_t>& order, int R);
+
 // Device copy of a handle, refreshed when its generation changed.
 // Caller holds h->mu.
 DeviceCSR& device_csr(scl_sparse_matrix* h, int device);

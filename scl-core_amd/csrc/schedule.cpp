@@ -120,4 +120,84 @@ std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx) {
 template std::vector<int32_t> locality_schedule<int64_t>(int64_t, const int64_t*, const int64_t*);
 template std::vector<int32_t> locality_schedule<int32_t>(int64_t, const int32_t*, const int32_t*);
 
+namespace {
+
+// Distinct columns summed over the groups of R consecutive rows of `rows`:
+// the gathers of the row-grouped form.
+int64_t grouped_union(int64_t n, const int64_t* ptr, const int64_t* idx,
+                      const std::vector<int32_t>& rows, int R) {
+    std::vector<int64_t> mark((size_t)n, -1);
+    int64_t total = 0;
+    for (size_t k = 0; k < rows.size(); ++k) {
+        const int64_t g = (int64_t)(k / (size_t)R);
+        const int32_t r = rows[k];
+        for (int64_t e = ptr[r]; e < ptr[r + 1]; ++e) {
+            const int64_t c = idx[e];
+            if (c >= 0 && c < n && mark[(size_t)c] != g) {
+                mark[(size_t)c] = g;
+                ++total;
+            }
+        }
+    }
+    return total;
+}
+
+}  // namespace
+
+std::vector<int32_t> neighbour_groups(int64_t n, const int64_t* ptr, const int64_t* idx,
+                                      const std::vector<int32_t>& order, int R) {
+    std::vector<int32_t> out;
+    out.reserve((size_t)n);
+    std::vector<uint8_t> used((size_t)n, 0);
+    // stamp[c] == current group: column c is in the group's union
+    std::vector<int32_t> stamp((size_t)n, -1);
+    int32_t group = 0;
+    size_t cursor = 0;  // first possibly unplaced position of the order
+    std::vector<int32_t> members;
+    members.reserve((size_t)R);
+    auto add = [&](int32_t row) {
+        used[row] = 1;
+        out.push_back(row);
+        members.push_back(row);
+        for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) stamp[(size_t)idx[k]] = group;
+    };
+    for (size_t at = 0; at < order.size(); ++at) {
+        const int32_t first = order[at];
+        if (used[first]) continue;
+        members.clear();
+        add(first);
+        while ((int)members.size() < R) {
+            // the unplaced neighbour of a member sharing most of the union
+            int32_t best = -1;
+            int64_t best_score = -1;
+            for (size_t m = 0; m < members.size(); ++m) {
+                const int32_t row = members[m];
+                for (int64_t k = ptr[row]; k < ptr[row + 1]; ++k) {
+                    const int64_t c = idx[k];
+                    if (c < 0 || c >= n || used[c]) continue;
+                    int64_t score = 0;
+                    for (int64_t e = ptr[c]; e < ptr[c + 1]; ++e)
+                        score += stamp[(size_t)idx[e]] == group;
+                    if (score > best_score) {
+                        best_score = score;
+                        best = (int32_t)c;
+                    }
+                }
+            }
+            if (best < 0) {
+                // no unplaced neighbour: the next unplaced row of the order
+                while (cursor < order.size() && used[order[cursor]]) ++cursor;
+                if (cursor == order.size()) break;
+                best = order[cursor];
+            }
+            add(best);
+        }
+        ++group;
+    }
+    // greedy groups can lose to plain consecutive rows (a pattern the
+    // locality order already walks well): keep whichever gathers less
+    if (grouped_union(n, ptr, idx, out, R) >= grouped_union(n, ptr, idx, order, R)) return order;
+    return out;
+}
+
 }  // namespace scl_amd
