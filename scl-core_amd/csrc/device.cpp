@@ -798,14 +798,16 @@ bool ensure_grouped(scl_sparse_matrix* h, DeviceCSR& d, int R) {
     g.gptr = dev_alloc<int32_t>(gptr.size(), "grouped gptr");
     g.gvptr = dev_alloc<int32_t>(gvptr.size(), "grouped gvptr");
     g.grow = dev_alloc<int32_t>(grow.size(), "grouped rows");
-    g.gword = dev_alloc<uint32_t>((size_t)std::max<int64_t>(ne, 1), "grouped words");
+    // padded by 32 zero words (column 0, no rows): a chunk of up to 16
+    // entries and the next chunk's words load without clamping
+    g.gword = dev_alloc<uint32_t>((size_t)ne + 32, "grouped words");
     g.gperm = dev_alloc<int32_t>((size_t)std::max<int64_t>(nnz, 1), "grouped perm");
     g.gval = dev_alloc<float>((size_t)nnz + 128, "grouped values");
     hip_check(hipMemcpy(g.gptr, gptr.data(), gptr.size() * 4, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(g.gvptr, gvptr.data(), gvptr.size() * 4, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemcpy(g.grow, grow.data(), grow.size() * 4, hipMemcpyHostToDevice), "upload");
-    if (ne > 0)
-        hip_check(hipMemcpy(g.gword, gword.data(), (size_t)ne * 4, hipMemcpyHostToDevice), "upload");
+    gword.resize((size_t)ne + 32, 0u);
+    hip_check(hipMemcpy(g.gword, gword.data(), gword.size() * 4, hipMemcpyHostToDevice), "upload");
     if (nnz > 0)
         hip_check(hipMemcpy(g.gperm, gperm.data(), (size_t)nnz * 4, hipMemcpyHostToDevice), "upload");
     hip_check(hipMemset(g.gval, 0, ((size_t)nnz + 128) * sizeof(float)), "clear grouped values");

@@ -327,7 +327,14 @@ __global__ __launch_bounds__(kBlock) void k_spmm_rowsplit(
 // chain as the row-split kernels, bit for bit.  The values of a chunk of U
 // entries (at most U * R <= 64 of them) are loaded one per lane and handed
 // to the FMAs with readlane, so no scalar load sits inside the row loop.
-template <int VEC, int R, int U, bool NT>
+//
+// The scalar unit carries the loop's bookkeeping, so the full chunks keep it
+// short: their U words come in one contiguous scalar load (the word array is
+// padded past its end), the next chunk's words load under the current
+// gathers, and with OFF32 (the gathered operand spans less than 4 GiB) a
+// gather's offset is one 32-bit multiply.  Only the last, partial chunk
+// clamps its entries and checks each against the group's end.
+template <int VEC, int R, int U, bool NT, bool OFF32>
 __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int32_t* __restrict__ gptr, const uint32_t* __restrict__ gword,
     const int32_t* __restrict__ gvptr, const float* __restrict__ gval,
@@ -348,8 +355,18 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
     const int ws = min(64 * VEC, F - f0);
     const bool active = fl < ws;
     const size_t plane = (size_t)f0 * (size_t)n_rows;
-    const float* Xf = (layout & 1) ? X + plane + (active ? fl : 0) : X + f0 + (active ? fl : 0);
+    const float* Xs = (layout & 1) ? X + plane : X + f0;   // wave-uniform
+    const float* Xf = Xs + (active ? fl : 0);
     const size_t xs = (layout & 1) ? (size_t)ws : (size_t)F;
+    const uint32_t xs4 = (uint32_t)xs * 4u, lo4 = (uint32_t)(active ? fl : 0) * 4u;
+    auto gather = [&](uint32_t wd) -> VT {
+        const uint32_t j = wd & 0xFFFFFFu;
+        if constexpr (OFF32)
+            return *reinterpret_cast<const VT*>(reinterpret_cast<const char*>(Xs) +
+                                                (j * xs4 + lo4));
+        else
+            return *reinterpret_cast<const VT*>(Xf + (size_t)j * xs);
+    };
     const int e1 = __builtin_amdgcn_readfirstlane(gptr[g + 1]);
     int e = __builtin_amdgcn_readfirstlane(gptr[g]);
     int vo = __builtin_amdgcn_readfirstlane(gvptr[g]);
@@ -360,20 +377,59 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
 #pragma unroll
         for (int c = 0; c < VEC; ++c) acc[r][c] = 0.0f;
 
-    for (; e < e1; e += U) {
+    if (e + U <= e1) {
+        uint32_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = gword[e + u];
+        while (true) {
+            // this chunk's values, one per lane, and the next 64 when a chunk
+            // can hold more (gval is padded by 128); loaded ahead of the
+            // gathers so that waiting for them does not wait for the gathers
+            const int vl = __builtin_bit_cast(int, gval[vo + lane]);
+            const int vh = U * R > 64 ? __builtin_bit_cast(int, gval[vo + 64 + lane]) : 0;
+            __builtin_amdgcn_sched_barrier(0);
+            VT x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = gather(w[u]);
+            // the next chunk's words (the array is padded by 2 U entries)
+            uint32_t wn[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) wn[u] = gword[e + U + u];
+            __builtin_amdgcn_sched_barrier(0);
+            int c = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t m = w[u] >> 24;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (m & (1u << r)) {
+                        const float v = __builtin_bit_cast(
+                            float, U * R > 64 && c >= 64 ? __builtin_amdgcn_readlane(vh, c - 64)
+                                                         : __builtin_amdgcn_readlane(vl, c));
+                        ++c;
+                        fma_vec(v, x[u], acc[r]);
+                    }
+                }
+            }
+            vo += c;
+            e += U;
+            if (e + U > e1) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) w[u] = wn[u];
+        }
+    }
+    if (e < e1) {
+        // the partial last chunk: entries past the group's end repeat its
+        // last one and add nothing
         uint32_t w[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) w[u] = gword[min(e + u, e1 - 1)];
-        // this chunk's values, one per lane, and the next 64 when a chunk
-        // can hold more (gval is padded by 128); loaded ahead of the gathers
-        // so that waiting for them does not wait for the gathers too
         const int vl = __builtin_bit_cast(int, gval[vo + lane]);
         const int vh = U * R > 64 ? __builtin_bit_cast(int, gval[vo + 64 + lane]) : 0;
         __builtin_amdgcn_sched_barrier(0);
         VT x[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            x[u] = *reinterpret_cast<const VT*>(Xf + (size_t)(w[u] & 0xFFFFFFu) * xs);
+        for (int u = 0; u < U; ++u) x[u] = gather(w[u]);
         int c = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -391,7 +447,6 @@ __global__ __launch_bounds__(kBlock) void k_spmm_grouped(
                 }
             }
         }
-        vo += c;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -833,15 +888,19 @@ namespace {
 template <int VEC, int R, int U, bool NT>
 void launch_grouped_vec(const GroupedCsr& G, const float* X, float* Y, int64_t F,
                         const SpmmShape& shape, int lds, int layout, int32_t n_rows,
-                        hipStream_t s) {
+                        int64_t n_gathered, hipStream_t s) {
     const int64_t n_gblocks = (G.n_groups + kBlock / 64 - 1) / (kBlock / 64);
     const int64_t n_slabs = (F + 64 * VEC - 1) / (64 * VEC);
     int64_t rb_per_xcd, rbb, nb;
     xcd_grid(shape, n_gblocks, n_slabs, rb_per_xcd, rbb, nb);
-    hipLaunchKernelGGL((k_spmm_grouped<VEC, R, U, NT>), dim3((unsigned)nb), dim3(kBlock),
-                       (unsigned)lds, s, G.gptr, G.gword, G.gvptr, G.gval, G.grow, X, Y,
-                       (int32_t)G.n_groups, (int32_t)F, (int32_t)n_gblocks,
-                       (int32_t)rb_per_xcd, (int32_t)rbb, (int32_t)layout, n_rows);
+    // 32-bit gather offsets when every byte of the gathered rows (of either
+    // layout) lies within 4 GiB of the slab's base
+    const bool off32 = (uint64_t)n_gathered * (uint64_t)F * 4u < ((uint64_t)1 << 32);
+    auto kern = off32 ? k_spmm_grouped<VEC, R, U, NT, true> : k_spmm_grouped<VEC, R, U, NT, false>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kBlock), (unsigned)lds, s, G.gptr,
+                       G.gword, G.gvptr, G.gval, G.grow, X, Y, (int32_t)G.n_groups, (int32_t)F,
+                       (int32_t)n_gblocks, (int32_t)rb_per_xcd, (int32_t)rbb, (int32_t)layout,
+                       n_rows);
 }
 
 // ---------------------------------------------------- LDS-staged groups --
@@ -1197,8 +1256,10 @@ void launch_spmm_grouped(const CsrView& P, const float* X, float* Y, int64_t F,
     // U entries per chunk with U * R <= 128 (a chunk's values fit two per lane)
 #define SCL_GROUPED(V, RR, UU)                                                          \
     (shape.nontemporal                                                                  \
-         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape, lds, layout, P.n, s) \
-         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape, lds, layout, P.n, s))
+         ? launch_grouped_vec<V, RR, UU, true>(*G, X, Y, F, shape, lds, layout, P.n,    \
+                                               P.n_sec, s)                              \
+         : launch_grouped_vec<V, RR, UU, false>(*G, X, Y, F, shape, lds, layout, P.n,   \
+                                                P.n_sec, s))
 #define SCL_GROUPED_V(RR, UU)                          \
     do {                                               \
         if (shape.vec == 4) SCL_GROUPED(4, RR, UU);    \

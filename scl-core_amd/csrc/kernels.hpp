@@ -29,7 +29,7 @@ struct GroupedCsr {
     int64_t n_groups = 0;
     int64_t n_entries = 0;
     int32_t* gptr = nullptr;      // n_groups + 1: first entry of each group
-    uint32_t* gword = nullptr;    // n_entries: column | mask << 24
+    uint32_t* gword = nullptr;    // n_entries + 32 (zero padding): column | mask << 24
     int32_t* gvptr = nullptr;     // n_groups + 1: first value of each group
     float* gval = nullptr;        // nnz + 128 (padded for two whole-wave loads)
     int32_t* gperm = nullptr;     // nnz: gval[i] = val[gperm[i]]
