@@ -49,16 +49,16 @@ void check_operator(const scl_sparse_matrix* P, int64_t n_nodes, int64_t F) {
 constexpr int64_t kGroupedMinRows = 4096;
 
 // The measured-best forward shape of wide operators (n >= kGroupedMinRows,
-// F >= 128): row groups of 4 (vec 4, 8 union entries per chunk,
+// F >= 128): row groups of 4 (vec 4, 16 union entries per chunk,
 // non-temporal stores, at most 4 workgroups per CU).  At n = 200,000, nnz
-// 8.9 M, F = 2000 with slab-planar iterates it runs 2.79 ms per step (16
-// entries per chunk: 2.81 ms capped, 2.93 ms uncapped; 8 entries uncapped:
-// 3.55 ms, its extra resident rows crowd the L2).
+// 8.9 M, F = 2000 with slab-planar iterates it runs 2.73 ms per step (8
+// entries per chunk: 2.76 ms; 16 uncapped: 2.82 ms, its extra resident
+// rows crowd the L2).
 SpmmShape wide_default_shape() {
     SpmmShape g;
     g.vec = 4;
     g.group = 64;
-    g.unroll = 8;
+    g.unroll = 16;
     g.rows_per_group = 4;
     g.wg_per_cu = 4;
     g.nontemporal = true;

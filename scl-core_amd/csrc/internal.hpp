@@ -209,8 +209,8 @@ std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 // Rows of a square pattern in groups of R for the row-grouped SpMM form:
 // walks `order`; each row not yet placed opens a group, which is filled
 // with the unplaced neighbour of its rows sharing the most columns with the
-// group's union so far (the next unplaced row of `order` when no neighbour
-// is left).  Returns every row once, each group's rows consecutive, groups
+// group's union so far, among rows at most 512 places after the group's
+// first row in `order` (the next unplaced row of `order` when none is left).  Returns every row once, each group's rows consecutive, groups
 // in the order of their first row; or `order` itself when its consecutive
 // groups have fewer distinct columns in all.
 std::vector<int32_t> neighbour_groups(int64_t n, const int64_t* ptr, const int64_t* idx,
