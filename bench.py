@@ -16,7 +16,10 @@ device-resident P and X.
   cpu_baseline      the FP32 CPU restatement of the reference algorithm
                     (OpenMP, all rows, a column sample, scaled to full width)
 
-Further legs (in "legs"; --legs selects, "c2" alone skips them):
+Further legs (in "legs"; --legs selects, "c2" alone skips them; "cr" runs
+only when named: the headline kernel on operators whose columns are random
+within the first M rows, M = 2048 .. 200,000, to time L2, Infinity Cache
+and HBM-served misses):
   c3  knn30_500k, F=128, t=6: forward with checkpoints every 2 + dL/dY of
       0.5||Y-T||^2 + fused adjoint/edge-gradient reverse pass (one hipGraph),
       vs the same work as plain kernels (6 SpMM + 6 transposed SpMM + 6 edge-
@@ -656,6 +659,50 @@ def leg_c5(steps: int) -> dict:
                             "how": "max over 8 sampled graphs, FP64 oracle gradient"}}
 
 
+def leg_cr(steps: int, shape: dict, n: int, nnz_rows, F: int,
+           ranges=(2048, 16384, 32768, 65536, 200000)) -> dict:
+    """The headline kernel on operators with the headline's row lengths whose
+    columns are random (distinct, any spacing) within the first M rows: at
+    M = 2048 every slab of them fits an XCD's L2, at 16384 all eight slabs
+    fit the 256 MB Infinity Cache (125 MiB), from 65536 on they do not.  The
+    times show what the kernel's L2 misses cost when served from each level."""
+    import numpy as np
+    import torch
+    from scl_amd import CsrMatrix, synth, torch_ops
+    lens = np.diff(nnz_rows)
+    ip = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    Xd = torch.from_numpy(synth.normal(n, F, seed=12)).to(dev)
+    out = {"workload": "headline row lengths, columns random within the first M rows, t=8",
+           "ms_per_launch": {}}
+    for M in ranges:
+        rng = np.random.Generator(np.random.PCG64(12))
+        base = rng.integers(0, M, n)
+        step = 1 + 2 * rng.integers(0, max(1, M // 2), n)
+        ix = np.empty(int(ip[-1]), np.int64)
+        for i in range(n):
+            st = int(step[i])
+            while math.gcd(st, M) != 1:   # coprime to M: the columns are distinct
+                st += 1
+            ix[ip[i]:ip[i + 1]] = np.sort((base[i] + st * np.arange(lens[i])) % M)
+        ok = all(ip[i + 1] - ip[i] == len(np.unique(ix[ip[i]:ip[i + 1]])) for i in range(0, n, 997))
+        P = CsrMatrix.from_arrays(ip, ix, np.full(len(ix), 1.0 / 64, np.float32), (n, n))
+        kw = dict(shape)
+        torch_ops.set_shape(P, F, kw.pop("vec"), kw.pop("group"), kw.pop("unroll"), **kw)
+        torch_ops.diffuse(P, Xd, 8)
+        torch.cuda.synchronize()
+        ms = timed(lambda: torch_ops.diffuse(P, Xd, 8), steps, stream) / 8
+        out["ms_per_launch"][str(M)] = round(ms, 4)
+        out.setdefault("distinct_columns_checked", True)
+        out["distinct_columns_checked"] = out["distinct_columns_checked"] and bool(ok)
+        log(f"leg cr: M={M} {ms:.4f} ms per launch")
+        del P
+        torch.cuda.empty_cache()
+    out["shape"] = dict(shape)
+    return out
+
+
 def leg_l2(steps: int, shape: dict, n: int, nnz_rows, F: int) -> dict:
     """The headline kernel with every gather L2-resident: an operator with
     the headline's row lengths whose columns all fall in 2048 rows (a slab of
@@ -847,6 +894,14 @@ def main() -> int:
             l2 = {"error": f"{type(e).__name__}: {e}"}
         log(f"leg l2: {l2}")
 
+    cr = None
+    if "cr" in args.legs.split(","):
+        log("leg cr (column ranges: L2, Infinity Cache and HBM-served misses)")
+        try:
+            cr = leg_cr(max(3, args.steps // 2), torch_ops.get_shape(A, F), n, g.indptr, F)
+        except Exception as e:
+            cr = {"error": f"{type(e).__name__}: {e}"}
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (restatement, OpenMP)")
@@ -908,6 +963,7 @@ def main() -> int:
             "traffic": traffic,
         },
         "l2_bound": l2,
+        **({"column_ranges": cr} if cr is not None else {}),
         "cpu_baseline": cpu,
         "legs": legs,
     }
