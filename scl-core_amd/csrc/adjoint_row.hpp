@@ -111,16 +111,19 @@ struct Scatter {
 // range; li is the lane's index in its group.  The source rows and weights
 // of the next U entries, and the running values of the current ones, are
 // fetched while the current gathers are in flight.  dsrc may equal ddst
-// (in-place accumulation).
+// (in-place accumulation).  beg_in / end_in, when beg_in >= 0, are the
+// row's bounds t_row_ptr[row], t_row_ptr[row + 1] loaded by the caller.
 template <int VEC, int G, int U, int NC, bool EDGE, bool SCAT = false>
 __device__ __forceinline__ void adjoint_edge_row(
     const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
     const float* __restrict__ t_val, const float* __restrict__ Gs,
     const float* __restrict__ Xprev, float* __restrict__ Gp, const float* dsrc, float* ddst,
     int accumulate, int32_t F, int32_t f0, int32_t Fp, int row, int li,
-    const int32_t* __restrict__ scat = nullptr) {
+    const int32_t* __restrict__ scat = nullptr, int beg_in = -1, int end_in = -1) {
     using VT = typename Vec<VEC>::T;
-    const int beg = t_row_ptr[row], end = t_row_ptr[row + 1];
+    // the row's bounds, unless the caller loaded them already
+    const int beg = beg_in >= 0 ? beg_in : t_row_ptr[row];
+    const int end = beg_in >= 0 ? end_in : t_row_ptr[row + 1];
 
     bool act[NC];
     int off[NC];  // feature offset of chunk c for this lane (0 when inactive)

@@ -30,24 +30,42 @@ __device__ __forceinline__ int xcd_of_wave() {
     return (int)(__builtin_amdgcn_s_getreg(kHwRegXccId | (0 << 6) | ((4 - 1) << 11)) & 0xFu);
 }
 
+// A work item's descriptor through the scalar unit: the queues are
+// read-only in the launch, so their loads go through the constant address
+// space, independent of the vector memory counter that the item's own loads
+// and stores hold up.
+__device__ __forceinline__ PersistItem load_item(const PersistItem* p) {
+    using CI = const __attribute__((address_space(4))) int32_t;
+    CI* q = (CI*)(p);
+    PersistItem r;
+    r.g = q[0];
+    r.phase = q[1];
+    r.begin = q[2];
+    r.end = q[3];
+    r.need = q[4];
+    r.slab = q[5];
+    r.pad1 = 0;
+    r.pad2 = 0;
+    return r;
+}
+
 __device__ __forceinline__ int load_agent(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One forward row: Y[row, slab] = sum_k val_k X[col_k, slab] in stored
-// order, one fp32 FMA per nonzero, for feature slab `slab` of G * 4
+// One forward row (stored entries [beg, end)): Y[row, slab] = sum_k val_k
+// X[col_k, slab] in stored order, one fp32 FMA per nonzero, for feature slab `slab` of G * 4
 // features (lanes past F load column 0 and store nothing); with dY set also dY[row, :] = Y[row, :] -
 // T[row, :].  The (col, val) pairs of the next U nonzeros load while the
 // current U gathers are in flight; gathers past the row's end repeat its
 // last entry and are not added.
 template <int G, int U>
-__device__ __forceinline__ void forward_row(const int32_t* __restrict__ row_ptr,
-                                            const int32_t* __restrict__ col,
+__device__ __forceinline__ void forward_row(const int32_t* __restrict__ col,
                                             const float* __restrict__ val,
                                             const float* __restrict__ X, float* __restrict__ Y,
                                             const float* __restrict__ T, float* __restrict__ dY,
-                                            int32_t F, int slab, int row, int li) {
-    const int beg = row_ptr[row], end = row_ptr[row + 1];
+                                            int32_t F, int slab, int row, int li, int beg,
+                                            int end) {
     {
         const int f = (slab * G + li) * 4;
         const bool active = f < F;
@@ -186,16 +204,18 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
         const int first = __builtin_amdgcn_readlane(raw, 0);
         return (first & 63) == 0 ? first >> 6 : -1;
     };
-    // The next item is claimed as this one starts and read after its work,
-    // so the dequeue's round trip overlaps the work (a wave's items stay in
-    // queue order, so the drain argument of the queues holds).
+    // The next item is claimed as this one starts (a wave's items stay in
+    // queue order, so the drain argument of the queues holds); the compiler
+    // reads a folded atomic's result right away, so the wave waits for the
+    // claim there, together with the first row's bounds loaded just before,
+    // and the next descriptor's load then runs under this item's work.
     const int it = index_of(claim());
     if (it < 0) {
         __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
     if (it >= qn) return;
-    PersistItem item = a.items[q0 + it];
+    PersistItem item = load_item(a.items + q0 + it);
     // Completions are published per run of items of one (graph, phase): the
     // stores of the run drain once, then the run's count is added.  A wave
     // publishes everything pending before it waits, so it never waits on an
@@ -222,7 +242,22 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
         item.end = __builtin_amdgcn_readfirstlane(item.end);
         item.need = __builtin_amdgcn_readfirstlane(item.need);
         item.slab = __builtin_amdgcn_readfirstlane(item.slab);
-        const int next_claim = claim();
+        // the bounds of this lane group's first row load ahead of the claim,
+        // whose result the wave waits for at once: the two round trips overlap
+        const bool fwd = item.phase < t;
+        int b0 = 0, e0 = 0;
+        {
+            const int r = item.begin + (fwd ? grpf : grp);
+            const int32_t* rp = fwd ? a.row_ptr : a.t_row_ptr;
+            if (r < item.end) {
+                b0 = rp[r];
+                e0 = rp[r + 1];
+            }
+        }
+        // the next item: claimed now, its descriptor loaded under this one
+        const int next = index_of(claim());
+        PersistItem nitem{};
+        if (next >= 0 && next < qn) nitem = load_item(a.items + q0 + next);
         if (item.g != pend_g || item.phase != pend_phase) publish();
         if (item.need > 0 && !(item.g == ok_g && item.need <= ok_need)) {
             publish();
@@ -232,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
             ok_g = item.g;
             ok_need = item.need;
         }
-        if (item.phase < t) {
+        if (fwd) {
             // forward step s: X_s = P X_{s-1}
             const int s = item.phase + 1;
             const float* in = s == 1 ? a.X : a.Xw + (size_t)(s - 2) * a.plane;
@@ -240,9 +275,13 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
             const bool grad = s == t && a.dY != nullptr;
             for (int r0 = item.begin; r0 < item.end; r0 += RPWF) {
                 const int r = r0 + grpf;
-                if (r < item.end)
-                    forward_row<GF, UF>(a.row_ptr, a.col, a.val, in, out, grad ? a.T : nullptr,
-                                        grad ? a.dY : nullptr, F, item.slab, r, lif);
+                if (r < item.end) {
+                    const bool first = r0 == item.begin;
+                    const int beg = first ? b0 : a.row_ptr[r];
+                    const int end = first ? e0 : a.row_ptr[r + 1];
+                    forward_row<GF, UF>(a.col, a.val, in, out, grad ? a.T : nullptr,
+                                        grad ? a.dY : nullptr, F, item.slab, r, lif, beg, end);
+                }
             }
         } else if (BWD && item.phase < 2 * t) {
             // adjoint step s: G_{s-1} = P^T G_s, D_s = D_{s+1} + <G_s[i], X_{s-1}[j]>;
@@ -254,6 +293,9 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
             for (int r0 = item.begin; r0 < item.end; r0 += RPW) {
                 const int r = r0 + grp;
                 if (r >= item.end) continue;
+                const bool first = r0 == item.begin;
+                const int beg = first ? b0 : a.t_row_ptr[r];
+                const int end = first ? e0 : a.t_row_ptr[r + 1];
                 if (a.want_dp) {
                     const int64_t dp = a.dpad[item.g];
                     const float* ds = s == t ? nullptr : a.D + (size_t)(s - 1) * a.dstride + dp;
@@ -261,29 +303,28 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
                         float* dd = a.D + (size_t)(s - 2) * a.dstride + dp;
                         adjoint_edge_row<4, G, U, NC, true>(a.t_row_ptr, a.t_col, a.t_val, Gs,
                                                             Xprev, Gp, ds, dd, s < t ? 1 : 0, F,
-                                                            0, F, r, li);
+                                                            0, F, r, li, nullptr, beg, end);
                     } else {
                         adjoint_edge_row<4, G, U, NC, true, true>(
                             a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, a.dP,
-                            s < t ? 1 : 0, F, 0, F, r, li, a.t_src);
+                            s < t ? 1 : 0, F, 0, F, r, li, a.t_src, beg, end);
                     }
                 } else {
                     adjoint_edge_row<4, G, U, NC, false>(a.t_row_ptr, a.t_col, a.t_val, Gs,
                                                          nullptr, Gp, nullptr, nullptr, 0, F, 0,
-                                                         F, r, li);
+                                                         F, r, li, nullptr, beg, end);
                 }
             }
         }
         pend_g = item.g;
         pend_phase = item.phase;
         ++pend_n;
-        const int next = index_of(next_claim);
         if (next < 0) {
             __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
         }
         if (next >= qn) break;
-        item = a.items[q0 + next];
+        item = nitem;
     }
     publish();
 }
