@@ -65,29 +65,6 @@ SpmmShape wide_default_shape() {
     return g;
 }
 
-// Launch shape for F features: the one tuned or set for this operator, when
-// it fits the buffers; else, for wide forward operators whose rows are
-// sorted, the row-grouped shape (its grouped form is built on the first
-// such call when `build` is set, and the shape is kept for F); else the
-// default.  Caller holds the handle's mutex.
-SpmmShape choose_shape(scl_sparse_matrix* h, DeviceCSR& d, bool transposed, int64_t F,
-                       const void* X, const void* Y, bool build = true) {
-    auto& m = transposed ? d.t_shapes : d.shapes;
-    auto it = m.find(F);
-    if (it != m.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
-    if (!transposed && it == m.end() && d.n >= kGroupedMinRows && F >= 128) {
-        const SpmmShape g = wide_default_shape();
-        const bool ready = d.grouped[0].R == 4 || (build && d.pins == 0 && ensure_grouped(h, d, 4));
-        if (ready && spmm_shape_valid(g, F, nullptr, nullptr)) {
-            if (build) m[F] = g;
-            if (spmm_shape_valid(g, F, X, Y)) return g;
-        } else if (build) {
-            m[F] = spmm_default_shape(F, nullptr, nullptr);  // not groupable: do not retry
-        }
-    }
-    return spmm_default_shape(F, X, Y);
-}
-
 // Whether the iterates between the first and the last of t steps are kept
 // slab-planar: when F floats per row leave rows off the 128-byte lines and
 // the shape reads planar operands, a full slab's row is then line-aligned
@@ -189,6 +166,30 @@ void transition_impl(scl_sparse_matrix* A, int symmetric, double* degrees_out,
 }
 
 }  // namespace
+
+// Launch shape for F features: the one tuned or set for this operator, when
+// it fits the buffers; else, for wide forward operators whose rows are
+// sorted, the row-grouped shape (its grouped form is built on the first
+// such call when `build` is set, and the shape is kept for F); else the
+// default.  Caller holds the handle's mutex.
+SpmmShape scl_amd::choose_shape(scl_sparse_matrix* h, DeviceCSR& d, bool transposed,
+                                int64_t F, const void* X, const void* Y, bool build) {
+    auto& m = transposed ? d.t_shapes : d.shapes;
+    auto it = m.find(F);
+    if (it != m.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
+    if (!transposed && it == m.end() && d.n >= kGroupedMinRows && F >= 128) {
+        const SpmmShape g = wide_default_shape();
+        const bool ready = d.grouped[0].R == 4 || (build && d.pins == 0 && ensure_grouped(h, d, 4));
+        if (ready && spmm_shape_valid(g, F, nullptr, nullptr)) {
+            if (build) m[F] = g;
+            if (spmm_shape_valid(g, F, X, Y)) return g;
+        } else if (build) {
+            m[F] = spmm_default_shape(F, nullptr, nullptr);  // not groupable: do not retry
+        }
+    }
+    return spmm_default_shape(F, X, Y);
+}
+
 
 extern "C" {
 

@@ -206,6 +206,14 @@ constexpr int64_t kScheduleMinRows = 16384;
 template <typename I>
 std::vector<int32_t> locality_schedule(int64_t n, const I* ptr, const I* idx);
 
+// Launch shape of the forward (or transposed) SpMM for F features: the one
+// tuned or set for this operator when it fits X and Y; else, for wide
+// forward operators with sorted rows, the row-grouped shape (its grouped
+// form built on the first such call when `build` is set); else the default.
+// The caller holds the handle's mutex (diffusion.cpp).
+SpmmShape choose_shape(scl_sparse_matrix* h, DeviceCSR& d, bool transposed, int64_t F,
+                       const void* X, const void* Y, bool build = true);
+
 // Rows of a square pattern in groups of R for the row-grouped SpMM form:
 // walks `order`; each row not yet placed opens a group, which is filled
 // with the unplaced neighbour of its rows sharing the most columns with the

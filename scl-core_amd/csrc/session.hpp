@@ -69,12 +69,11 @@ struct Session {
         s = library_stream(dev);
     }
 
-    // The shape tuned or set for F features when it fits X and Y, else the
-    // default.
+    // The shape tuned or set for F features when it fits X and Y; else, for
+    // wide operators, the row-grouped shape the drop-in diffusion runs (its
+    // grouped form built on first use); else the default.
     SpmmShape shape(int64_t F, const float* X, const float* Y) const {
-        const auto it = d->shapes.find(F);
-        if (it != d->shapes.end() && spmm_shape_valid(it->second, F, X, Y)) return it->second;
-        return spmm_default_shape(F, X, Y);
+        return choose_shape(h, *d, false, F, X, Y);
     }
     // y = P x.  With the default single-vector shape, square operators with
     // a locality schedule run their renumbered copy (built on the first such
