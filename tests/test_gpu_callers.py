@@ -44,9 +44,10 @@ def test_impute_diffusion_known_answer(gpu):
     assert np.array_equal(got, [[1.25, 2, 1.25], [1.75, 1.5, 1.25], [1, 2.5, 2.5]])
 
 
-@pytest.mark.parametrize("csc", [False, True])
-def test_impute_diffusion_matches_restatement(gpu, csc):
-    n, G = 20_000, 96
+@pytest.mark.parametrize("csc,G", [(False, 96), (True, 96), (False, 160)])
+def test_impute_diffusion_matches_restatement(gpu, csc, G):
+    # 160 genes on 20,000 cells: a wide operator, so the row-grouped form runs
+    n = 20_000
     g = synth.uniform_knn(n, 12, seed=21).normalized()
     P = CsrMatrix.from_arrays(g.indptr, g.indices, g.data, (n, n))
     ip, ix, v = expression(n, G, 0.1, seed=22, csc=csc)
