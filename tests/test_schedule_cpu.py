@@ -132,3 +132,122 @@ def test_neighbour_groups_and_locality_order(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "ok" in out.stdout
+
+
+MODEL = r"""
+#include "internal.hpp"
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+using namespace scl_amd;
+
+static std::vector<int64_t> rd(const char* f) {
+    FILE* p = std::fopen(f, "rb");
+    std::fseek(p, 0, SEEK_END);
+    const long sz = std::ftell(p);
+    std::fseek(p, 0, SEEK_SET);
+    std::vector<int64_t> v((size_t)sz / 8);
+    if (std::fread(v.data(), 8, v.size(), p) != v.size()) v.clear();
+    std::fclose(p);
+    return v;
+}
+
+// LRU over slab rows: a doubly linked list with an in-cache flag per row
+struct Lru {
+    int64_t cap, head = -1, tail = -1, size = 0;
+    std::vector<int64_t> prev, next;
+    std::vector<char> in;
+    Lru(int64_t n, int64_t c) : cap(c), prev(n, -1), next(n, -1), in(n, 0) {}
+    void unlink(int64_t x) {
+        if (prev[x] >= 0) next[prev[x]] = next[x]; else head = next[x];
+        if (next[x] >= 0) prev[next[x]] = prev[x]; else tail = prev[x];
+    }
+    void front(int64_t x) {
+        prev[x] = -1; next[x] = head;
+        if (head >= 0) prev[head] = x;
+        head = x;
+        if (tail < 0) tail = x;
+    }
+    bool access(int64_t x) {
+        if (in[x]) { unlink(x); front(x); return true; }
+        if (size == cap) { const int64_t t = tail; unlink(t); in[t] = 0; --size; }
+        front(x); in[x] = 1; ++size;
+        return false;
+    }
+};
+
+// union entries and misses of groups of R consecutive rows of `rows`
+static void model(const std::vector<int64_t>& ip, const std::vector<int64_t>& ix,
+                  const std::vector<int32_t>& rows, int R, int64_t cap, int W,
+                  int64_t& acc, int64_t& miss) {
+    const int64_t n = (int64_t)ip.size() - 1, G = (n + R - 1) / R;
+    std::vector<int64_t> uptr(G + 1, 0), ucol, mark(n, -1);
+    for (int64_t g = 0; g < G; ++g) {
+        std::vector<int64_t> cols;
+        for (int64_t k = g * R; k < std::min<int64_t>(n, (g + 1) * R); ++k)
+            for (int64_t e = ip[rows[k]]; e < ip[rows[k] + 1]; ++e)
+                if (mark[ix[e]] != g) { mark[ix[e]] = g; cols.push_back(ix[e]); }
+        std::sort(cols.begin(), cols.end());
+        ucol.insert(ucol.end(), cols.begin(), cols.end());
+        uptr[g + 1] = (int64_t)ucol.size();
+    }
+    Lru L(n, cap);
+    acc = miss = 0;
+    std::vector<int64_t> sg(W, -1), se(W, 0);
+    int64_t ng = 0, active = 0;
+    for (int s = 0; s < W && ng < G; ++s, ++ng, ++active) { sg[s] = ng; se[s] = uptr[ng]; }
+    while (active > 0)
+        for (int s = 0; s < W; ++s) {
+            if (sg[s] < 0) continue;
+            const int64_t g = sg[s];
+            for (int u = 0; u < 16 && se[s] < uptr[g + 1]; ++u) {
+                ++acc;
+                if (!L.access(ucol[se[s]++])) ++miss;
+            }
+            if (se[s] >= uptr[g + 1]) {
+                if (ng < G) { sg[s] = ng; se[s] = uptr[ng]; ++ng; }
+                else { sg[s] = -1; --active; }
+            }
+        }
+}
+
+int main(int argc, char** argv) {
+    const std::vector<int64_t> ip = rd(argv[1]), ix = rd(argv[2]);
+    const int64_t n = (int64_t)ip.size() - 1;
+    const int64_t cap = 4096 * n / 200000;
+    const int W = (int)(512 * n / 200000);
+    const std::vector<int32_t> order = locality_schedule<int64_t>(n, ip.data(), ix.data());
+    const std::vector<int32_t> grp = neighbour_groups(n, ip.data(), ix.data(), order, 4);
+    int64_t a_acc, a_miss, b_acc, b_miss;
+    model(ip, ix, order, 4, cap, W, a_acc, a_miss);
+    model(ip, ix, grp, 4, cap, W, b_acc, b_miss);
+    const double a = a_acc + 1.5 * a_miss, b = b_acc + 1.5 * b_miss;
+    std::printf("consecutive: union %lld misses %lld cost %.0f\n", (long long)a_acc,
+                (long long)a_miss, a);
+    std::printf("groups:      union %lld misses %lld cost %.0f\n", (long long)b_acc,
+                (long long)b_miss, b);
+    if (!(b_acc < a_acc && b <= a)) { std::printf("FAIL\n"); return 1; }
+    std::printf("ok\n");
+    return 0;
+}
+"""
+
+
+def test_l2_model_of_grouped_order(tmp_path):
+    import numpy as np
+    from scl_amd import synth
+    g = synth.knn_graph(20_000, 30, seed=2)
+    ip_f, ix_f = tmp_path / "indptr.bin", tmp_path / "indices.bin"
+    g.indptr.astype(np.int64).tofile(ip_f)
+    g.indices.astype(np.int64).tofile(ix_f)
+    src = tmp_path / "model.cpp"
+    src.write_text(MODEL)
+    exe = tmp_path / "model"
+    cmd = ["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           f"-I{CSRC}", f"-I{ROOT / 'include'}", str(src), str(CSRC / "schedule.cpp"),
+           "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe), str(ip_f), str(ix_f)], capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0 and "ok" in out.stdout, out.stdout + out.stderr
