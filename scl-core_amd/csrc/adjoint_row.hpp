@@ -13,6 +13,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "kernels.hpp"
 
 namespace scl_amd {
 namespace dev {
@@ -113,7 +114,12 @@ struct Scatter {
 // fetched while the current gathers are in flight.  dsrc may equal ddst
 // (in-place accumulation).  beg_in / end_in, when beg_in >= 0, are the
 // row's bounds t_row_ptr[row], t_row_ptr[row + 1] loaded by the caller.
-template <int VEC, int G, int U, int NC, bool EDGE, bool SCAT = false>
+// PADDED: a chunk's indices load without clamping to the row (t_col and
+// t_val are padded past their end), so its gathers past the row's end read
+// other rows of Gs; a kernel whose launch writes rows of Gs that it may
+// also read (the persistent batch) clamps instead, so that no wave reads a
+// row before it is written.
+template <int VEC, int G, int U, int NC, bool EDGE, bool SCAT = false, bool PADDED = true>
 __device__ __forceinline__ void adjoint_edge_row(
     const int32_t* __restrict__ t_row_ptr, const int32_t* __restrict__ t_col,
     const float* __restrict__ t_val, const float* __restrict__ Gs,
@@ -153,14 +159,18 @@ __device__ __forceinline__ void adjoint_edge_row(
     const int my_e = kScatter ? SC::entry(li) : 0;
     const bool writer = kScatter ? SC::writer(li) : li == 0;
 
+    // PADDED: t_col and t_val hold kTransposePad (>= U) entries past the
+    // last row, so a chunk's indices and weights load without clamping
+    // (entries past the row's end name valid rows and are not added)
+    static_assert(U <= kTransposePad, "the transposed arrays' padding covers a chunk");
+    auto at = [&](int kk) { return PADDED ? kk : min(kk, end - 1); };
     int i[U];
     float w[U];
     if (beg < end) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int kk = min(beg + u, end - 1);
-            i[u] = t_col[kk];
-            w[u] = t_val[kk];
+            i[u] = t_col[at(beg + u)];
+            w[u] = t_val[at(beg + u)];
         }
     }
     for (int k = beg; k < end; k += U) {
@@ -185,14 +195,13 @@ __device__ __forceinline__ void adjoint_edge_row(
         float wc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) wc[u] = w[u];
-        // next chunk's indices (clamped into this row; unused past its end)
+        // next chunk's indices (unused past the row's end)
         const int kn = k + U;
         if (kn < end) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int kk = min(kn + u, end - 1);
-                i[u] = t_col[kk];
-                w[u] = t_val[kk];
+                i[u] = t_col[at(kn + u)];
+                w[u] = t_val[at(kn + u)];
             }
         }
 #pragma unroll

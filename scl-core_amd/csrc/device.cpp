@@ -450,8 +450,10 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
             tsrc[dst] = (int32_t)k;
         }
     d.t_row_ptr = dev_alloc<int32_t>((size_t)q + 1, "transpose row_ptr");
-    d.t_col = dev_alloc<int32_t>((size_t)nnz, "transpose col");
-    d.t_val = dev_alloc<float>((size_t)nnz, "transpose val");
+    d.t_col = dev_alloc<int32_t>((size_t)nnz + kTransposePad, "transpose col");
+    d.t_val = dev_alloc<float>((size_t)nnz + kTransposePad, "transpose val");
+    hip_check(hipMemset(d.t_col + nnz, 0, kTransposePad * sizeof(int32_t)), "transpose pad");
+    hip_check(hipMemset(d.t_val + nnz, 0, kTransposePad * sizeof(float)), "transpose pad");
     d.t_src = dev_alloc<int32_t>((size_t)nnz, "transpose src");
     hip_check(hipMemcpy(d.t_row_ptr, trp.data(), trp.size() * 4, hipMemcpyHostToDevice),
               "upload transpose");

@@ -55,7 +55,8 @@ size_t backward_bytes(int64_t n, int64_t F, int64_t nnz, int64_t every, bool wan
     const size_t plane = round_up((size_t)n * (size_t)F * sizeof(float));
     size_t b = plane * (size_t)(2 + (every - 1));
     if (want_dp) b += round_up((size_t)nnz * sizeof(float));
-    if (values) b += round_up((size_t)nnz * sizeof(float));  // transposed values
+    // transposed values, with the pad the adjoint kernels read past the end
+    if (values) b += round_up(((size_t)nnz + kTransposePad) * sizeof(float));
     return b;
 }
 

@@ -24,6 +24,11 @@ struct MergeScratch {
 // rows' column indices in increasing order, each with the mask of the rows
 // holding it.  gval lists the values entry by entry (row slots in order), so
 // every row still meets its own nonzeros in stored (sorted) order.
+// Entries past the last stored one in the transposed pattern's device
+// arrays (t_col: row 0, t_val: 0): the adjoint kernels load a chunk of
+// indices and weights without clamping it to the row.
+constexpr int kTransposePad = 8;
+
 struct GroupedCsr {
     int R = 0;                    // rows per group (4 or 8); 0: not built
     int64_t n_groups = 0;

@@ -301,18 +301,20 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
                     const float* ds = s == t ? nullptr : a.D + (size_t)(s - 1) * a.dstride + dp;
                     if (s > 1) {
                         float* dd = a.D + (size_t)(s - 2) * a.dstride + dp;
-                        adjoint_edge_row<4, G, U, NC, true>(a.t_row_ptr, a.t_col, a.t_val, Gs,
-                                                            Xprev, Gp, ds, dd, s < t ? 1 : 0, F,
-                                                            0, F, r, li, nullptr, beg, end);
+                        // clamped index loads: rows of other graphs are being
+                        // written in this launch (see adjoint_edge_row)
+                        adjoint_edge_row<4, G, U, NC, true, false, false>(
+                            a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, dd,
+                            s < t ? 1 : 0, F, 0, F, r, li, nullptr, beg, end);
                     } else {
-                        adjoint_edge_row<4, G, U, NC, true, true>(
+                        adjoint_edge_row<4, G, U, NC, true, true, false>(
                             a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, a.dP,
                             s < t ? 1 : 0, F, 0, F, r, li, a.t_src, beg, end);
                     }
                 } else {
-                    adjoint_edge_row<4, G, U, NC, false>(a.t_row_ptr, a.t_col, a.t_val, Gs,
-                                                         nullptr, Gp, nullptr, nullptr, 0, F, 0,
-                                                         F, r, li, nullptr, beg, end);
+                    adjoint_edge_row<4, G, U, NC, false, false, false>(
+                        a.t_row_ptr, a.t_col, a.t_val, Gs, nullptr, Gp, nullptr, nullptr, 0, F,
+                        0, F, r, li, nullptr, beg, end);
                 }
             }
         }
