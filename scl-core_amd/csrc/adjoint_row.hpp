@@ -25,6 +25,18 @@ __device__ __forceinline__ float dotv(f4 a, f4 b) { return dot4(a, b); }
 __device__ __forceinline__ float dotv(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
 __device__ __forceinline__ float dotv(float a, float b) { return a * b; }
 
+// s + <a, b> as an explicit FMA chain over the components: the rounding of
+// a lane's partial dot product is then fixed by the source, whatever the
+// compiler would contract in a * b + c elsewhere, so kernels that share
+// this row code with other chunk lengths produce the same bits.
+__device__ __forceinline__ float fma_dot(f4 a, f4 b, float s) {
+    return fmaf(a.w, b.w, fmaf(a.z, b.z, fmaf(a.y, b.y, fmaf(a.x, b.x, s))));
+}
+__device__ __forceinline__ float fma_dot(f2 a, f2 b, float s) {
+    return fmaf(a.y, b.y, fmaf(a.x, b.x, s));
+}
+__device__ __forceinline__ float fma_dot(float a, float b, float s) { return fmaf(a, b, s); }
+
 // Cross-lane sums.  Inside a 16-lane row, stage b pairs lane li with
 // li ^ (2^(b+1) - 1) through DPP (quad_perm, row_half_mirror, row_mirror),
 // which costs no LDS traffic; lanes 16 and 32 apart are paired with
@@ -215,9 +227,9 @@ __device__ __forceinline__ void adjoint_edge_row(
             float d[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                float sdot = dotv(g[u][0], xj[0]);
+                float sdot = 0.0f;
 #pragma unroll
-                for (int c = 1; c < NC; ++c) sdot += dotv(g[u][c], xj[c]);
+                for (int c = 0; c < NC; ++c) sdot = fma_dot(g[u][c], xj[c], sdot);
                 d[u] = sdot;
             }
             if constexpr (kScatter) {

@@ -160,7 +160,7 @@ __device__ __forceinline__ bool wait_done(const int32_t* done_g, int32_t* abort_
 template <int G, int NC, bool BWD>
 __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
     constexpr int RPW = 64 / G;
-    constexpr int U = 8;    // adjoint chunk (fixes dP's reduction order)
+    constexpr int U = 8;    // adjoint chunk (the per-step kernel's default)
     // forward gathers in flight per lane: with the adjoint's registers
     // (4 waves per SIMD either way) 12; alone, 8 keeps 5 waves per SIMD
     constexpr int UF = BWD ? 12 : 8;
