@@ -299,18 +299,16 @@ __global__ __launch_bounds__(kBlock) void k_persist(const PersistArgs a) {
                 if (a.want_dp) {
                     const int64_t dp = a.dpad[item.g];
                     const float* ds = s == t ? nullptr : a.D + (size_t)(s - 1) * a.dstride + dp;
-                    if (s > 1) {
-                        float* dd = a.D + (size_t)(s - 2) * a.dstride + dp;
-                        // clamped index loads: rows of other graphs are being
-                        // written in this launch (see adjoint_edge_row)
-                        adjoint_edge_row<4, G, U, NC, true, false, false>(
-                            a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, dd,
-                            s < t ? 1 : 0, F, 0, F, r, li, nullptr, beg, end);
-                    } else {
-                        adjoint_edge_row<4, G, U, NC, true, true, false>(
-                            a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, a.dP,
-                            s < t ? 1 : 0, F, 0, F, r, li, a.t_src, beg, end);
-                    }
+                    // step s > 1 writes D_{s-1} (slot s - 2); step 1 the totals
+                    // in transposed order (slot t - 1), which a gather after
+                    // this kernel puts in CSR order: stores scattered from
+                    // inside the step cost more than that pass
+                    float* dd = a.D + (size_t)(s > 1 ? s - 2 : t - 1) * a.dstride + dp;
+                    // clamped index loads: rows of other graphs are being
+                    // written in this launch (see adjoint_edge_row)
+                    adjoint_edge_row<4, G, U, NC, true, false, false>(
+                        a.t_row_ptr, a.t_col, a.t_val, Gs, Xprev, Gp, ds, dd,
+                        s < t ? 1 : 0, F, 0, F, r, li, nullptr, beg, end);
                 } else {
                     adjoint_edge_row<4, G, U, NC, false, false, false>(
                         a.t_row_ptr, a.t_col, a.t_val, Gs, nullptr, Gp, nullptr, nullptr, 0, F,

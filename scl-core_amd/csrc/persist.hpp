@@ -65,17 +65,16 @@ struct PersistArgs {
     const int32_t* t_row_ptr;
     const int32_t* t_col;
     const float* t_val;
-    const int32_t* t_src;
-    // caller buffers (n x F row-major; dP in CSR order)
+    // caller buffers (n x F row-major)
     const float* X;
     float* Y;
     const float* T;
     float* dY;
     float* dX;
-    float* dP;
-    // plan buffers: X_1 .. X_{t-1}, G_1 .. G_{t-1} (planes of n x F) and the
-    // dP partials D_2 .. D_t (dstride floats each, graph g at dpad[g] plus
-    // its transposed positions; step 1 writes dP itself)
+    // plan buffers: X_1 .. X_{t-1}, G_1 .. G_{t-1} (planes of n x F) and t
+    // slots of dstride floats, graph g at dpad[g] plus its transposed
+    // positions: the dP partials D_1 .. D_{t-1} (slot s - 1 holds D_s) and,
+    // in slot t - 1, the totals step 1 leaves, in transposed order
     float* Xw;
     float* Gw;
     float* D;

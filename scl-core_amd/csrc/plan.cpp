@@ -176,6 +176,8 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
         for (int l = 0; l < kPersistLanes; ++l)
             if ((int64_t)(Q.q_off[l + 1] - Q.q_off[l]) + 2 > std::numeric_limits<int32_t>::max() / 64)
                 fail(SCL_ERROR_INVALID_ARGUMENT, "Too many work items in one queue");
+        if (Q.dstride > std::numeric_limits<int32_t>::max())
+            fail(SCL_ERROR_INVALID_ARGUMENT, "Too many stored entries for one batched plan");
         const int dev = current_device();
         DeviceCSR* dp = nullptr;
         {
@@ -200,21 +202,31 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
             a.t_row_ptr = dp->t_row_ptr;
             a.t_col = dp->t_col;
             a.t_val = dp->t_val;
-            a.t_src = dp->t_src;
             a.X = X_dev;
             a.Y = Y_dev;
             a.T = T_dev;
             a.dY = backward ? dY_dev : nullptr;
             a.dX = dX_dev;
-            a.dP = want_dp ? dP_dev : nullptr;
             a.Xw = t >= 2 ? plan_alloc<float>(plan, plane * (size_t)(t - 1), "plan iterates") : nullptr;
             a.Gw = backward && t >= 2
                        ? plan_alloc<float>(plan, plane * (size_t)(t - 1), "plan adjoint iterates")
                        : nullptr;
-            // partials D_2 .. D_t (step 1 writes dP itself)
-            a.D = want_dp && t >= 2
-                      ? plan_alloc<float>(plan, (size_t)Q.dstride * (size_t)(t - 1), "plan dP partials")
-                      : nullptr;
+            // partials D_1 .. D_{t-1} and the totals of step 1 (transposed
+            // order, padded per graph), and where each CSR entry's total is
+            a.D = want_dp ? plan_alloc<float>(plan, (size_t)Q.dstride * (size_t)t, "plan dP partials")
+                          : nullptr;
+            int32_t* d_dp_map = nullptr;
+            if (want_dp) {
+                std::vector<int32_t> t_src((size_t)h->nnz), map((size_t)h->nnz);
+                hip_check(hipMemcpy(t_src.data(), dp->t_src, t_src.size() * 4,
+                                    hipMemcpyDeviceToHost), "plan download");
+                for (int64_t g = 0; g < G; ++g)
+                    for (int64_t q = nz[g]; q < nz[g + 1]; ++q)
+                        map[(size_t)t_src[(size_t)q]] = (int32_t)(Q.dpad[g] + q);
+                d_dp_map = plan_alloc<int32_t>(plan, map.size(), "plan dP map");
+                hip_check(hipMemcpy(d_dp_map, map.data(), map.size() * 4, hipMemcpyHostToDevice),
+                          "plan upload");
+            }
             int32_t* d_dpad = plan_alloc<int32_t>(plan, (size_t)G, "plan offsets");
             PersistItem* d_items = plan_alloc<PersistItem>(plan, Q.items.size(), "plan items");
             int32_t* d_qoff = plan_alloc<int32_t>(plan, Q.q_off.size(), "plan queues");
@@ -247,6 +259,9 @@ SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
             hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
             launch_zero_i32(plan->ctl, n_ctl, s);
             launch_persist(a, grid, s);
+            if (want_dp)
+                launch_gather_f32(d_dp_map, a.D + (size_t)(t - 1) * (size_t)Q.dstride, dP_dev,
+                                  h->nnz, s);
             hipGraph_t g = nullptr;
             hip_check(hipStreamEndCapture(s, &g), "end capture");
             plan->graph = g;
