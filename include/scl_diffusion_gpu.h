@@ -348,10 +348,10 @@ SCL_EXPORT scl_error_t scl_gpu_plan_destroy(scl_gpu_plan_t* plan);
 
 /* Batched plan (no reference counterpart): the same sequence for every graph
  * of a block-diagonal stack (scl_sparse_block_diag) in ONE persistent kernel
- * launch, recorded into a hipGraph: t forward steps, dY = Y - T, t fused
- * adjoint + edge-gradient steps and the permutation of dP to CSR order, each
- * graph's steps ordered by the graph's own progress rather than by launch
- * boundaries.  block_rows[g] are the rows of block g (they add up to the
+ * launch, recorded into a hipGraph: t forward steps, dY = Y - T and t fused
+ * adjoint + edge-gradient steps, each graph's steps ordered by the graph's
+ * own progress rather than by launch boundaries; a gather recorded after the
+ * kernel puts dP in CSR order.  block_rows[g] are the rows of block g (they add up to the
  * operator's rows, and no stored entry leaves its block: refused otherwise).
  * Each graph runs on one XCD (bin-packed by rows + nonzeros); rows_per_item
  * rows form one work item (0: 16) and the queue of an XCD walks its graphs
@@ -361,7 +361,7 @@ SCL_EXPORT scl_error_t scl_gpu_plan_destroy(scl_gpu_plan_t* plan);
  * are bit-identical to scl_gpu_plan_diffusion with every = 1 and the default
  * adjoint group (vec 4, one lane group per row covering F).  The plan
  * allocates its own intermediates ((t - 1) planes of iterates, (t - 1)
- * adjoint planes, t dP partials).  Launch with scl_gpu_plan_launch. */
+ * adjoint planes, t dP partials and the dP gather map).  Launch with scl_gpu_plan_launch. */
 SCL_EXPORT scl_error_t scl_gpu_plan_diffusion_batched(
     scl_sparse_t transition, const scl_index_t* block_rows, scl_size_t n_blocks,
     const scl_real_t* X_dev, scl_real_t* Y_dev, const scl_real_t* T_dev, scl_real_t* dY_dev,
