@@ -299,7 +299,8 @@ void DeviceCSR::release_transpose() {
     dev_free(t_val);
     dev_free(t_src);
     dev_free(t_sched);
-    t_row_ptr = t_col = t_src = t_sched = nullptr;
+    dev_free(t_dst);
+    t_row_ptr = t_col = t_src = t_dst = t_sched = nullptr;
     t_val = nullptr;
     t_generation = ~uint64_t(0);
 }
@@ -437,7 +438,8 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
     const int64_t p = h->primary(), q = h->secondary(), nnz = h->nnz;
     // counting sort by column; visiting source rows in order keeps each
     // transposed row sorted by source row
-    std::vector<int32_t> trp((size_t)q + 1, 0), tcol((size_t)nnz), tsrc((size_t)nnz);
+    std::vector<int32_t> trp((size_t)q + 1, 0), tcol((size_t)nnz), tsrc((size_t)nnz),
+        tdst((size_t)nnz);
     std::vector<float> tval((size_t)nnz);
     for (int64_t k = 0; k < nnz; ++k) trp[h->indices[k] + 1]++;
     for (int64_t j = 0; j < q; ++j) trp[j + 1] += trp[j];
@@ -448,6 +450,7 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
             tcol[dst] = (int32_t)i;
             tval[dst] = h->data[k];
             tsrc[dst] = (int32_t)k;
+            tdst[(size_t)k] = dst;
         }
     d.t_row_ptr = dev_alloc<int32_t>((size_t)q + 1, "transpose row_ptr");
     d.t_col = dev_alloc<int32_t>((size_t)nnz + kTransposePad, "transpose col");
@@ -455,6 +458,7 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
     hip_check(hipMemset(d.t_col + nnz, 0, kTransposePad * sizeof(int32_t)), "transpose pad");
     hip_check(hipMemset(d.t_val + nnz, 0, kTransposePad * sizeof(float)), "transpose pad");
     d.t_src = dev_alloc<int32_t>((size_t)nnz, "transpose src");
+    d.t_dst = dev_alloc<int32_t>((size_t)nnz, "transpose dst");
     hip_check(hipMemcpy(d.t_row_ptr, trp.data(), trp.size() * 4, hipMemcpyHostToDevice),
               "upload transpose");
     if (nnz > 0) {
@@ -463,6 +467,8 @@ void ensure_transpose(scl_sparse_matrix* h, DeviceCSR& d, hipStream_t s) {
         hip_check(hipMemcpy(d.t_val, tval.data(), tval.size() * 4, hipMemcpyHostToDevice),
                   "upload transpose");
         hip_check(hipMemcpy(d.t_src, tsrc.data(), tsrc.size() * 4, hipMemcpyHostToDevice),
+                  "upload transpose");
+        hip_check(hipMemcpy(d.t_dst, tdst.data(), tdst.size() * 4, hipMemcpyHostToDevice),
                   "upload transpose");
     }
     if (q >= kScheduleMinRows) d.t_sched = upload_schedule(locality_schedule(q, trp.data(), tcol.data()));

@@ -246,6 +246,7 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_device(
         SpmmShape sh, adj;
         bool has_adj = false;
         const int32_t* t_src = nullptr;
+        const int32_t* t_dst = nullptr;
         {
             std::lock_guard<std::mutex> lk(transition->mu);
             DeviceCSR& d = device_csr(transition, dev);
@@ -253,6 +254,7 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_device(
             fv = fwd_view(d);
             av = adj_view(d);
             t_src = d.t_src;
+            t_dst = d.t_dst;
             sh = fwd_shape(d, F, X_dev, work_dev);
             const auto ai = d.adj_shapes.find(F);
             if (ai != d.adj_shapes.end()) {
@@ -305,7 +307,8 @@ SCL_EXPORT scl_error_t scl_gpu_diffuse_backward_values_device(
                 G = Gn;
             }
         }
-        if (want_dp) launch_scatter_f32(t_src, dPT, dP_dev, transition->nnz, s);
+        // dP in CSR order: a gather (coalesced stores) through the inverse map
+        if (want_dp) launch_gather_f32(t_dst, dPT, dP_dev, transition->nnz, s);
         SCL_API_RETURN_OK;
     SCL_API_CATCH
 }

@@ -88,6 +88,9 @@ struct DeviceCSR {
     int32_t* t_col = nullptr;
     float* t_val = nullptr;
     int32_t* t_src = nullptr;
+    // t_dst[k] = transposed position of entry k (the inverse of t_src), so
+    // that putting transposed-order values in CSR order is a gather
+    int32_t* t_dst = nullptr;
     int32_t* t_sched = nullptr;
     // number of captured plans that hold raw pointers into this copy: while
     // non-zero, refreshes update the arrays in place and never reallocate
